@@ -1,0 +1,312 @@
+"""MI355X ikd-Tree: Python mirror of hku-mars/ikd-Tree's ``KD_TREE<PointType>``.
+
+The tree lives in device memory and every method runs HIP kernels (gfx950)
+through the C ABI of ``libikd_hip.so`` (declared in ``include/ikd_tree_amd.h``).
+Method names, argument meaning and defaults follow ikd-Tree/ikd_Tree.h:225-248.
+There is no CPU fallback: importing this package without the built library
+raises ``ImportError``.
+
+Import it by path (the directory name is not a Python identifier)::
+
+    import importlib, sys
+    sys.path.insert(0, "/path/to/repo")
+    ikd = importlib.import_module("ikd-tree_amd")
+    tree = ikd.KD_TREE(0.5, 0.6, 0.5)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libikd_hip.so")
+CSRC = os.path.join(_HERE, "csrc")
+
+IKD_OK = 0
+IKD_ERR_ARG = -1
+IKD_ERR_HIP = -2
+IKD_ERR_NOMEM = -3
+IKD_ERR_CAPACITY = -4
+IKD_ERR_UNSUPPORTED = -5
+IKD_ERR_INTERNAL = -6
+
+# every entry point include/ikd_tree_amd.h declares
+EXPORTS = (
+    "ikd_create", "ikd_destroy", "ikd_set_delete_criterion_param", "ikd_set_balance_criterion_param",
+    "ikd_set_downsample_param", "ikd_initialize", "ikd_build", "ikd_nearest_search",
+    "ikd_nearest_search_device", "ikd_nearest_search_counters", "ikd_box_search", "ikd_radius_search",
+    "ikd_add_points", "ikd_add_points_device", "ikd_delete_points", "ikd_delete_point_boxes", "ikd_size",
+    "ikd_validnum", "ikd_tree_range", "ikd_root_alpha", "ikd_flatten", "ikd_synchronize", "ikd_stream",
+    "ikd_pool_stats", "ikd_last_error",
+)
+
+
+class IkdError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"ikd error {code}: {msg}")
+        self.code = code
+
+
+def build_library(jobs: int = 8) -> str:
+    """Compile libikd_hip.so for gfx950 with hipcc (csrc/Makefile)."""
+    subprocess.run(["make", "-C", CSRC, f"-j{jobs}", "-B"], check=True, stdout=subprocess.DEVNULL)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def load_library():
+    """Load the HIP library; raises ImportError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with build_library() or `make -C {CSRC}`")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, i64, i32, f32, f64 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_float, ctypes.c_double
+    sig = {
+        "ikd_create": [f32, f32, f32, ctypes.POINTER(vp)],
+        "ikd_destroy": [vp],
+        "ikd_set_delete_criterion_param": [vp, f32],
+        "ikd_set_balance_criterion_param": [vp, f32],
+        "ikd_set_downsample_param": [vp, f32],
+        "ikd_initialize": [vp, f32, f32, f32],
+        "ikd_build": [vp, vp, vp, i64],
+        "ikd_nearest_search": [vp, vp, i64, i32, f64, vp, vp, vp, vp],
+        "ikd_nearest_search_device": [vp, vp, i64, i32, f64, vp, vp, vp, vp],
+        "ikd_nearest_search_counters": [vp, vp, i64, i32, f64, vp, vp],
+        "ikd_box_search": [vp, vp, i64, vp, vp, vp, i64],
+        "ikd_radius_search": [vp, vp, vp, i64, vp, vp, vp, i64],
+        "ikd_add_points": [vp, vp, vp, i64, i32, vp],
+        "ikd_add_points_device": [vp, vp, vp, i64, i32, vp],
+        "ikd_delete_points": [vp, vp, i64],
+        "ikd_delete_point_boxes": [vp, vp, i64, vp],
+        "ikd_size": [vp, vp],
+        "ikd_validnum": [vp, vp],
+        "ikd_tree_range": [vp, vp],
+        "ikd_root_alpha": [vp, vp, vp],
+        "ikd_flatten": [vp, vp, vp, i64, vp],
+        "ikd_synchronize": [vp],
+        "ikd_stream": [vp, ctypes.POINTER(vp)],
+        "ikd_pool_stats": [vp, vp, vp, vp, vp],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = i32
+    lib.ikd_last_error.argtypes = [vp]
+    lib.ikd_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data_as(ctypes.c_void_p)
+    return ctypes.c_void_p(int(a))  # a raw device address
+
+
+def _pts(points) -> np.ndarray:
+    return np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+
+
+def _tensor_ptr(t):
+    """Device address of a torch tensor (contiguous, on the GPU)."""
+    if not t.is_contiguous():
+        raise ValueError("device tensors must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class KD_TREE:
+    """``KD_TREE<PointType>`` (ikd-Tree/ikd_Tree.h:59-249) on one MI355X."""
+
+    def __init__(self, delete_param: float = 0.5, balance_param: float = 0.6, box_length: float = 0.2):
+        self._lib = load_library()
+        h = ctypes.c_void_p()
+        self._check(self._lib.ikd_create(delete_param, balance_param, box_length, ctypes.byref(h)), handle=None)
+        self._h = h
+
+    # ------------------------------------------------------------ plumbing
+    def _check(self, rc: int, handle="self"):
+        if rc != IKD_OK:
+            h = self._h if handle == "self" else None
+            msg = self._lib.ikd_last_error(h).decode() if h else "create failed"
+            raise IkdError(rc, msg)
+        return rc
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.ikd_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ parameters
+    def Set_delete_criterion_param(self, delete_param: float):
+        self._check(self._lib.ikd_set_delete_criterion_param(self._h, delete_param))
+
+    def Set_balance_criterion_param(self, balance_param: float):
+        self._check(self._lib.ikd_set_balance_criterion_param(self._h, balance_param))
+
+    def set_downsample_param(self, box_length: float):
+        self._check(self._lib.ikd_set_downsample_param(self._h, box_length))
+
+    def InitializeKDTree(self, delete_param: float = 0.5, balance_param: float = 0.7, box_length: float = 0.2):
+        self._check(self._lib.ikd_initialize(self._h, delete_param, balance_param, box_length))
+
+    # ------------------------------------------------------------ build
+    def Build(self, points, ids=None):
+        p = _pts(points)
+        i = None if ids is None else np.ascontiguousarray(ids, np.uint32)
+        self._check(self._lib.ikd_build(self._h, _ptr(p), _ptr(i), p.shape[0]))
+
+    # ------------------------------------------------------------ kNN
+    def knn(self, queries, k: int, max_dist: float = float("inf")):
+        """Batched Nearest_Search: (xyz[nq,k,3], d2[nq,k], ids[nq,k], counts[nq])."""
+        q = _pts(queries)
+        nq = q.shape[0]
+        xyz = np.zeros((nq, k, 3), np.float32)
+        d2 = np.zeros((nq, k), np.float32)
+        ids = np.zeros((nq, k), np.uint32)
+        cnt = np.zeros(nq, np.int32)
+        self._check(self._lib.ikd_nearest_search(self._h, _ptr(q), nq, k, float(max_dist), _ptr(xyz), _ptr(d2),
+                                                 _ptr(ids), _ptr(cnt)))
+        return xyz, d2, ids, cnt
+
+    def Nearest_Search(self, point, k_nearest: int, max_dist: float = float("inf")):
+        """One query (cpp:366-397): (Nearest_Points[n,3], Point_Distance[n]) ascending."""
+        xyz, d2, _, cnt = self.knn(np.asarray(point, np.float32).reshape(1, 3), k_nearest, max_dist)
+        n = int(cnt[0])
+        return xyz[0, :n].copy(), d2[0, :n].copy()
+
+    def knn_counters(self, queries, k: int, max_dist: float = float("inf")):
+        """Per-query nodes visited and child boxes evaluated (instrumentation)."""
+        q = _pts(queries)
+        nq = q.shape[0]
+        v = np.zeros(nq, np.uint32)
+        b = np.zeros(nq, np.uint32)
+        self._check(self._lib.ikd_nearest_search_counters(self._h, _ptr(q), nq, k, float(max_dist), _ptr(v), _ptr(b)))
+        return v, b
+
+    def knn_device(self, q, k: int, max_dist: float, out_xyz, out_d2, out_ids, out_counts):
+        """Stream-ordered kNN on device tensors (torch, contiguous): q[nq,3] float32,
+        out_xyz[nq,k,3], out_d2[nq,k] float32, out_ids[nq,k] int32/uint32, out_counts[nq] int32."""
+        self._check(self._lib.ikd_nearest_search_device(self._h, _tensor_ptr(q), q.shape[0], k, float(max_dist),
+                                                        _tensor_ptr(out_xyz), _tensor_ptr(out_d2),
+                                                        _tensor_ptr(out_ids), _tensor_ptr(out_counts)))
+
+    # ------------------------------------------------------------ range queries
+    def _range(self, fn, args, n):
+        off = np.zeros(n + 1, np.int64)
+        self._check(fn(self._h, *args, n, _ptr(off), None, None, 0))
+        total = int(off[n])
+        xyz = np.zeros((max(total, 1), 3), np.float32)
+        ids = np.zeros(max(total, 1), np.uint32)
+        self._check(fn(self._h, *args, n, _ptr(off), _ptr(xyz), _ptr(ids), total))
+        return off, xyz[:total], ids[:total]
+
+    def box_search_batch(self, boxes):
+        b = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
+        return self._range(self._lib.ikd_box_search, (_ptr(b),), b.shape[0])
+
+    def Box_Search(self, box):
+        """Live points in the half-open box (cpp:399-404)."""
+        _, xyz, _ = self.box_search_batch(np.asarray(box, np.float32).reshape(1, 6))
+        return xyz
+
+    def radius_search_batch(self, centers, radii):
+        c = _pts(centers)
+        r = np.ascontiguousarray(np.broadcast_to(np.asarray(radii, np.float32), (c.shape[0],)))
+        return self._range(self._lib.ikd_radius_search, (_ptr(c), _ptr(r)), c.shape[0])
+
+    def Radius_Search(self, point, radius: float):
+        """Live points within radius (cpp:406-411)."""
+        _, xyz, _ = self.radius_search_batch(np.asarray(point, np.float32).reshape(1, 3), [radius])
+        return xyz
+
+    # ------------------------------------------------------------ updates
+    def Add_Points(self, points, downsample_on: bool, ids=None) -> int:
+        p = _pts(points)
+        i = None if ids is None else np.ascontiguousarray(ids, np.uint32)
+        added = ctypes.c_int32(0)
+        self._check(self._lib.ikd_add_points(self._h, _ptr(p), _ptr(i), p.shape[0], 1 if downsample_on else 0,
+                                             ctypes.byref(added)))
+        return int(added.value)
+
+    def add_points_device(self, xyz, downsample_on: bool, ids=None) -> int:
+        """Add_Points with a device tensor xyz[n,3] (and optional ids[n])."""
+        added = ctypes.c_int32(0)
+        self._check(self._lib.ikd_add_points_device(self._h, _tensor_ptr(xyz),
+                                                    None if ids is None else _tensor_ptr(ids), xyz.shape[0],
+                                                    1 if downsample_on else 0, ctypes.byref(added)))
+        return int(added.value)
+
+    def Delete_Points(self, points):
+        p = _pts(points)
+        self._check(self._lib.ikd_delete_points(self._h, _ptr(p), p.shape[0]))
+
+    def Delete_Point_Boxes(self, boxes) -> int:
+        b = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
+        out = ctypes.c_int32(0)
+        self._check(self._lib.ikd_delete_point_boxes(self._h, _ptr(b), b.shape[0], ctypes.byref(out)))
+        return int(out.value)
+
+    # ------------------------------------------------------------ accessors
+    def size(self) -> int:
+        v = ctypes.c_int64(0)
+        self._check(self._lib.ikd_size(self._h, ctypes.byref(v)))
+        return int(v.value)
+
+    def validnum(self) -> int:
+        v = ctypes.c_int64(0)
+        self._check(self._lib.ikd_validnum(self._h, ctypes.byref(v)))
+        return int(v.value)
+
+    def tree_range(self) -> np.ndarray:
+        out = np.zeros(6, np.float32)
+        self._check(self._lib.ikd_tree_range(self._h, _ptr(out)))
+        return out
+
+    def root_alpha(self):
+        a = ctypes.c_float(0)
+        d = ctypes.c_float(0)
+        self._check(self._lib.ikd_root_alpha(self._h, ctypes.byref(a), ctypes.byref(d)))
+        return float(a.value), float(d.value)
+
+    def flatten(self):
+        n = ctypes.c_int64(0)
+        self._check(self._lib.ikd_flatten(self._h, None, None, 0, ctypes.byref(n)))
+        cnt = int(n.value)
+        xyz = np.zeros((max(cnt, 1), 3), np.float32)
+        ids = np.zeros(max(cnt, 1), np.uint32)
+        self._check(self._lib.ikd_flatten(self._h, _ptr(xyz), _ptr(ids), cnt, ctypes.byref(n)))
+        return xyz[:cnt], ids[:cnt]
+
+    def pool_stats(self):
+        used = ctypes.c_int64(0)
+        cap = ctypes.c_int64(0)
+        depth = ctypes.c_int32(0)
+        reb = ctypes.c_int64(0)
+        self._check(self._lib.ikd_pool_stats(self._h, ctypes.byref(used), ctypes.byref(cap), ctypes.byref(depth),
+                                             ctypes.byref(reb)))
+        return {"used_nodes": used.value, "capacity_nodes": cap.value, "max_depth": depth.value,
+                "rebuilds": reb.value}
+
+    def synchronize(self):
+        self._check(self._lib.ikd_synchronize(self._h))
+
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        self._check(self._lib.ikd_stream(self._h, ctypes.byref(s)))
+        return s.value or 0

@@ -1,0 +1,1258 @@
+// ikd_api.hip — the C ABI (include/ikd_tree_amd.h) and the host-side
+// orchestration of the device kernels: node-pool management, the build
+// driver, refit + Criterion_Check + Rebuild, and one function per public
+// KD_TREE method.
+//
+// Every public call leaves the tree consistent: lazy labels only below
+// tree_deleted nodes, counts and AABBs refit up every modified path, and no
+// node on a modified path violating Criterion_Check (the topmost violator of
+// each path is rebuilt, then its ancestors are re-checked, as the reference's
+// unwinding does, ikd_Tree.cpp:704-707, 860-863).
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ikd_internal.h"
+#include "ikd_tree_amd.h"
+
+using namespace ikd;
+
+namespace {
+
+constexpr int BS = 256;
+inline int nblk(long long n, int bs = BS) { return (int)((n + bs - 1) / bs); }
+inline int bits_for(long long v) {
+    int b = 0;
+    while ((1ll << b) <= v) ++b;
+    return b < 1 ? 1 : b;
+}
+
+// ----------------------------------------------------------- small kernels
+__global__ void k_reset_counters(DevHdr* h, int which) {
+    if (which & 1) { h->dirty_count = 0; h->viol_count = 0; h->top_count = 0; }
+    if (which & 2) h->rec_count = 0;
+    if (which & 4) h->conflicts = 0;
+    if (which & 8) { h->split_index = INT_MAX; h->added = 0; }
+    if (which & 16) h->group_count = 0;
+}
+
+__global__ void k_link_bits(NodeHot* hot, int node, uint32_t set, uint32_t clear) {
+    uint32_t* p = reinterpret_cast<uint32_t*>(&hot[node].f[3]);
+    uint32_t v = *p;
+    *p = (v | set) & ~clear;
+}
+
+__global__ void k_init_pool(Pool pool, DevHdr* hdr) {
+    // sentinel (node 0): its left child slot is the root (node 2, pair 1)
+    float* f = pool.hot[SENTINEL].f;
+    f[0] = f[1] = f[2] = 0.0f;
+    f[3] = bitsf(1u << LK_PAIR_SHIFT);
+    for (int i = 4; i < 16; ++i) f[i] = 0.0f;
+    pool.meta[SENTINEL] = M_PDEL;
+    pool.cnt[SENTINEL] = make_int4(0, 0, 0, -1);
+    pool.par[SENTINEL] = -1;
+    pool.ids[SENTINEL] = 0xffffffffu;
+    pool.meta[1] = M_PDEL;
+    pool.cnt[1] = make_int4(0, 0, 0, -1);
+    pool.par[1] = -1;
+    DevHdr z = {};
+    z.pair_used = 2;
+    z.split_index = INT_MAX;
+    *hdr = z;
+}
+
+__global__ void k_gather_pts(const float4* __restrict__ in, const uint32_t* __restrict__ idx, int n,
+                             float4* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[idx[i]];
+}
+
+__global__ void k_seeds_from_attach(const int* __restrict__ attach, DevHdr* hdr, int bound, int* __restrict__ seeds) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= bound) return;
+    seeds[g] = g < hdr->group_count ? (attach[g] >> 1) : -1;
+}
+
+__global__ void k_group_maxlen(const int4* __restrict__ seg, DevHdr* hdr, int bound, int* __restrict__ out) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= bound || g >= hdr->group_count) return;
+    atomicMax(out, seg[g].y - seg[g].x + 1);
+}
+
+// (TreeSize, invalid, depth, parent) of the listed nodes
+__global__ void k_node_info(Pool pool, const int* __restrict__ nodes, int n, int4* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int u = nodes[i];
+    int4 c = pool.cnt[u];
+    out[i] = make_int4(c.x, c.y, c.w, pool.par[u]);
+}
+
+__global__ void k_seg_of_from_keys(const unsigned long long* __restrict__ keys, int n, int* __restrict__ seg_of) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) seg_of[i] = (int)(keys[i] >> 32);
+}
+
+__global__ void k_unpack_points(const float4* __restrict__ in, int n, float* __restrict__ xyz,
+                                uint32_t* __restrict__ ids) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p = in[i];
+    xyz[3 * i] = p.x;
+    xyz[3 * i + 1] = p.y;
+    xyz[3 * i + 2] = p.z;
+    if (ids) ids[i] = fbits(p.w);
+}
+
+// root summary for the host: cnt[ROOT], sentinel link, root AABB
+struct RootInfo {
+    int4 cnt;
+    int4 child;  // TreeSize of left, right child (-1 if absent), link word, pad
+    float box[8];
+};
+__global__ void k_root_info(Pool pool, RootInfo* out) {
+    RootInfo r = {};
+    uint32_t lk0 = fbits(pool.hot[SENTINEL].f[3]);
+    r.child.z = (int)lk0;
+    if (lk0 & LK_HASL) {
+        r.cnt = pool.cnt[ROOT];
+        uint32_t lk = fbits(pool.hot[ROOT].f[3]);
+        int pair = (int)lk_pair(lk);
+        r.child.x = (lk & LK_HASL) ? pool.cnt[2 * pair].x : -1;
+        r.child.y = (lk & LK_HASR) ? pool.cnt[2 * pair + 1].x : -1;
+        for (int i = 0; i < 6; ++i) r.box[i] = pool.hot[SENTINEL].f[4 + i];
+    }
+    *out = r;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct ikd_tree_s {
+    Params prm{0.5f, 0.6f, 0.2f};
+    hipStream_t st = nullptr;
+    std::string err;
+    // node pool
+    long long cap = 0;
+    Pool pool{};
+    DevHdr* d_hdr = nullptr;
+    DevHdr* h_hdr = nullptr;  // pinned mirror
+    RootInfo* d_root = nullptr;
+    RootInfo* h_root = nullptr;
+    uint32_t next_id = 0;
+    long long rebuilds = 0;
+    int max_depth = 0;
+    long long pool_used = 4;
+    // scratch
+    DevBuf b[80];
+};
+
+namespace {
+
+enum Buf {
+    B_TMP = 0, B_PTS, B_PTS2, B_SEGOF, B_KEYA, B_KEYB, B_VALA, B_PERM0, B_PERM1 = B_PERM0 + 3, B_SID = B_PERM1 + 3,
+    B_SID2, B_SIDE, B_SEG0, B_SEG1, B_SEGD0, B_SEGD1, B_SEGAX, B_SEGMID, B_SEGCH, B_SEGFL, B_SEGSC, B_POSFL, B_POSSC,
+    B_SEEDS, B_DIRTY, B_VIOL, B_TOP, B_INFO, B_FRONT0, B_FRONT1, B_EKEY0, B_EKEY1, B_RECS, B_GFLAG, B_GSCAN, B_ATTACH,
+    B_PFLAG, B_PSCAN, B_ISEG, B_ISEGD, B_IO0, B_IO1, B_IO2, B_IO3, B_IO4, B_VSTART, B_DELBOX, B_DELFL, B_INSPT,
+    B_INSFL, B_CBOX, B_CPTS, B_MISC, B_MORTK, B_MORTV, B_MORTK2, B_MORTV2, B_CNT64, B_OFF64, B_ADDX, B_ADDI, B_COUNT
+};
+static_assert(B_COUNT <= 80, "scratch table");
+
+#define HIPCHK(x)                                                   \
+    do {                                                            \
+        hipError_t e_ = (x);                                        \
+        if (e_ != hipSuccess) {                                     \
+            t->err = std::string(#x) + ": " + hipGetErrorString(e_); \
+            return IKD_ERR_HIP;                                     \
+        }                                                           \
+    } while (0)
+#define CHK(x)                  \
+    do {                        \
+        int r_ = (x);           \
+        if (r_ != IKD_OK) return r_; \
+    } while (0)
+
+template <class T>
+int ensure(ikd_tree_t t, int slot, long long count, T** out) {
+    size_t need = (size_t)std::max<long long>(count, 1) * sizeof(T);
+    DevBuf& d = t->b[slot];
+    if (d.bytes < need) {
+        if (d.p) {
+            hipStreamSynchronize(t->st);  // queued kernels may still read the old buffer
+            hipFree(d.p);
+        }
+        size_t nb = std::max(need, d.bytes + d.bytes / 2);
+        if (hipMalloc(&d.p, nb) != hipSuccess) {
+            d.p = nullptr;
+            d.bytes = 0;
+            t->err = "device allocation of scratch failed";
+            return IKD_ERR_NOMEM;
+        }
+        d.bytes = nb;
+    }
+    *out = reinterpret_cast<T*>(d.p);
+    return IKD_OK;
+}
+
+int sync_hdr(ikd_tree_t t) {
+    HIPCHK(hipMemcpyAsync(t->h_hdr, t->d_hdr, sizeof(DevHdr), hipMemcpyDeviceToHost, t->st));
+    HIPCHK(hipStreamSynchronize(t->st));
+    t->pool_used = 2ll * t->h_hdr->pair_used;
+    t->max_depth = t->h_hdr->max_depth;
+    if (t->h_hdr->overflow) {
+        t->err = "device traversal stack overflow";
+        return IKD_ERR_INTERNAL;
+    }
+    return IKD_OK;
+}
+
+int read_root(ikd_tree_t t) {
+    hipLaunchKernelGGL(k_root_info, dim3(1), dim3(1), 0, t->st, t->pool, t->d_root);
+    HIPCHK(hipMemcpyAsync(t->h_root, t->d_root, sizeof(RootInfo), hipMemcpyDeviceToHost, t->st));
+    HIPCHK(hipStreamSynchronize(t->st));
+    return IKD_OK;
+}
+bool has_root(ikd_tree_t t) { return (t->h_root->child.z & (int)LK_HASL) != 0; }
+long long live_count(ikd_tree_t t) {
+    if (!has_root(t)) return 0;
+    return (long long)t->h_root->cnt.x - t->h_root->cnt.y;
+}
+
+template <class T>
+int prim_tmp(ikd_tree_t t, size_t bytes, void** out) {
+    (void)sizeof(T);
+    char* p;
+    CHK(ensure(t, B_TMP, (long long)bytes, &p));
+    *out = p;
+    return IKD_OK;
+}
+
+int sort_pairs(ikd_tree_t t, const unsigned long long* kin, unsigned long long* kout, const uint32_t* vin,
+               uint32_t* vout, int n, int end_bit) {
+    if (n <= 0) return IKD_OK;
+    size_t bytes = 0;
+    HIPCHK(prim_sort_pairs_u64(nullptr, bytes, kin, kout, vin, vout, n, 0, end_bit, t->st));
+    void* tmp;
+    CHK(prim_tmp<char>(t, bytes, &tmp));
+    HIPCHK(prim_sort_pairs_u64(tmp, bytes, kin, kout, vin, vout, n, 0, end_bit, t->st));
+    return IKD_OK;
+}
+int sort_keys(ikd_tree_t t, const unsigned long long* kin, unsigned long long* kout, int n, int end_bit) {
+    if (n <= 0) return IKD_OK;
+    size_t bytes = 0;
+    HIPCHK(prim_sort_keys_u64(nullptr, bytes, kin, kout, n, 0, end_bit, t->st));
+    void* tmp;
+    CHK(prim_tmp<char>(t, bytes, &tmp));
+    HIPCHK(prim_sort_keys_u64(tmp, bytes, kin, kout, n, 0, end_bit, t->st));
+    return IKD_OK;
+}
+int exscan_u64(ikd_tree_t t, const unsigned long long* in, unsigned long long* out, int n) {
+    if (n <= 0) return IKD_OK;
+    size_t bytes = 0;
+    HIPCHK(prim_exscan_u64(nullptr, bytes, in, out, n, t->st));
+    void* tmp;
+    CHK(prim_tmp<char>(t, bytes, &tmp));
+    HIPCHK(prim_exscan_u64(tmp, bytes, in, out, n, t->st));
+    return IKD_OK;
+}
+int exscan_i32(ikd_tree_t t, const int* in, int* out, int n) {
+    if (n <= 0) return IKD_OK;
+    size_t bytes = 0;
+    HIPCHK(prim_exscan_i32(nullptr, bytes, in, out, n, t->st));
+    void* tmp;
+    CHK(prim_tmp<char>(t, bytes, &tmp));
+    HIPCHK(prim_exscan_i32(tmp, bytes, in, out, n, t->st));
+    return IKD_OK;
+}
+int exscan_i64(ikd_tree_t t, const long long* in, long long* out, int n) {
+    if (n <= 0) return IKD_OK;
+    size_t bytes = 0;
+    HIPCHK(prim_exscan_i64(nullptr, bytes, in, out, n, t->st));
+    void* tmp;
+    CHK(prim_tmp<char>(t, bytes, &tmp));
+    HIPCHK(prim_exscan_i64(tmp, bytes, in, out, n, t->st));
+    return IKD_OK;
+}
+
+// ------------------------------------------------------------- node pool
+int alloc_pool(ikd_tree_t t, long long cap, Pool* p) {
+    if (hipMalloc(&p->hot, cap * sizeof(NodeHot)) != hipSuccess) return IKD_ERR_NOMEM;
+    if (hipMalloc(&p->ids, cap * sizeof(uint32_t)) != hipSuccess) return IKD_ERR_NOMEM;
+    if (hipMalloc(&p->par, cap * sizeof(int32_t)) != hipSuccess) return IKD_ERR_NOMEM;
+    if (hipMalloc(&p->cnt, cap * sizeof(int4)) != hipSuccess) return IKD_ERR_NOMEM;
+    if (hipMalloc(&p->meta, cap * sizeof(uint32_t)) != hipSuccess) return IKD_ERR_NOMEM;
+    (void)t;
+    return IKD_OK;
+}
+void free_pool(Pool* p) {
+    hipFree(p->hot);
+    hipFree(p->ids);
+    hipFree(p->par);
+    hipFree(p->cnt);
+    hipFree(p->meta);
+    *p = Pool{};
+}
+
+// Make room for `extra` more nodes (pool_used is exact after the last sync).
+int reserve_nodes(ikd_tree_t t, long long extra) {
+    long long need = t->pool_used + extra + 8;
+    if (need <= t->cap) return IKD_OK;
+    if (need >= (1ll << 28)) {
+        t->err = "node pool would exceed 2^28 nodes";
+        return IKD_ERR_NOMEM;
+    }
+    long long ncap = std::max(need + need / 2, 2 * t->cap);
+    ncap = std::min(ncap, 1ll << 28);
+    Pool np{};
+    if (alloc_pool(t, ncap, &np) != IKD_OK) {
+        free_pool(&np);
+        t->err = "node pool allocation failed";
+        return IKD_ERR_NOMEM;
+    }
+    long long used = t->pool_used;
+    if (t->cap > 0) {
+        HIPCHK(hipMemcpyAsync(np.hot, t->pool.hot, used * sizeof(NodeHot), hipMemcpyDeviceToDevice, t->st));
+        HIPCHK(hipMemcpyAsync(np.ids, t->pool.ids, used * sizeof(uint32_t), hipMemcpyDeviceToDevice, t->st));
+        HIPCHK(hipMemcpyAsync(np.par, t->pool.par, used * sizeof(int32_t), hipMemcpyDeviceToDevice, t->st));
+        HIPCHK(hipMemcpyAsync(np.cnt, t->pool.cnt, used * sizeof(int4), hipMemcpyDeviceToDevice, t->st));
+        HIPCHK(hipMemcpyAsync(np.meta, t->pool.meta, used * sizeof(uint32_t), hipMemcpyDeviceToDevice, t->st));
+        HIPCHK(hipStreamSynchronize(t->st));
+        free_pool(&t->pool);
+    }
+    t->pool = np;
+    t->cap = ncap;
+    return IKD_OK;
+}
+
+int reset_pool(ikd_tree_t t) {
+    t->pool_used = 4;
+    t->max_depth = 0;
+    CHK(reserve_nodes(t, 4));
+    hipLaunchKernelGGL(k_init_pool, dim3(1), dim3(1), 0, t->st, t->pool, t->d_hdr);
+    HIPCHK(hipGetLastError());
+    return IKD_OK;
+}
+
+// -------------------------------------------------------------- the build
+// Builds S subtrees at once.  pts: m points grouped by segment (segment g at
+// positions [seg[g].x, seg[g].y]); seg_of: segment of each point; seg /
+// seg_depth: device arrays (l, r, slot node, slot parent) + depth, S entries.
+int run_build(ikd_tree_t t, const float4* pts, const int* seg_of, int m, const int4* d_seg, const int* d_seg_depth,
+              int S, int maxlen) {
+    if (m <= 0 || S <= 0) return IKD_OK;
+    CHK(reserve_nodes(t, 2ll * m + 4));
+    hipStream_t st = t->st;
+    unsigned long long *keya, *keyb;
+    uint32_t* vala;
+    CHK(ensure(t, B_KEYA, m, &keya));
+    CHK(ensure(t, B_KEYB, m, &keyb));
+    CHK(ensure(t, B_VALA, m, &vala));
+    uint32_t* perm[2][3];
+    for (int a = 0; a < 3; ++a) {
+        CHK(ensure(t, B_PERM0 + a, m, &perm[0][a]));
+        CHK(ensure(t, B_PERM1 + a, m, &perm[1][a]));
+    }
+    int *sid[2];
+    CHK(ensure(t, B_SID, m, &sid[0]));
+    CHK(ensure(t, B_SID2, m, &sid[1]));
+    uint8_t* side;
+    CHK(ensure(t, B_SIDE, m, &side));
+    long long segcap = (long long)m + S + 1;
+    int4* seg[2];
+    int* segd[2];
+    CHK(ensure(t, B_SEG0, segcap, &seg[0]));
+    CHK(ensure(t, B_SEG1, segcap, &seg[1]));
+    CHK(ensure(t, B_SEGD0, segcap, &segd[0]));
+    CHK(ensure(t, B_SEGD1, segcap, &segd[1]));
+    int *segax, *segmid;
+    int2* segch;
+    unsigned long long *segfl, *segsc, *posfl, *possc;
+    CHK(ensure(t, B_SEGAX, segcap, &segax));
+    CHK(ensure(t, B_SEGMID, segcap, &segmid));
+    CHK(ensure(t, B_SEGCH, segcap, &segch));
+    CHK(ensure(t, B_SEGFL, segcap, &segfl));
+    CHK(ensure(t, B_SEGSC, segcap, &segsc));
+    CHK(ensure(t, B_POSFL, 3ll * m, &posfl));
+    CHK(ensure(t, B_POSSC, 3ll * m, &possc));
+
+    int end_bit = 32 + bits_for(S);
+    for (int a = 0; a < 3; ++a) {
+        launch_build_keys(pts, seg_of, m, a, keya, vala, st);
+        CHK(sort_pairs(t, keya, keyb, vala, perm[0][a], m, end_bit));
+        if (a == 0) launch_build_init_sid(keyb, m, sid[0], st);
+    }
+    HIPCHK(hipMemcpyAsync(seg[0], d_seg, S * sizeof(int4), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(segd[0], d_seg_depth, S * sizeof(int), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(&t->d_hdr->seg_count[0], &S, sizeof(int), hipMemcpyHostToDevice, st));
+    // a pageable host source: make sure it is consumed before S goes away
+    HIPCHK(hipStreamSynchronize(st));
+
+    int levels = bits_for(maxlen);  // floor(log2(maxlen)) + 1
+    for (int L = 0; L < levels; ++L) {
+        int cur = L & 1;
+        long long bound = std::min<long long>((long long)S << std::min(L, 30), m);
+        BuildLevel BL;
+        BL.pts = pts;
+        for (int a = 0; a < 3; ++a) {
+            BL.perm[a] = perm[cur][a];
+            BL.perm_next[a] = perm[cur ^ 1][a];
+        }
+        BL.sid = sid[cur];
+        BL.sid_next = sid[cur ^ 1];
+        BL.side = side;
+        BL.seg = seg[cur];
+        BL.seg_depth = segd[cur];
+        BL.seg_next = seg[cur ^ 1];
+        BL.seg_depth_next = segd[cur ^ 1];
+        BL.seg_axis = segax;
+        BL.seg_mid = segmid;
+        BL.seg_child = segch;
+        BL.seg_flags = segfl;
+        BL.seg_scan = segsc;
+        BL.pos_flags = posfl;
+        BL.pos_scan = possc;
+        BL.m = m;
+        BL.cur = cur;
+        launch_build_seg_a(BL, (int)bound, t->d_hdr, st);
+        CHK(exscan_u64(t, segfl, segsc, (int)bound));
+        launch_build_seg_b(BL, (int)bound, t->pool, t->d_hdr, st);
+        launch_build_side(BL, t->d_hdr, st);
+        if (L + 1 < levels) {
+            launch_build_flags(BL, st);
+            CHK(exscan_u64(t, posfl, possc, 3 * m));
+            launch_build_scatter(BL, st);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    return IKD_OK;
+}
+
+// ---------------------------------------------------------- enumerate live
+// Live points of the subtrees rooted at roots[] (host list), grouped by root
+// in root order, each group ordered by node index.  Returns the points in
+// B_PTS2 (float4) and their root index in B_SEGOF.
+int enumerate_live(ikd_tree_t t, const std::vector<int>& roots, long long total_nodes, long long total_live,
+                   float4** out_pts, int** out_seg) {
+    hipStream_t st = t->st;
+    int R = (int)roots.size();
+    long long cap = std::max<long long>(total_nodes, 1);
+    int2 *f0, *f1;
+    unsigned long long *k0, *k1;
+    CHK(ensure(t, B_FRONT0, cap, &f0));
+    CHK(ensure(t, B_FRONT1, cap, &f1));
+    CHK(ensure(t, B_EKEY0, std::max<long long>(total_live, 1), &k0));
+    CHK(ensure(t, B_EKEY1, std::max<long long>(total_live, 1), &k1));
+    std::vector<int2> h(R);
+    for (int i = 0; i < R; ++i) h[i] = make_int2(roots[i], i);
+    HIPCHK(hipMemcpyAsync(f0, h.data(), R * sizeof(int2), hipMemcpyHostToDevice, st));
+    int init[3] = {R, 0, 0};
+    HIPCHK(hipMemcpyAsync(t->d_hdr->front3, init, sizeof(init), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, t->d_hdr, 2);
+    int levels = t->max_depth + 2;
+    for (int L = 0; L < levels; ++L) {
+        int2* fin = (L & 1) ? f1 : f0;
+        int2* fout = (L & 1) ? f0 : f1;
+        launch_enum_level(t->pool, fin, L, fout, k0, t->d_hdr, (int)cap, (int)std::max<long long>(total_live, 1), st);
+    }
+    CHK(sync_hdr(t));
+    if (t->h_hdr->rec_count != total_live) {
+        char msg[160];
+        snprintf(msg, sizeof msg, "enumerate found %d live points, counts say %lld", t->h_hdr->rec_count, total_live);
+        t->err = msg;
+        return IKD_ERR_INTERNAL;
+    }
+    int n = (int)total_live;
+    CHK(sort_keys(t, k0, k1, n, 32 + bits_for(R)));
+    float4* pts;
+    int* segof;
+    CHK(ensure(t, B_PTS2, std::max(n, 1), &pts));
+    CHK(ensure(t, B_SEGOF, std::max(n, 1), &segof));
+    launch_gather_points(t->pool, k1, n, pts, st);
+    if (n > 0) hipLaunchKernelGGL(k_seg_of_from_keys, dim3(nblk(n)), dim3(BS), 0, st, k1, n, segof);
+    HIPCHK(hipGetLastError());
+    *out_pts = pts;
+    *out_seg = segof;
+    return IKD_OK;
+}
+
+// ------------------------------------------------- refit, criterion, rebuild
+int refit(ikd_tree_t t, const int* seeds, int nseeds, int stop_at_tdel) {
+    hipStream_t st = t->st;
+    int *dirty, *viol, *top;
+    long long cap = std::max<long long>(t->pool_used, 16);
+    CHK(ensure(t, B_DIRTY, cap, &dirty));
+    CHK(ensure(t, B_VIOL, cap, &viol));
+    CHK(ensure(t, B_TOP, cap, &top));
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, t->d_hdr, 1);
+    launch_mark_dirty(t->pool, seeds, nullptr, nseeds, stop_at_tdel, dirty, t->d_hdr, st);
+    for (int d = t->max_depth; d >= 0; --d) launch_refit_level(t->pool, t->prm, dirty, t->d_hdr, d, viol, st);
+    launch_find_top(t->pool, viol, (int)cap, t->d_hdr, top, st);
+    launch_clear_viol(t->pool, viol, (int)cap, t->d_hdr, st);
+    HIPCHK(hipGetLastError());
+    return IKD_OK;
+}
+
+int rebuild_tops(ikd_tree_t t) {
+    hipStream_t st = t->st;
+    for (int iter = 0; iter < 64; ++iter) {
+        CHK(sync_hdr(t));
+        int ntop = t->h_hdr->top_count;
+        if (ntop == 0) return IKD_OK;
+        int* top;
+        int4* info;
+        CHK(ensure(t, B_TOP, ntop, &top));
+        CHK(ensure(t, B_INFO, ntop, &info));
+        hipLaunchKernelGGL(k_node_info, dim3(nblk(ntop)), dim3(BS), 0, st, t->pool, top, ntop, info);
+        std::vector<int> htop(ntop);
+        std::vector<int4> hinfo(ntop);
+        HIPCHK(hipMemcpyAsync(htop.data(), top, ntop * sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(hinfo.data(), info, ntop * sizeof(int4), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        std::vector<int> roots, seeds;
+        std::vector<int4> segs;
+        std::vector<int> segd;
+        long long total_nodes = 0, total_live = 0;
+        int maxlen = 0;
+        bool root_rebuilt = false;
+        for (int i = 0; i < ntop; ++i) {
+            int u = htop[i];
+            int live = hinfo[i].x - hinfo[i].y;
+            int parent = hinfo[i].w;
+            seeds.push_back(parent);
+            if (u == ROOT) root_rebuilt = true;
+            if (live <= 0) {
+                // Rebuild of an all-deleted subtree leaves an empty slot (BuildTree of
+                // an empty range, cpp:574).
+                uint32_t clr = lk_has(u & 1) | lk_td(u & 1);
+                hipLaunchKernelGGL(k_link_bits, dim3(1), dim3(1), 0, st, t->pool.hot, parent, 0u, clr);
+                continue;
+            }
+            segs.push_back(make_int4((int)total_live, (int)(total_live + live - 1), u, parent));
+            segd.push_back(hinfo[i].z);
+            roots.push_back(u);
+            total_nodes += hinfo[i].x;
+            total_live += live;
+            maxlen = std::max(maxlen, live);
+        }
+        if (!roots.empty()) {
+            float4* pts;
+            int* segof;
+            CHK(enumerate_live(t, roots, total_nodes, total_live, &pts, &segof));
+            int4* dseg;
+            int* dsegd;
+            CHK(ensure(t, B_ISEG, (long long)segs.size(), &dseg));
+            CHK(ensure(t, B_ISEGD, (long long)segs.size(), &dsegd));
+            HIPCHK(hipMemcpyAsync(dseg, segs.data(), segs.size() * sizeof(int4), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(dsegd, segd.data(), segd.size() * sizeof(int), hipMemcpyHostToDevice, st));
+            if (root_rebuilt) {
+                int zero = 0;
+                HIPCHK(hipMemcpyAsync(&t->d_hdr->max_depth, &zero, sizeof(int), hipMemcpyHostToDevice, st));
+            }
+            CHK(run_build(t, pts, segof, (int)total_live, dseg, dsegd, (int)segs.size(), maxlen));
+        }
+        t->rebuilds += ntop;
+        // ancestors of the rebuilt subtrees shrink: refit and re-check them
+        int* dseeds;
+        CHK(ensure(t, B_SEEDS, (long long)seeds.size(), &dseeds));
+        HIPCHK(hipMemcpyAsync(dseeds, seeds.data(), seeds.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        CHK(sync_hdr(t));
+        CHK(refit(t, dseeds, (int)seeds.size(), 0));
+    }
+    t->err = "rebuild did not converge";
+    return IKD_ERR_INTERNAL;
+}
+
+// Rebuild the whole tree from its live points when the pool holds mostly
+// garbage or the tree got deeper than the traversal stacks allow.
+int compact_if_needed(ikd_tree_t t, bool force) {
+    CHK(read_root(t));
+    CHK(sync_hdr(t));
+    long long live = live_count(t);
+    bool deep = t->max_depth + 2 > MAX_DEPTH;
+    bool sparse = t->pool_used > 4 * live + (1ll << 22);
+    if (!force && !deep && !sparse) return IKD_OK;
+    if (!has_root(t)) return reset_pool(t);
+    float4* pts;
+    int* segof;
+    std::vector<int> roots = {ROOT};
+    CHK(enumerate_live(t, roots, t->h_root->cnt.x, live, &pts, &segof));
+    // move the points out of the scratch slot the build reuses
+    float4* keep;
+    CHK(ensure(t, B_PTS, live, &keep));
+    HIPCHK(hipMemcpyAsync(keep, pts, live * sizeof(float4), hipMemcpyDeviceToDevice, t->st));
+    CHK(reset_pool(t));
+    if (live == 0) return IKD_OK;
+    int* segof0;
+    CHK(ensure(t, B_SEGOF, live, &segof0));
+    launch_fill_i32(segof0, 0, live, t->st);
+    int4 s0 = make_int4(0, (int)live - 1, ROOT, SENTINEL);
+    int d0 = 0;
+    int4* dseg;
+    int* dsegd;
+    CHK(ensure(t, B_ISEG, 1, &dseg));
+    CHK(ensure(t, B_ISEGD, 1, &dsegd));
+    HIPCHK(hipMemcpyAsync(dseg, &s0, sizeof(s0), hipMemcpyHostToDevice, t->st));
+    HIPCHK(hipMemcpyAsync(dsegd, &d0, sizeof(d0), hipMemcpyHostToDevice, t->st));
+    CHK(run_build(t, keep, segof0, (int)live, dseg, dsegd, 1, (int)live));
+    hipLaunchKernelGGL(k_link_bits, dim3(1), dim3(1), 0, t->st, t->pool.hot, SENTINEL, LK_HASL, LK_TDL);
+    t->rebuilds += 1;
+    return sync_hdr(t);
+}
+
+// ---------------------------------------------------------------- insert
+// Inserts m packed points (device, float4 x,y,z,id).  The caller refits.
+int insert_points(ikd_tree_t t, const float4* pts, int m) {
+    if (m <= 0) return IKD_OK;
+    hipStream_t st = t->st;
+    CHK(reserve_nodes(t, 4ll * m + 16));
+    unsigned long long *ka, *kb;
+    uint32_t *va, *vb;
+    CHK(ensure(t, B_IO0, m, &ka));
+    CHK(ensure(t, B_IO1, m, &kb));
+    CHK(ensure(t, B_IO2, m, &va));
+    CHK(ensure(t, B_IO3, m, &vb));
+    launch_insert_descend(t->pool, pts, m, ka, va, t->d_hdr, st);
+    CHK(sort_pairs(t, ka, kb, va, vb, m, 1 + bits_for(t->pool_used)));
+    int *gflag, *gscan, *attach, *pflag, *pscan, *segof, *segd;
+    int4* seg;
+    CHK(ensure(t, B_GFLAG, m, &gflag));
+    CHK(ensure(t, B_GSCAN, m, &gscan));
+    CHK(ensure(t, B_ATTACH, m, &attach));
+    CHK(ensure(t, B_PFLAG, m, &pflag));
+    CHK(ensure(t, B_PSCAN, m, &pscan));
+    CHK(ensure(t, B_SEGOF, m, &segof));
+    CHK(ensure(t, B_ISEG, m, &seg));
+    CHK(ensure(t, B_ISEGD, m, &segd));
+    launch_insert_groups(kb, m, gflag, st);
+    CHK(exscan_i32(t, gflag, gscan, m));
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, t->d_hdr, 16);
+    launch_insert_segments2(t->pool, kb, gscan, gflag, m, seg, segd, segof, attach, pflag, t->d_hdr, st);
+    CHK(exscan_i32(t, pflag, pscan, m));
+    launch_insert_pairs(t->pool, attach, pflag, pscan, t->d_hdr, m, seg, st);
+    int* misc;
+    CHK(ensure(t, B_MISC, 4, &misc));
+    launch_fill_i32(misc, 0, 1, st);
+    hipLaunchKernelGGL(k_group_maxlen, dim3(nblk(m)), dim3(BS), 0, st, seg, t->d_hdr, m, misc);
+    float4* spts;
+    CHK(ensure(t, B_PTS2, m, &spts));
+    hipLaunchKernelGGL(k_gather_pts, dim3(nblk(m)), dim3(BS), 0, st, pts, vb, m, spts);
+    int maxlen = 0;
+    HIPCHK(hipMemcpyAsync(&maxlen, misc, sizeof(int), hipMemcpyDeviceToHost, st));
+    CHK(sync_hdr(t));
+    int S = t->h_hdr->group_count;
+    CHK(run_build(t, spts, segof, m, seg, segd, S, maxlen));
+    launch_insert_link(t->pool, attach, seg, t->d_hdr, S, st);
+    int* seeds;
+    CHK(ensure(t, B_SEEDS, S, &seeds));
+    hipLaunchKernelGGL(k_seeds_from_attach, dim3(nblk(S)), dim3(BS), 0, st, attach, t->d_hdr, S, seeds);
+    CHK(sync_hdr(t));
+    CHK(refit(t, seeds, S, 0));
+    return rebuild_tops(t);
+}
+
+// ---------------------------------------------------------- box deletion
+int delete_boxes_dev(ikd_tree_t t, const float* d_boxes, int nb, int ds) {
+    if (nb <= 0) return IKD_OK;
+    hipStream_t st = t->st;
+    unsigned int* recs;
+    long long cap = std::max<long long>(t->pool_used, 16) + 4ll * nb;
+    CHK(ensure(t, B_RECS, cap, &recs));
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, t->d_hdr, 2);
+    launch_box_delete_traverse(t->pool, d_boxes, nb, recs, t->d_hdr, (int)cap, t->pool_used + 16, st);
+    CHK(sync_hdr(t));
+    int nrec = t->h_hdr->rec_count;
+    if (nrec > cap) {
+        // overlapping boxes recorded more than one entry per node: the union
+        // is order-free, so apply the boxes in halves
+        if (nb == 1) {
+            t->err = "box delete: record buffer overflow";
+            return IKD_ERR_INTERNAL;
+        }
+        int h = nb / 2;
+        CHK(delete_boxes_dev(t, d_boxes, h, ds));
+        return delete_boxes_dev(t, d_boxes + 6ll * h, nb - h, ds);
+    }
+    if (nrec == 0) return IKD_OK;
+    int* seeds;
+    CHK(ensure(t, B_SEEDS, nrec, &seeds));
+    launch_box_delete_apply(t->pool, recs, t->d_hdr, nrec, ds, seeds, st);
+    CHK(refit(t, seeds, nrec, 1));
+    return rebuild_tops(t);
+}
+
+// ---------------------------------------------------- Add_Points pipeline
+int add_points_ds_chunk(ikd_tree_t t, const float* d_xyz, const uint32_t* d_ids, uint32_t id_base, int n, int check,
+                        bool apply, int* split_out, int* added_out) {
+    hipStream_t st = t->st;
+    unsigned long long *ka, *kb;
+    uint32_t *va, *vb;
+    float4* pts;
+    int *flag, *scan, *vstart, *bad;
+    CHK(ensure(t, B_MORTK, n, &ka));
+    CHK(ensure(t, B_MORTK2, n, &kb));
+    CHK(ensure(t, B_MORTV, n, &va));
+    CHK(ensure(t, B_MORTV2, n, &vb));
+    CHK(ensure(t, B_PTS, n, &pts));
+    CHK(ensure(t, B_GFLAG, n, &flag));
+    CHK(ensure(t, B_GSCAN, n, &scan));
+    CHK(ensure(t, B_VSTART, n + 1, &vstart));
+    CHK(ensure(t, B_MISC, 4, &bad));
+    launch_fill_i32(bad, 0, 4, st);
+    launch_voxel_keys2(d_xyz, d_ids, id_base, n, t->prm.box_length, ka, va, pts, bad, st);
+    CHK(sort_pairs(t, ka, kb, va, vb, n, 63));
+    launch_voxel_flags(kb, n, flag, st);
+    CHK(exscan_i32(t, flag, scan, n));
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, t->d_hdr, 8 | 16);
+    launch_voxel_starts(flag, scan, n, vstart, t->d_hdr, st);
+    float* delbox;
+    float4* inspt;
+    int *delfl, *insfl;
+    CHK(ensure(t, B_DELBOX, 6ll * n, &delbox));
+    CHK(ensure(t, B_DELFL, n, &delfl));
+    CHK(ensure(t, B_INSPT, n, &inspt));
+    CHK(ensure(t, B_INSFL, n, &insfl));
+    launch_voxel_fold2(t->pool, pts, vb, kb, vstart, n, n, t->prm.box_length, check, delbox, delfl, inspt, insfl,
+                       t->d_hdr, t->pool_used + 16, st);
+    int hbad = 0;
+    HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
+    CHK(sync_hdr(t));
+    if (hbad) {
+        t->err = "Add_Points: a voxel index floor(p/box_length) is beyond +-2^20";
+        return IKD_ERR_UNSUPPORTED;
+    }
+    *split_out = t->h_hdr->split_index;
+    *added_out = t->h_hdr->added;
+    if (!apply || t->h_hdr->split_index != INT_MAX) return IKD_OK;
+    int nv = t->h_hdr->group_count;
+    // compact: boxes to delete (downsample labels), then points to insert
+    float* cbox;
+    float4* cpts;
+    int* counts;
+    CHK(ensure(t, B_CBOX, 6ll * nv, &cbox));
+    CHK(ensure(t, B_CPTS, nv, &cpts));
+    CHK(ensure(t, B_CNT64, 2, &counts));
+    launch_fill_i32(counts, 0, 2, st);
+    CHK(exscan_i32(t, delfl, scan, nv));
+    launch_compact_boxes(delfl, scan, delbox, nv, t->d_hdr, cbox, counts, st);
+    int hc[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(&hc[0], counts, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (hc[0] > 0) CHK(delete_boxes_dev(t, cbox, hc[0], 1));
+    CHK(exscan_i32(t, insfl, scan, nv));
+    launch_compact_pts(insfl, scan, inspt, nv, t->d_hdr, cpts, counts + 1, st);
+    HIPCHK(hipMemcpyAsync(&hc[1], counts + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return insert_points(t, cpts, hc[1]);
+}
+
+bool exact_voxels(float ds) {
+    int e;
+    float m = std::frexp(ds, &e);
+    return ds > 0.0f && m == 0.5f;
+}
+
+int add_points_dev(ikd_tree_t t, const float* d_xyz, const uint32_t* d_ids, int n, int downsample, int* added) {
+    *added = 0;
+    if (n <= 0) return IKD_OK;
+    uint32_t base = t->next_id;
+    if (!d_ids) t->next_id += (uint32_t)n;
+    CHK(sync_hdr(t));
+    if (!downsample) {
+        float4* pts;
+        CHK(ensure(t, B_PTS, n, &pts));
+        launch_pack_points(d_xyz, d_ids, base, n, pts, t->st);
+        CHK(insert_points(t, pts, n));
+        return compact_if_needed(t, false);
+    }
+    if (!(t->prm.box_length > 0.0f)) {
+        t->err = "Add_Points: downsample needs box_length > 0";
+        return IKD_ERR_ARG;
+    }
+    int check = exact_voxels(t->prm.box_length) ? 0 : 1;
+    int start = 0;
+    while (start < n) {
+        int len = n - start;
+        const float* x = d_xyz + 3ll * start;
+        const uint32_t* ids = d_ids ? d_ids + start : nullptr;
+        uint32_t b0 = base + (uint32_t)start;
+        int split = INT_MAX, a = 0;
+        CHK(add_points_ds_chunk(t, x, ids, b0, len, check, true, &split, &a));
+        if (split == INT_MAX) {
+            *added += a;
+            break;
+        }
+        // ordered handling: the conflict-free prefix, then (for a point that
+        // sits in two voxel boxes or outside its own) that point alone
+        int idx = split >> 1;
+        bool single = (split & 1) == 0;
+        if (idx > 0) {
+            CHK(add_points_ds_chunk(t, x, ids, b0, idx, check, true, &split, &a));
+            if (split != INT_MAX) {
+                t->err = "Add_Points: ordered prefix still conflicts";
+                return IKD_ERR_INTERNAL;
+            }
+            *added += a;
+        }
+        if (single) {
+            CHK(add_points_ds_chunk(t, x + 3ll * idx, ids ? ids + idx : nullptr, b0 + (uint32_t)idx, 1, 0, true,
+                                    &split, &a));
+            *added += a;
+            start += idx + 1;
+        } else {
+            start += idx;
+        }
+    }
+    return compact_if_needed(t, false);
+}
+
+float max_dist_threshold(double max_dist) {
+    double m = max_dist * max_dist;
+    if (std::isnan(m)) return -INFINITY;
+    float f = (float)m;
+    if ((double)f > m) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+
+int knn_dev(ikd_tree_t t, const float* d_q, int nq, int k, double max_dist, float* d_xyz, float* d_d2, uint32_t* d_ids,
+            int* d_cnt, unsigned int* d_visits, unsigned int* d_boxes) {
+    if (k < 1) {
+        t->err = "Nearest_Search: k must be >= 1";
+        return IKD_ERR_ARG;
+    }
+    if (k > 64) {
+        t->err = "Nearest_Search: k above 64 is not supported";
+        return IKD_ERR_UNSUPPORTED;
+    }
+    if (nq <= 0) return IKD_OK;
+    if (t->max_depth + 2 > MAX_DEPTH) CHK(compact_if_needed(t, true));
+    KnnArgs a{};
+    a.hot = t->pool.hot;
+    a.ids = t->pool.ids;
+    a.q = d_q;
+    a.order = nullptr;
+    a.nq = nq;
+    a.k = k;
+    a.thr = max_dist_threshold(max_dist);
+    a.out_xyz = d_xyz;
+    a.out_d2 = d_d2;
+    a.out_ids = d_ids;
+    a.out_counts = d_cnt;
+    a.visits = d_visits;
+    a.boxes = d_boxes;
+    a.overflow = &t->d_hdr->overflow;
+    a.max_steps = (int)std::min<long long>(t->pool_used + 16, INT_MAX);
+    if (nq >= 4096) {
+        unsigned long long *mk, *mk2;
+        uint32_t *mv, *mv2;
+        CHK(ensure(t, B_MORTK, nq, &mk));
+        CHK(ensure(t, B_MORTK2, nq, &mk2));
+        CHK(ensure(t, B_MORTV, nq, &mv));
+        CHK(ensure(t, B_MORTV2, nq, &mv2));
+        launch_morton(d_q, nq, t->pool.hot, mk, mv, t->st);
+        CHK(sort_pairs(t, mk, mk2, mv, mv2, nq, 30));
+        a.order = mv2;
+    }
+    int stack = t->max_depth + 2 <= 32 ? 32 : 64;
+    if (launch_knn(a, stack, 64, t->st) != 0) return IKD_ERR_UNSUPPORTED;
+    HIPCHK(hipGetLastError());
+    return IKD_OK;
+}
+
+int range_dev(ikd_tree_t t, int mode, const float* d_boxes, const float* d_centers, const float* d_radii, int nq,
+              int64_t* h_offsets, float* h_xyz, uint32_t* h_ids, int64_t capacity) {
+    hipStream_t st = t->st;
+    long long *cnt, *off;
+    CHK(ensure(t, B_CNT64, nq + 1, &cnt));
+    CHK(ensure(t, B_OFF64, nq + 1, &off));
+    HIPCHK(hipMemsetAsync(cnt + nq, 0, sizeof(long long), st));
+    RangeArgs a{};
+    a.max_steps = t->pool_used + 16;
+    a.pool = t->pool;
+    a.boxes = d_boxes;
+    a.centers = d_centers;
+    a.radii = d_radii;
+    a.nq = nq;
+    a.mode = mode;
+    a.counts = cnt;
+    a.overflow = &t->d_hdr->overflow;
+    launch_range(a, 0, st);
+    CHK(exscan_i64(t, cnt, off, nq + 1));
+    HIPCHK(hipMemcpyAsync(h_offsets, off, (nq + 1) * sizeof(long long), hipMemcpyDeviceToHost, st));
+    CHK(sync_hdr(t));
+    long long total = h_offsets[nq];
+    if (!h_xyz) return IKD_OK;
+    if (capacity < total) {
+        t->err = "output capacity below the result count";
+        return IKD_ERR_CAPACITY;
+    }
+    if (total == 0) return IKD_OK;
+    float* dx;
+    uint32_t* di;
+    CHK(ensure(t, B_IO0, 3 * total, &dx));
+    CHK(ensure(t, B_IO1, total, &di));
+    a.offsets = off;
+    a.out_xyz = dx;
+    a.out_ids = di;
+    launch_range(a, 1, st);
+    HIPCHK(hipMemcpyAsync(h_xyz, dx, 3 * total * sizeof(float), hipMemcpyDeviceToHost, st));
+    if (h_ids) HIPCHK(hipMemcpyAsync(h_ids, di, total * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    return sync_hdr(t);
+}
+
+template <class T>
+int upload(ikd_tree_t t, int slot, const T* h, long long n, T** d) {
+    CHK(ensure(t, slot, std::max<long long>(n, 1), d));
+    if (n > 0) HIPCHK(hipMemcpyAsync(*d, h, n * sizeof(T), hipMemcpyHostToDevice, t->st));
+    return IKD_OK;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+#define CHECK_T()                        \
+    do {                                 \
+        if (!t) return IKD_ERR_ARG;      \
+    } while (0)
+
+extern "C" {
+
+int ikd_create(float delete_param, float balance_param, float box_length, ikd_tree_t* out) {
+    if (!out) return IKD_ERR_ARG;
+    *out = nullptr;
+    ikd_tree_t t = new ikd_tree_s();
+    t->prm = Params{delete_param, balance_param, box_length};
+    if (hipStreamCreateWithFlags(&t->st, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&t->d_hdr, sizeof(DevHdr)) != hipSuccess ||
+        hipHostMalloc(&t->h_hdr, sizeof(DevHdr), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc(&t->d_root, sizeof(RootInfo)) != hipSuccess ||
+        hipHostMalloc(&t->h_root, sizeof(RootInfo), hipHostMallocDefault) != hipSuccess) {
+        delete t;
+        return IKD_ERR_HIP;
+    }
+    int rc = reset_pool(t);
+    if (rc == IKD_OK) rc = sync_hdr(t);
+    if (rc == IKD_OK) rc = read_root(t);
+    if (rc != IKD_OK) {
+        ikd_destroy(t);
+        return rc;
+    }
+    *out = t;
+    return IKD_OK;
+}
+
+int ikd_destroy(ikd_tree_t t) {
+    CHECK_T();
+    if (t->st) hipStreamSynchronize(t->st);
+    if (t->cap > 0) free_pool(&t->pool);
+    for (auto& b : t->b)
+        if (b.p) hipFree(b.p);
+    if (t->d_hdr) hipFree(t->d_hdr);
+    if (t->h_hdr) hipHostFree(t->h_hdr);
+    if (t->d_root) hipFree(t->d_root);
+    if (t->h_root) hipHostFree(t->h_root);
+    if (t->st) hipStreamDestroy(t->st);
+    delete t;
+    return IKD_OK;
+}
+
+int ikd_set_delete_criterion_param(ikd_tree_t t, float v) {
+    CHECK_T();
+    t->prm.delete_param = v;
+    return IKD_OK;
+}
+int ikd_set_balance_criterion_param(ikd_tree_t t, float v) {
+    CHECK_T();
+    t->prm.balance_param = v;
+    return IKD_OK;
+}
+int ikd_set_downsample_param(ikd_tree_t t, float v) {
+    CHECK_T();
+    t->prm.box_length = v;
+    return IKD_OK;
+}
+int ikd_initialize(ikd_tree_t t, float d, float b, float l) {
+    CHECK_T();
+    t->prm = Params{d, b, l};
+    return IKD_OK;
+}
+
+int ikd_build(ikd_tree_t t, const float* xyz, const uint32_t* ids, int64_t n) {
+    CHECK_T();
+    if (n < 0 || (n > 0 && !xyz) || n >= (1ll << 27)) return IKD_ERR_ARG;
+    CHK(reset_pool(t));
+    t->next_id = ids ? t->next_id : (uint32_t)n;
+    if (n > 0) {
+        float* dx;
+        uint32_t* di = nullptr;
+        CHK(upload(t, B_IO4, xyz, 3 * n, &dx));
+        if (ids) CHK(upload(t, B_IO2, ids, n, &di));
+        float4* pts;
+        int* segof;
+        CHK(ensure(t, B_PTS, n, &pts));
+        CHK(ensure(t, B_SEGOF, n, &segof));
+        launch_pack_points(dx, di, 0u, (int)n, pts, t->st);
+        launch_fill_i32(segof, 0, n, t->st);
+        int4 s0 = make_int4(0, (int)n - 1, ROOT, SENTINEL);
+        int d0 = 0;
+        int4* dseg;
+        int* dsegd;
+        CHK(upload(t, B_ISEG, &s0, 1, &dseg));
+        CHK(upload(t, B_ISEGD, &d0, 1, &dsegd));
+        HIPCHK(hipStreamSynchronize(t->st));
+        CHK(run_build(t, pts, segof, (int)n, dseg, dsegd, 1, (int)n));
+        hipLaunchKernelGGL(k_link_bits, dim3(1), dim3(1), 0, t->st, t->pool.hot, SENTINEL, LK_HASL, LK_TDL);
+    }
+    CHK(sync_hdr(t));
+    return read_root(t);
+}
+
+int ikd_nearest_search_device(ikd_tree_t t, const float* d_q, int64_t nq, int k, double max_dist, float* d_xyz,
+                              float* d_d2, uint32_t* d_ids, int32_t* d_cnt) {
+    CHECK_T();
+    if (nq < 0 || nq >= (1ll << 31)) return IKD_ERR_ARG;
+    return knn_dev(t, d_q, (int)nq, k, max_dist, d_xyz, d_d2, d_ids, d_cnt, nullptr, nullptr);
+}
+
+int ikd_nearest_search(ikd_tree_t t, const float* q, int64_t nq, int k, double max_dist, float* out_xyz,
+                       float* out_d2, uint32_t* out_ids, int32_t* out_counts) {
+    CHECK_T();
+    if (nq < 0 || nq >= (1ll << 31) || (nq > 0 && (!q || !out_xyz || !out_d2 || !out_ids || !out_counts)))
+        return IKD_ERR_ARG;
+    if (k < 1) {
+        t->err = "Nearest_Search: k must be >= 1";
+        return IKD_ERR_ARG;
+    }
+    if (nq == 0) return IKD_OK;
+    float *dq, *dx, *dd;
+    uint32_t* di;
+    int* dc;
+    CHK(upload(t, B_IO0, q, 3 * nq, &dq));
+    CHK(ensure(t, B_IO1, 3 * nq * k, &dx));
+    CHK(ensure(t, B_IO2, nq * k, &dd));
+    CHK(ensure(t, B_IO3, nq * k, &di));
+    CHK(ensure(t, B_IO4, nq, &dc));
+    CHK(knn_dev(t, dq, (int)nq, k, max_dist, dx, dd, di, dc, nullptr, nullptr));
+    HIPCHK(hipMemcpyAsync(out_xyz, dx, 3 * nq * k * sizeof(float), hipMemcpyDeviceToHost, t->st));
+    HIPCHK(hipMemcpyAsync(out_d2, dd, nq * k * sizeof(float), hipMemcpyDeviceToHost, t->st));
+    HIPCHK(hipMemcpyAsync(out_ids, di, nq * k * sizeof(uint32_t), hipMemcpyDeviceToHost, t->st));
+    HIPCHK(hipMemcpyAsync(out_counts, dc, nq * sizeof(int32_t), hipMemcpyDeviceToHost, t->st));
+    return sync_hdr(t);
+}
+
+int ikd_nearest_search_counters(ikd_tree_t t, const float* q, int64_t nq, int k, double max_dist,
+                                uint32_t* out_visits, uint32_t* out_boxes) {
+    CHECK_T();
+    if (nq < 0 || nq >= (1ll << 31) || (nq > 0 && (!q || !out_visits || !out_boxes))) return IKD_ERR_ARG;
+    if (k < 1) return IKD_ERR_ARG;
+    if (nq == 0) return IKD_OK;
+    float *dq, *dx, *dd;
+    uint32_t *di, *dv, *db;
+    int* dc;
+    CHK(upload(t, B_IO0, q, 3 * nq, &dq));
+    CHK(ensure(t, B_IO1, 3 * nq * k, &dx));
+    CHK(ensure(t, B_IO2, nq * k, &dd));
+    CHK(ensure(t, B_IO3, nq * k, &di));
+    CHK(ensure(t, B_IO4, nq, &dc));
+    CHK(ensure(t, B_DELBOX, nq, &dv));
+    CHK(ensure(t, B_DELFL, nq, &db));
+    CHK(knn_dev(t, dq, (int)nq, k, max_dist, dx, dd, di, dc, dv, db));
+    HIPCHK(hipMemcpyAsync(out_visits, dv, nq * sizeof(uint32_t), hipMemcpyDeviceToHost, t->st));
+    HIPCHK(hipMemcpyAsync(out_boxes, db, nq * sizeof(uint32_t), hipMemcpyDeviceToHost, t->st));
+    return sync_hdr(t);
+}
+
+int ikd_box_search(ikd_tree_t t, const float* boxes, int64_t nb, int64_t* out_offsets, float* out_xyz,
+                   uint32_t* out_ids, int64_t capacity) {
+    CHECK_T();
+    if (nb < 0 || nb >= (1ll << 31) || !out_offsets || (nb > 0 && !boxes)) return IKD_ERR_ARG;
+    if (nb == 0) {
+        out_offsets[0] = 0;
+        return IKD_OK;
+    }
+    float* db;
+    CHK(upload(t, B_IO4, boxes, 6 * nb, &db));
+    return range_dev(t, 0, db, nullptr, nullptr, (int)nb, out_offsets, out_xyz, out_ids, capacity);
+}
+
+int ikd_radius_search(ikd_tree_t t, const float* centers, const float* radii, int64_t nq, int64_t* out_offsets,
+                      float* out_xyz, uint32_t* out_ids, int64_t capacity) {
+    CHECK_T();
+    if (nq < 0 || nq >= (1ll << 31) || !out_offsets || (nq > 0 && (!centers || !radii))) return IKD_ERR_ARG;
+    if (nq == 0) {
+        out_offsets[0] = 0;
+        return IKD_OK;
+    }
+    float *dc, *dr;
+    CHK(upload(t, B_IO4, centers, 3 * nq, &dc));
+    CHK(upload(t, B_IO3, radii, nq, &dr));
+    return range_dev(t, 1, nullptr, dc, dr, (int)nq, out_offsets, out_xyz, out_ids, capacity);
+}
+
+int ikd_add_points_device(ikd_tree_t t, const float* d_xyz, const uint32_t* d_ids, int64_t n, int downsample_on,
+                          int32_t* out_added) {
+    CHECK_T();
+    if (n < 0 || n >= (1ll << 27) || (n > 0 && !d_xyz)) return IKD_ERR_ARG;
+    int added = 0;
+    int rc = add_points_dev(t, d_xyz, d_ids, (int)n, downsample_on, &added);
+    if (out_added) *out_added = added;
+    if (rc != IKD_OK) return rc;
+    return read_root(t);
+}
+
+int ikd_add_points(ikd_tree_t t, const float* xyz, const uint32_t* ids, int64_t n, int downsample_on,
+                   int32_t* out_added) {
+    CHECK_T();
+    if (n < 0 || n >= (1ll << 27) || (n > 0 && !xyz)) return IKD_ERR_ARG;
+    if (out_added) *out_added = 0;
+    if (n == 0) return IKD_OK;
+    float* dx;
+    uint32_t* di = nullptr;
+    CHK(upload(t, B_ADDX, xyz, 3 * n, &dx));
+    if (ids) CHK(upload(t, B_ADDI, ids, n, &di));
+    HIPCHK(hipStreamSynchronize(t->st));
+    return ikd_add_points_device(t, dx, di, n, downsample_on, out_added);
+}
+
+int ikd_delete_points(ikd_tree_t t, const float* xyz, int64_t n) {
+    CHECK_T();
+    if (n < 0 || n >= (1ll << 31) || (n > 0 && !xyz)) return IKD_ERR_ARG;
+    if (n == 0) return IKD_OK;
+    float* dx;
+    int *cand, *seeds;
+    CHK(upload(t, B_IO4, xyz, 3 * n, &dx));
+    CHK(ensure(t, B_IO0, n, &cand));
+    CHK(ensure(t, B_SEEDS, n, &seeds));
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, t->st, t->d_hdr, 4);
+    launch_delete_points_find(t->pool, dx, (int)n, cand, t->d_hdr, t->st);
+    CHK(sync_hdr(t));
+    if (t->h_hdr->conflicts == 0) {
+        launch_delete_points_apply(t->pool, cand, (int)n, seeds, t->st);
+    } else {
+        launch_delete_points_unclaim(t->pool, cand, (int)n, seeds, t->st);
+        launch_delete_points_serial(t->pool, dx, (int)n, seeds, t->st);
+    }
+    CHK(refit(t, seeds, (int)n, 0));
+    CHK(rebuild_tops(t));
+    CHK(compact_if_needed(t, false));
+    return read_root(t);
+}
+
+int ikd_delete_point_boxes(ikd_tree_t t, const float* boxes, int64_t nb, int32_t* out_deleted) {
+    CHECK_T();
+    if (nb < 0 || nb >= (1ll << 31) || (nb > 0 && !boxes)) return IKD_ERR_ARG;
+    if (out_deleted) *out_deleted = 0;
+    if (nb == 0) return IKD_OK;
+    CHK(read_root(t));
+    long long before = live_count(t);
+    float* db;
+    CHK(upload(t, B_IO4, boxes, 6 * nb, &db));
+    CHK(delete_boxes_dev(t, db, (int)nb, 0));
+    CHK(compact_if_needed(t, false));
+    CHK(read_root(t));
+    if (out_deleted) *out_deleted = (int32_t)(before - live_count(t));
+    return IKD_OK;
+}
+
+int ikd_size(ikd_tree_t t, int64_t* out) {
+    CHECK_T();
+    if (!out) return IKD_ERR_ARG;
+    CHK(read_root(t));
+    *out = has_root(t) ? t->h_root->cnt.x : 0;
+    return IKD_OK;
+}
+
+int ikd_validnum(ikd_tree_t t, int64_t* out) {
+    CHECK_T();
+    if (!out) return IKD_ERR_ARG;
+    CHK(read_root(t));
+    *out = live_count(t);
+    return IKD_OK;
+}
+
+int ikd_tree_range(ikd_tree_t t, float* out6) {
+    CHECK_T();
+    if (!out6) return IKD_ERR_ARG;
+    CHK(read_root(t));
+    for (int i = 0; i < 6; ++i) out6[i] = has_root(t) ? t->h_root->box[i] : 0.0f;
+    return IKD_OK;
+}
+
+// Update's root statistics, cpp:1315-1321, evaluated on the current root.
+int ikd_root_alpha(ikd_tree_t t, float* alpha_bal, float* alpha_del) {
+    CHECK_T();
+    if (!alpha_bal || !alpha_del) return IKD_ERR_ARG;
+    CHK(read_root(t));
+    *alpha_bal = 0.0f;
+    *alpha_del = 0.0f;
+    if (!has_root(t) || t->h_root->cnt.x <= 3) return IKD_OK;
+    int size = t->h_root->cnt.x;
+    int son = t->h_root->child.x >= 0 ? t->h_root->child.x : t->h_root->child.y;
+    float bal = float(son) / (size - 1);
+    *alpha_del = float(t->h_root->cnt.y) / size;
+    *alpha_bal = (bal >= 0.5 - EPSS) ? bal : 1 - bal;
+    return IKD_OK;
+}
+
+int ikd_flatten(ikd_tree_t t, float* out_xyz, uint32_t* out_ids, int64_t capacity, int64_t* out_n) {
+    CHECK_T();
+    if (!out_n) return IKD_ERR_ARG;
+    CHK(read_root(t));
+    CHK(sync_hdr(t));
+    long long live = live_count(t);
+    *out_n = live;
+    if (live == 0) return IKD_OK;
+    if (!out_xyz) return IKD_OK;
+    if (capacity < live) {
+        t->err = "flatten: capacity below validnum()";
+        return IKD_ERR_CAPACITY;
+    }
+    float4* pts;
+    int* segof;
+    std::vector<int> roots = {ROOT};
+    CHK(enumerate_live(t, roots, t->h_root->cnt.x, live, &pts, &segof));
+    float* dx;
+    uint32_t* di;
+    CHK(ensure(t, B_IO0, 3 * live, &dx));
+    CHK(ensure(t, B_IO1, live, &di));
+    hipLaunchKernelGGL(k_unpack_points, dim3(nblk(live)), dim3(BS), 0, t->st, pts, (int)live, dx, di);
+    HIPCHK(hipMemcpyAsync(out_xyz, dx, 3 * live * sizeof(float), hipMemcpyDeviceToHost, t->st));
+    if (out_ids) HIPCHK(hipMemcpyAsync(out_ids, di, live * sizeof(uint32_t), hipMemcpyDeviceToHost, t->st));
+    return sync_hdr(t);
+}
+
+int ikd_synchronize(ikd_tree_t t) {
+    CHECK_T();
+    return sync_hdr(t);
+}
+
+int ikd_stream(ikd_tree_t t, void** out) {
+    CHECK_T();
+    if (!out) return IKD_ERR_ARG;
+    *out = (void*)t->st;
+    return IKD_OK;
+}
+
+int ikd_pool_stats(ikd_tree_t t, int64_t* used, int64_t* capacity, int32_t* max_depth, int64_t* rebuilds) {
+    CHECK_T();
+    CHK(sync_hdr(t));
+    if (used) *used = t->pool_used;
+    if (capacity) *capacity = t->cap;
+    if (max_depth) *max_depth = t->max_depth;
+    if (rebuilds) *rebuilds = t->rebuilds;
+    return IKD_OK;
+}
+
+const char* ikd_last_error(ikd_tree_t t) {
+    if (!t) return "null tree handle";
+    return t->err.c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,182 @@
+// ikd_internal.h — declarations shared between the kernel translation units
+// and the host orchestration (not part of the public C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ikd_common.h"
+
+namespace ikd {
+
+// Small device-resident header: counters the kernels advance and the host
+// reads back when it needs a size.
+struct DevHdr {
+    int pair_used;      // next free sibling pair
+    int max_depth;      // deepest node written so far
+    int seg_count[2];   // build: active segments at the current / next level
+    int dirty_count;    // refit: nodes on modified paths
+    int viol_count;     // Criterion_Check violators found by the refit
+    int top_count;      // topmost violators (subtrees to rebuild)
+    int rec_count;      // delete records (box traversal) / enumerate output
+    int front_count[2]; // enumerate: frontier sizes
+    int conflicts;      // Delete_Points lanes that hit the same node
+    int group_count;    // insert: distinct attach slots
+    int new_pairs;      // insert: pairs allocated for leaf attach points
+    int split_index;    // downsample: first batch index that needs ordered handling
+    int overflow;       // a traversal stack overflowed (must stay 0)
+    int pair_pending;   // build: pairs taken by the level just written
+    int front3[3];      // enumerate: rotating frontier counters
+    int added;          // downsample: Add_Points return value of the batch
+    int pad[2];
+};
+
+struct Pool {
+    NodeHot* hot;
+    uint32_t* ids;
+    int32_t* par;
+    int4* cnt;  // TreeSize, invalid_point_num, down_del_num, depth
+    uint32_t* meta;
+};
+
+struct Params {
+    float delete_param;
+    float balance_param;
+    float box_length;
+};
+
+// ---------------------------------------------------------------- build
+struct BuildLevel {
+    const float4* pts;     // x, y, z, id bits
+    const uint32_t* perm[3];
+    uint32_t* perm_next[3];
+    const int* sid;        // active segment per position, -1 when placed
+    int* sid_next;
+    uint8_t* side;         // per point: 0 left, 1 right, 2 pivot
+    const int4* seg;       // l, r, node, parent
+    const int* seg_depth;
+    int4* seg_next;
+    int* seg_depth_next;
+    int* seg_axis;
+    int* seg_mid;
+    int2* seg_child;       // next-level indices of the two children, -1 if none
+    unsigned long long* seg_flags;  // scan input: pairs | children<<32
+    unsigned long long* seg_scan;
+    unsigned long long* pos_flags;  // 3m: isL | isR<<32
+    unsigned long long* pos_scan;
+    int m;
+    int cur;               // 0/1: which DevHdr::seg_count holds this level's count
+};
+
+void launch_build_keys(const float4* pts, const int* seg_of, int m, int axis, unsigned long long* keys,
+                       uint32_t* vals, hipStream_t s);
+void launch_build_init_sid(const unsigned long long* sorted_keys0, int m, int* sid, hipStream_t s);
+void launch_build_seg_a(const BuildLevel& L, int bound, DevHdr* hdr, hipStream_t s);
+void launch_build_seg_b(const BuildLevel& L, int bound, Pool pool, DevHdr* hdr, hipStream_t s);
+void launch_build_side(const BuildLevel& L, DevHdr* hdr, hipStream_t s);
+void launch_build_flags(const BuildLevel& L, hipStream_t s);
+void launch_build_scatter(const BuildLevel& L, hipStream_t s);
+
+// ---------------------------------------------------------------- refit
+void launch_mark_dirty(Pool pool, const int* seeds, const int* nseeds_dev, int bound, int stop_at_tdel,
+                       int* dirty, DevHdr* hdr, hipStream_t s);
+void launch_refit_level(Pool pool, Params prm, const int* dirty, DevHdr* hdr, int depth, int* viol,
+                        hipStream_t s);
+void launch_find_top(Pool pool, const int* viol, int bound, DevHdr* hdr, int* top, hipStream_t s);
+void launch_clear_viol(Pool pool, const int* viol, int bound, DevHdr* hdr, hipStream_t s);
+
+// ---------------------------------------------------------------- enumerate
+void launch_enum_level(Pool pool, const int2* front, int lvl, int2* front_next, unsigned long long* out,
+                       DevHdr* hdr, int bound, int out_cap, hipStream_t s);
+void launch_gather_points(Pool pool, const unsigned long long* keys, int n, float4* out, hipStream_t s);
+
+// ---------------------------------------------------------------- insert
+void launch_insert_descend(Pool pool, const float4* pts, int n, unsigned long long* keys, uint32_t* vals,
+                           DevHdr* hdr, hipStream_t s);
+void launch_insert_groups(const unsigned long long* sorted_keys, int n, int* grp_flag, hipStream_t s);
+void launch_insert_segments2(Pool pool, const unsigned long long* keys, const int* gscan, const int* gflag, int n,
+                             int4* seg, int* seg_depth, int* seg_of, int* attach, int* pair_flag, DevHdr* hdr,
+                             hipStream_t s);
+void launch_insert_pairs(Pool pool, const int* attach, const int* pair_flag, const int* pair_scan, DevHdr* hdr,
+                         int bound, int4* seg, hipStream_t s);
+void launch_insert_link(Pool pool, const int* attach, const int4* seg, DevHdr* hdr, int bound, hipStream_t s);
+
+// ---------------------------------------------------------------- deletes
+void launch_box_delete_traverse(Pool pool, const float* boxes, int nb, unsigned int* recs, DevHdr* hdr, int cap,
+                                long long max_steps, hipStream_t s);
+void launch_box_delete_apply(Pool pool, const unsigned int* recs, DevHdr* hdr, int bound, int ds, int* seeds,
+                             hipStream_t s);
+void launch_delete_points_find(Pool pool, const float* xyz, int n, int* cand, DevHdr* hdr, hipStream_t s);
+void launch_delete_points_apply(Pool pool, const int* cand, int n, int* seeds, hipStream_t s);
+void launch_delete_points_unclaim(Pool pool, const int* cand, int n, int* seeds, hipStream_t s);
+void launch_delete_points_serial(Pool pool, const float* xyz, int n, int* seeds, hipStream_t s);
+
+// ---------------------------------------------------------------- downsample
+void launch_voxel_keys2(const float* xyz, const uint32_t* ids, uint32_t id_base, int n, float ds,
+                        unsigned long long* keys, uint32_t* vals, float4* pts, int* bad, hipStream_t s);
+void launch_voxel_flags(const unsigned long long* keys, int n, int* flag, hipStream_t s);
+void launch_voxel_starts(const int* flag, const int* scan, int n, int* vstart, DevHdr* hdr, hipStream_t s);
+void launch_voxel_fold2(Pool pool, const float4* pts, const uint32_t* order, const unsigned long long* keys,
+                        const int* vstart, int n, int bound, float ds, int check, float* del_boxes, int* del_flag,
+                        float4* ins_pts, int* ins_flag, DevHdr* hdr, long long max_steps, hipStream_t s);
+void launch_compact_boxes(const int* flag, const int* scan, const float* in, int bound, DevHdr* hdr, float* out,
+                          int* out_count, hipStream_t s);
+void launch_compact_pts(const int* flag, const int* scan, const float4* in, int bound, DevHdr* hdr, float4* out,
+                        int* out_count, hipStream_t s);
+
+// ---------------------------------------------------------------- queries
+struct KnnArgs {
+    const NodeHot* hot;
+    const uint32_t* ids;
+    const float* q;         // packed xyz
+    const uint32_t* order;  // optional query permutation (Morton order), may be null
+    int nq;
+    int k;
+    float thr;              // largest float <= max_dist^2 (double)
+    float* out_xyz;
+    float* out_d2;
+    uint32_t* out_ids;
+    int* out_counts;
+    unsigned int* visits;   // optional instrumentation: visited nodes per query
+    unsigned int* boxes;    // optional: child boxes evaluated per query
+    int* overflow;
+    int max_steps;          // visit bound per query (the pool size): a corrupt tree cannot hang a lane
+};
+int launch_knn(const KnnArgs& a, int stack, int block, hipStream_t s);
+
+void launch_morton(const float* q, int nq, const NodeHot* hot, unsigned long long* keys, uint32_t* vals,
+                   hipStream_t s);
+
+struct RangeArgs {
+    Pool pool;
+    const float* boxes;   // box search: 6 floats per query
+    const float* centers; // radius search: xyz per query
+    const float* radii;
+    int nq;
+    int mode;             // 0 box, 1 radius
+    const long long* offsets;  // fill pass: output offsets (nq+1)
+    long long* counts;    // count pass output
+    float* out_xyz;
+    uint32_t* out_ids;
+    int* overflow;
+    long long max_steps;
+};
+void launch_range(const RangeArgs& a, int fill, hipStream_t s);
+
+// ---------------------------------------------------------------- misc
+void launch_fill_i32(int* p, int v, long long n, hipStream_t s);
+void launch_pack_points(const float* xyz, const uint32_t* ids, uint32_t id_base, int n, float4* out,
+                        hipStream_t s);
+
+// ---------------------------------------------------------------- primitives (rocPRIM)
+hipError_t prim_sort_pairs_u64(void* tmp, size_t& tmp_bytes, const unsigned long long* kin,
+                               unsigned long long* kout, const uint32_t* vin, uint32_t* vout, int n,
+                               int begin_bit, int end_bit, hipStream_t s);
+hipError_t prim_sort_keys_u64(void* tmp, size_t& tmp_bytes, const unsigned long long* kin,
+                              unsigned long long* kout, int n, int begin_bit, int end_bit, hipStream_t s);
+hipError_t prim_exscan_u64(void* tmp, size_t& tmp_bytes, const unsigned long long* in,
+                           unsigned long long* out, int n, hipStream_t s);
+hipError_t prim_exscan_i32(void* tmp, size_t& tmp_bytes, const int* in, int* out, int n, hipStream_t s);
+hipError_t prim_exscan_i64(void* tmp, size_t& tmp_bytes, const long long* in, long long* out, int n,
+                           hipStream_t s);
+
+}  // namespace ikd

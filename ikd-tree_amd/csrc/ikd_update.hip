@@ -1,0 +1,603 @@
+// ikd_update.hip — device kernels for the update path: lazy delete labels,
+// bottom-up refit (Update), Criterion_Check, subtree enumeration (flatten),
+// batched insertion (Add_by_point), box and point deletion, and the voxel
+// downsample fold of Add_Points.
+//
+// Lazy labels.  As in the reference (Delete_by_range, cpp:656-669), a box
+// that covers a whole subtree only marks the subtree root tree_deleted; the
+// nodes below keep stale labels and are never read by a traversal, because
+// every traversal stops at a tree_deleted node.  The only walk that enters a
+// deleted subtree is insertion (Add_by_point descends through it, cpp:830-
+// 858); it pushes the labels down along its path exactly as Push_Down does
+// (cpp:1109-1181), so the refit above the new leaf sees correct children.
+#include "ikd_internal.h"
+
+namespace ikd {
+
+static inline int blocks_for(long long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+__device__ __forceinline__ uint32_t* link_ptr(NodeHot* hot, int n) {
+    return reinterpret_cast<uint32_t*>(&hot[n].f[3]);
+}
+__device__ __forceinline__ uint32_t load_link(const NodeHot* hot, int n) { return fbits(hot[n].f[3]); }
+
+__device__ __forceinline__ void write_child_box(NodeHot* hot, int parent, int side, const float* b) {
+    float* f = hot[parent].f;
+    if (side == 0) {
+        *reinterpret_cast<float4*>(f + 4) = make_float4(b[0], b[1], b[2], b[3]);
+        *reinterpret_cast<float2*>(f + 8) = make_float2(b[4], b[5]);
+    } else {
+        *reinterpret_cast<float2*>(f + 10) = make_float2(b[0], b[1]);
+        *reinterpret_cast<float4*>(f + 12) = make_float4(b[2], b[3], b[4], b[5]);
+    }
+}
+
+// std::min / std::max as the reference's Update uses them (cpp:1204-1297):
+// min(a,b) = b<a ? b : a, max(a,b) = a<b ? b : a.
+__device__ __forceinline__ float smin(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float smax(float a, float b) { return a < b ? b : a; }
+
+// ============================================================ refit / Update
+__global__ void k_mark_dirty(Pool pool, const int* __restrict__ seeds, const int* nseeds_dev, int bound,
+                             int stop_at_tdel, int* __restrict__ dirty, DevHdr* hdr) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= bound) return;
+    if (nseeds_dev && i >= *nseeds_dev) return;
+    int v = seeds[i];
+    if (v < ROOT) return;
+    if (stop_at_tdel) {
+        // a seed under a subtree that this batch deleted whole is hidden:
+        // refitting it (or its covered ancestor) would read stale labels.
+        int u = v;
+        for (int step = 0;; ++step) {
+            if (pool.meta[u] & M_TDEL) return;
+            if (u == ROOT) break;
+            u = pool.par[u];
+            if (step > WALK_LIMIT || u < ROOT) {
+                atomicOr(&hdr->overflow, 16);
+                return;
+            }
+        }
+    }
+    int u = v;
+    for (int step = 0;; ++step) {
+        uint32_t old = atomicOr(&pool.meta[u], M_DIRTY);
+        if (old & M_DIRTY) break;
+        int pos = atomicAdd(&hdr->dirty_count, 1);
+        dirty[pos] = u;
+        if (u == ROOT) break;
+        u = pool.par[u];
+        if (step > WALK_LIMIT || u < ROOT) {
+            atomicOr(&hdr->overflow, 16);
+            return;
+        }
+    }
+}
+
+// Update (cpp:1183-1323) of one node from its children, then
+// Criterion_Check (cpp:1089-1107).
+__device__ void refit_node(Pool pool, Params prm, int u, int* viol, DevHdr* hdr) {
+    const NodeHot& h = pool.hot[u];
+    uint32_t lk = fbits(h.f[3]);
+    uint32_t m = pool.meta[u];
+    bool pdel = (m & M_PDEL) != 0, pds = (m & M_PDS) != 0;
+    int size = 1, inv = pdel ? 1 : 0, dd = pds ? 1 : 0;
+    bool all_td = true, all_tds = true, any_td = false;
+    int pair = (int)lk_pair(lk);
+    bool has[2] = {(lk & LK_HASL) != 0, (lk & LK_HASR) != 0};
+    bool ctd[2] = {false, false};
+    int csize[2] = {0, 0};
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        if (!has[s]) continue;
+        int c = 2 * pair + s;
+        int4 cc = pool.cnt[c];
+        uint32_t mc = pool.meta[c];
+        size += cc.x;
+        inv += cc.y;
+        dd += cc.z;
+        csize[s] = cc.x;
+        ctd[s] = (mc & M_TDEL) != 0;
+        all_td = all_td && ctd[s];
+        all_tds = all_tds && ((mc & M_TDS) != 0);
+        any_td = any_td || ctd[s];
+    }
+    bool tdel = all_td && pdel;
+    bool tds = all_tds && pds;
+    bool everything = tdel || (!any_td && !pdel);
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        if (has[s] && (everything || !ctd[s])) {
+            const float* b = h.f + 4 + 6 * s;
+            lo[0] = smin(lo[0], b[0]); hi[0] = smax(hi[0], b[3]);
+            lo[1] = smin(lo[1], b[1]); hi[1] = smax(hi[1], b[4]);
+            lo[2] = smin(lo[2], b[2]); hi[2] = smax(hi[2], b[5]);
+        }
+    }
+    if (everything || !pdel) {
+        lo[0] = smin(lo[0], h.f[0]); hi[0] = smax(hi[0], h.f[0]);
+        lo[1] = smin(lo[1], h.f[1]); hi[1] = smax(hi[1], h.f[1]);
+        lo[2] = smin(lo[2], h.f[2]); hi[2] = smax(hi[2], h.f[2]);
+    }
+    int depth = pool.cnt[u].w;
+    pool.cnt[u] = make_int4(size, inv, dd, depth);
+    uint32_t nm = (m & ~(M_TDEL | M_TDS | M_DIRTY | M_VIOL | M_TOP)) | (tdel ? M_TDEL : 0u) | (tds ? M_TDS : 0u);
+    if (size > MIN_UNBALANCED_SIZE) {
+        int son = has[0] ? csize[0] : csize[1];
+        float del_eval = (float)inv / (float)size;
+        float bal_eval = (float)son / (float)(size - 1);
+        if (del_eval > prm.delete_param || bal_eval > prm.balance_param || bal_eval < 1.0f - prm.balance_param) {
+            nm |= M_VIOL;
+            int pos = atomicAdd(&hdr->viol_count, 1);
+            viol[pos] = u;
+        }
+    }
+    pool.meta[u] = nm;
+    int p = pool.par[u];
+    float box[6] = {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]};
+    write_child_box(pool.hot, p, u & 1, box);
+    uint32_t tdbit = lk_td(u & 1);
+    if (tdel) atomicOr(link_ptr(pool.hot, p), tdbit);
+    else atomicAnd(link_ptr(pool.hot, p), ~tdbit);
+}
+
+__global__ void k_refit_level(Pool pool, Params prm, const int* __restrict__ dirty, DevHdr* hdr, int depth,
+                              int* __restrict__ viol) {
+    int n = hdr->dirty_count;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int u = dirty[i];
+        if (pool.cnt[u].w != depth) continue;
+        refit_node(pool, prm, u, viol, hdr);
+    }
+}
+
+// A violator is rebuilt only if no ancestor violates too: the reference
+// checks bottom-up on the way out of the recursion and ends up rebuilding
+// the highest violating node of the path (cpp:862-863, 706-707).
+__global__ void k_find_top(Pool pool, const int* __restrict__ viol, DevHdr* hdr, int* __restrict__ top) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= hdr->viol_count) return;
+    int v = viol[i];
+    int u = v;
+    for (int step = 0; u != ROOT; ++step) {
+        u = pool.par[u];
+        if (step > WALK_LIMIT || u < ROOT) {
+            atomicOr(&hdr->overflow, 16);
+            return;
+        }
+        if (pool.meta[u] & M_VIOL) return;
+    }
+    atomicOr(&pool.meta[v], M_TOP);
+    int pos = atomicAdd(&hdr->top_count, 1);
+    top[pos] = v;
+}
+
+__global__ void k_clear_viol(Pool pool, const int* __restrict__ viol, DevHdr* hdr) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= hdr->viol_count) return;
+    atomicAnd(&pool.meta[viol[i]], ~M_VIOL);
+}
+
+void launch_mark_dirty(Pool pool, const int* seeds, const int* nseeds_dev, int bound, int stop_at_tdel,
+                       int* dirty, DevHdr* hdr, hipStream_t s) {
+    if (bound <= 0) return;
+    hipLaunchKernelGGL(k_mark_dirty, dim3(blocks_for(bound, 256)), dim3(256), 0, s, pool, seeds, nseeds_dev, bound,
+                       stop_at_tdel, dirty, hdr);
+}
+void launch_refit_level(Pool pool, Params prm, const int* dirty, DevHdr* hdr, int depth, int* viol,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_refit_level, dim3(512), dim3(256), 0, s, pool, prm, dirty, hdr, depth, viol);
+}
+void launch_find_top(Pool pool, const int* viol, int bound, DevHdr* hdr, int* top, hipStream_t s) {
+    if (bound <= 0) return;
+    hipLaunchKernelGGL(k_find_top, dim3(blocks_for(bound, 256)), dim3(256), 0, s, pool, viol, hdr, top);
+}
+void launch_clear_viol(Pool pool, const int* viol, int bound, DevHdr* hdr, hipStream_t s) {
+    if (bound <= 0) return;
+    hipLaunchKernelGGL(k_clear_viol, dim3(blocks_for(bound, 256)), dim3(256), 0, s, pool, viol, hdr);
+}
+
+// ======================================================= enumerate (flatten)
+// Breadth-first walk from a set of subtree roots, stopping at tree_deleted
+// nodes; emits (segment << 32 | node) for every live node.  Three rotating
+// frontier counters let a level reset the counter of the level after next.
+__global__ void k_enum_level(Pool pool, const int2* __restrict__ front, int lvl, int2* __restrict__ front_next,
+                             unsigned long long* __restrict__ out, DevHdr* hdr, int bound, int out_cap) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) hdr->front3[(lvl + 2) % 3] = 0;
+    if (i >= bound) return;
+    if (i >= hdr->front3[lvl % 3]) return;
+    int2 e = front[i];
+    int u = e.x;
+    uint32_t m = pool.meta[u];
+    if (m & M_TDEL) return;
+    if (!(m & M_PDEL)) {
+        int pos = atomicAdd(&hdr->rec_count, 1);
+        if (pos < out_cap) out[pos] = ((unsigned long long)(uint32_t)e.y << 32) | (uint32_t)u;
+    }
+    uint32_t lk = load_link(pool.hot, u);
+    int pair = (int)lk_pair(lk);
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        if ((lk & lk_has(s)) && !(lk & lk_td(s))) {
+            int pos = atomicAdd(&hdr->front3[(lvl + 1) % 3], 1);
+            if (pos < bound) front_next[pos] = make_int2(2 * pair + s, e.y);
+            else atomicOr(&hdr->overflow, 32);
+        }
+    }
+}
+
+__global__ void k_gather_points(Pool pool, const unsigned long long* __restrict__ keys, int n,
+                                float4* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int u = (int)(uint32_t)(keys[i] & 0xffffffffull);
+    float4 p = *reinterpret_cast<const float4*>(pool.hot[u].f);
+    out[i] = make_float4(p.x, p.y, p.z, bitsf(pool.ids[u]));
+}
+
+void launch_enum_level(Pool pool, const int2* front, int lvl, int2* front_next, unsigned long long* out,
+                       DevHdr* hdr, int bound, int out_cap, hipStream_t s) {
+    hipLaunchKernelGGL(k_enum_level, dim3(blocks_for(bound > 0 ? bound : 1, 256)), dim3(256), 0, s, pool, front, lvl,
+                       front_next, out, hdr, bound, out_cap);
+}
+void launch_gather_points(Pool pool, const unsigned long long* keys, int n, float4* out, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gather_points, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, keys, n, out);
+}
+
+// ================================================================== insert
+// Marks node u as wholly deleted, as Push_Down writes a child of a
+// tree_deleted parent (cpp:1125-1131).
+__device__ void push_deleted(Pool pool, int u, bool ds) {
+    uint32_t bits = M_PDEL | M_TDEL | (ds ? (M_PDS | M_TDS) : 0u);
+    atomicOr(&pool.meta[u], bits);
+    atomicOr(link_ptr(pool.hot, u), LK_PDEL);
+    int sz = pool.cnt[u].x;
+    pool.cnt[u].y = sz;
+    if (ds) pool.cnt[u].z = sz;
+}
+
+// Add_by_point's descent (cpp:830-858): p[axis] < node[axis] goes left,
+// everything else right, through deleted nodes too, down to an empty child
+// slot.  Emits (node << 1 | side) of that slot.
+__global__ void k_insert_descend(Pool pool, const float4* __restrict__ pts, int n,
+                                 unsigned long long* __restrict__ keys, uint32_t* __restrict__ vals, DevHdr* hdr) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 P = pts[i];
+    vals[i] = (uint32_t)i;
+    uint32_t lk0 = load_link(pool.hot, SENTINEL);
+    if (!(lk0 & LK_HASL)) {
+        keys[i] = 0ull;  // empty tree: the slot is the sentinel's left child (the root)
+        return;
+    }
+    int u = ROOT;
+    bool del = false, ds = false;
+    int side = 0;
+    for (int step = 0;; ++step) {
+        if (step > WALK_LIMIT) {
+            atomicOr(&hdr->overflow, 64);
+            break;
+        }
+        uint32_t m = pool.meta[u];
+        uint32_t lk = load_link(pool.hot, u);
+        if (del || (m & M_TDEL)) {
+            del = true;
+            ds = ds || (m & M_TDS);
+            push_deleted(pool, u, ds);
+            int pair = (int)lk_pair(lk);
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                if (lk & lk_has(s)) {
+                    push_deleted(pool, 2 * pair + s, ds);
+                    atomicOr(link_ptr(pool.hot, u), lk_td(s));
+                }
+            }
+        }
+        int axis = (int)(m & M_AXIS);
+        const float* f = pool.hot[u].f;
+        float pc = axis == 0 ? P.x : (axis == 1 ? P.y : P.z);
+        side = pc < f[axis] ? 0 : 1;
+        if (!(lk & lk_has(side))) break;
+        u = 2 * (int)lk_pair(lk) + side;
+    }
+    keys[i] = ((unsigned long long)(uint32_t)u << 1) | (unsigned long long)side;
+}
+
+__global__ void k_insert_groups(const unsigned long long* __restrict__ keys, int n, int* __restrict__ flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
+}
+
+// One segment per attach slot; a slot under a childless node needs a new
+// sibling pair (one per node even when both sides receive points).
+__global__ void k_insert_segments(Pool pool, const unsigned long long* __restrict__ keys,
+                                  const int* __restrict__ gscan, const int* __restrict__ gflag, int n,
+                                  int4* __restrict__ seg, int* __restrict__ seg_depth, int* __restrict__ seg_of,
+                                  int* __restrict__ attach, int* __restrict__ pair_flag, DevHdr* hdr) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int g = gscan[i] + gflag[i] - 1;
+    seg_of[i] = g;
+    unsigned long long key = keys[i];
+    int X = (int)(key >> 1);
+    int side = (int)(key & 1ull);
+    if (gflag[i]) {
+        seg[g].x = i;
+        seg[g].w = X;
+        attach[g] = (int)key;
+        seg_depth[g] = X == SENTINEL ? 0 : pool.cnt[X].w + 1;
+        uint32_t lk = load_link(pool.hot, X);
+        bool first_of_node = (i == 0) || ((int)(keys[i - 1] >> 1) != X);
+        pair_flag[g] = (lk_pair(lk) == 0 && first_of_node) ? 1 : 0;
+        (void)side;
+    }
+    if (i == n - 1 || keys[i + 1] != key) seg[g].y = i;
+    if (i == n - 1) hdr->group_count = g + 1;
+}
+
+__global__ void k_insert_pairs(Pool pool, const int* __restrict__ attach, const int* __restrict__ pair_flag,
+                               const int* __restrict__ pair_scan, DevHdr* hdr, int bound, int4* __restrict__ seg) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    int ng = hdr->group_count;
+    if (g == 0 && ng > 0) hdr->new_pairs = pair_scan[ng - 1] + pair_flag[ng - 1];
+    if (g >= bound || g >= ng) return;
+    int key = attach[g];
+    int X = key >> 1, side = key & 1;
+    uint32_t lk = load_link(pool.hot, X);
+    int pair = (int)lk_pair(lk);
+    if (pair == 0) pair = hdr->pair_used + (pair_flag[g] ? pair_scan[g] : pair_scan[g] - 1);
+    seg[g].z = 2 * pair + side;
+}
+
+__global__ void k_commit_pairs(DevHdr* hdr) {
+    hdr->pair_used += hdr->new_pairs;
+    hdr->new_pairs = 0;
+}
+
+// After the build: hook each new subtree into its slot.
+__global__ void k_insert_link(Pool pool, const int* __restrict__ attach, const int4* __restrict__ seg, DevHdr* hdr,
+                              int bound) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= bound || g >= hdr->group_count) return;
+    int key = attach[g];
+    int X = key >> 1, side = key & 1;
+    int slot = seg[g].z;
+    uint32_t bits = lk_has(side) | ((uint32_t)(slot >> 1) << LK_PAIR_SHIFT);
+    atomicOr(link_ptr(pool.hot, X), bits);
+    atomicAnd(link_ptr(pool.hot, X), ~lk_td(side));
+}
+
+void launch_insert_descend(Pool pool, const float4* pts, int n, unsigned long long* keys, uint32_t* vals,
+                           DevHdr* hdr, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_insert_descend, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, pts, n, keys, vals, hdr);
+}
+void launch_insert_groups(const unsigned long long* keys, int n, int* flag, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_insert_groups, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n, flag);
+}
+void launch_insert_segments2(Pool pool, const unsigned long long* keys, const int* gscan, const int* gflag, int n,
+                             int4* seg, int* seg_depth, int* seg_of, int* attach, int* pair_flag, DevHdr* hdr,
+                             hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_insert_segments, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, keys, gscan, gflag, n, seg,
+                       seg_depth, seg_of, attach, pair_flag, hdr);
+}
+void launch_insert_pairs(Pool pool, const int* attach, const int* pair_flag, const int* pair_scan, DevHdr* hdr,
+                         int bound, int4* seg, hipStream_t s) {
+    if (bound <= 0) return;
+    hipLaunchKernelGGL(k_insert_pairs, dim3(blocks_for(bound, 256)), dim3(256), 0, s, pool, attach, pair_flag,
+                       pair_scan, hdr, bound, seg);
+    hipLaunchKernelGGL(k_commit_pairs, dim3(1), dim3(1), 0, s, hdr);
+}
+void launch_insert_link(Pool pool, const int* attach, const int4* seg, DevHdr* hdr, int bound, hipStream_t s) {
+    if (bound <= 0) return;
+    hipLaunchKernelGGL(k_insert_link, dim3(blocks_for(bound, 256)), dim3(256), 0, s, pool, attach, seg, hdr, bound);
+}
+
+// ============================================================ box deletion
+// Delete_by_range (cpp:647-710) for many boxes at once, one lane per box.
+// Pass 1 records what each box deletes: COVER for a subtree whose AABB lies
+// inside the half-open box (the subtree root takes the lazy label), POINT
+// for a single live node point inside it.
+constexpr uint32_t REC_COVER = 0x80000000u;
+
+__global__ void k_box_delete_traverse(Pool pool, const float* __restrict__ boxes, int nb,
+                                      unsigned int* __restrict__ recs, DevHdr* hdr, int cap, long long max_steps) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    float box[6];
+#pragma unroll
+    for (int a = 0; a < 6; ++a) box[a] = boxes[6 * b + a];
+    uint32_t lk0 = load_link(pool.hot, SENTINEL);
+    if (!(lk0 & LK_HASL) || (lk0 & LK_TDL)) return;
+    auto rec = [&](uint32_t v) {
+        int pos = atomicAdd(&hdr->rec_count, 1);
+        if (pos < cap) recs[pos] = v;
+    };
+    const float* rb = pool.hot[SENTINEL].f + 4;
+    if (box_disjoint(box, rb)) return;
+    if (box_covers(box, rb)) {
+        rec(REC_COVER | (uint32_t)ROOT);
+        return;
+    }
+    int stack[2 * MAX_DEPTH + 4];
+    int sp = 0;
+    stack[sp++] = ROOT;
+    long long steps = 0;
+    while (sp > 0) {
+        if (++steps > max_steps) {
+            atomicOr(&hdr->overflow, 2);
+            break;
+        }
+        int u = stack[--sp];
+        const float* f = pool.hot[u].f;
+        uint32_t lk = fbits(f[3]);
+        if (!(lk & LK_PDEL) && box_contains(box, f[0], f[1], f[2])) rec((uint32_t)u);
+        int pair = (int)lk_pair(lk);
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            if (!(lk & lk_has(s)) || (lk & lk_td(s))) continue;
+            const float* cb = f + 4 + 6 * s;
+            if (box_disjoint(box, cb)) continue;
+            int c = 2 * pair + s;
+            if (box_covers(box, cb)) {
+                rec(REC_COVER | (uint32_t)c);
+            } else if (sp < 2 * MAX_DEPTH + 4) {
+                stack[sp++] = c;
+            } else {
+                atomicOr(&hdr->overflow, 2);
+            }
+        }
+    }
+}
+
+// Pass 2 applies the records (idempotent, so overlapping boxes are fine) and
+// names the refit seeds: the node itself for POINT, the parent for COVER.
+__global__ void k_box_delete_apply(Pool pool, const unsigned int* __restrict__ recs, DevHdr* hdr, int bound,
+                                   int ds, int* __restrict__ seeds) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= bound) return;
+    if (i >= hdr->rec_count) return;
+    uint32_t r = recs[i];
+    int u = (int)(r & ~REC_COVER);
+    if (r & REC_COVER) {
+        uint32_t bits = M_PDEL | M_TDEL | (ds ? (M_PDS | M_TDS) : 0u);
+        atomicOr(&pool.meta[u], bits);
+        atomicOr(link_ptr(pool.hot, u), LK_PDEL);
+        int sz = pool.cnt[u].x;
+        pool.cnt[u].y = sz;
+        if (ds) pool.cnt[u].z = sz;
+        int p = pool.par[u];
+        atomicOr(link_ptr(pool.hot, p), lk_td(u & 1));
+        seeds[i] = p;
+    } else {
+        atomicOr(&pool.meta[u], M_PDEL | (ds ? M_PDS : 0u));
+        atomicOr(link_ptr(pool.hot, u), LK_PDEL);
+        seeds[i] = u;
+    }
+}
+
+void launch_box_delete_traverse(Pool pool, const float* boxes, int nb, unsigned int* recs, DevHdr* hdr, int cap,
+                                long long max_steps, hipStream_t s) {
+    if (nb <= 0) return;
+    hipLaunchKernelGGL(k_box_delete_traverse, dim3(blocks_for(nb, 64)), dim3(64), 0, s, pool, boxes, nb, recs, hdr,
+                       cap, max_steps);
+}
+void launch_box_delete_apply(Pool pool, const unsigned int* recs, DevHdr* hdr, int bound, int ds, int* seeds,
+                             hipStream_t s) {
+    if (bound <= 0) return;
+    hipLaunchKernelGGL(k_box_delete_apply, dim3(blocks_for(bound, 256)), dim3(256), 0, s, pool, recs, hdr, bound, ds,
+                       seeds);
+}
+
+// ========================================================== point deletion
+// Delete_by_point (cpp:712-760): follow the insertion rule from the root,
+// stop at a missing or tree_deleted node, take the first live node that is
+// same_point() with the query.
+__device__ int find_point(Pool pool, float x, float y, float z) {
+    uint32_t lk0 = load_link(pool.hot, SENTINEL);
+    if (!(lk0 & LK_HASL)) return -1;
+    int u = ROOT;
+    for (int step = 0; step <= WALK_LIMIT; ++step) {
+        uint32_t m = pool.meta[u];
+        if (m & M_TDEL) return -1;
+        const float* f = pool.hot[u].f;
+        if (!(m & M_PDEL) && same_point(f[0], f[1], f[2], x, y, z)) return u;
+        uint32_t lk = fbits(f[3]);
+        int axis = (int)(m & M_AXIS);
+        float pc = axis == 0 ? x : (axis == 1 ? y : z);
+        int side = pc < f[axis] ? 0 : 1;
+        if (!(lk & lk_has(side))) return -1;
+        u = 2 * (int)lk_pair(lk) + side;
+    }
+    return -1;
+}
+
+__global__ void k_delete_points_find(Pool pool, const float* __restrict__ xyz, int n, int* __restrict__ cand,
+                                     DevHdr* hdr) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int u = find_point(pool, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+    cand[i] = u;
+    if (u >= 0) {
+        uint32_t old = atomicOr(&pool.meta[u], M_CLAIM);
+        if (old & M_CLAIM) atomicAdd(&hdr->conflicts, 1);
+    }
+}
+
+// mode 0: apply the (conflict-free) candidates; mode 1: only drop the claims.
+__global__ void k_delete_points_apply(Pool pool, const int* __restrict__ cand, int n, int* __restrict__ seeds,
+                                      int mode) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int u = cand[i];
+    seeds[i] = -1;
+    if (u < 0) return;
+    atomicAnd(&pool.meta[u], ~M_CLAIM);
+    if (mode == 0) {
+        atomicOr(&pool.meta[u], M_PDEL);
+        atomicOr(link_ptr(pool.hot, u), LK_PDEL);
+        seeds[i] = u;
+    }
+}
+
+// Ordered fallback when two points of one batch reach the same node: one
+// lane walks the batch in order, as the reference's loop does (cpp:514-531).
+__global__ void k_delete_points_serial(Pool pool, const float* __restrict__ xyz, int n, int* __restrict__ seeds) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int i = 0; i < n; ++i) {
+        int u = find_point(pool, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+        seeds[i] = u;
+        if (u >= 0) {
+            pool.meta[u] |= M_PDEL;
+            *link_ptr(pool.hot, u) |= LK_PDEL;
+            __threadfence();
+        }
+    }
+}
+
+void launch_delete_points_find(Pool pool, const float* xyz, int n, int* cand, DevHdr* hdr, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_delete_points_find, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, xyz, n, cand, hdr);
+}
+void launch_delete_points_apply(Pool pool, const int* cand, int n, int* seeds, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_delete_points_apply, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, cand, n, seeds, 0);
+}
+void launch_delete_points_unclaim(Pool pool, const int* cand, int n, int* seeds, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_delete_points_apply, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, cand, n, seeds, 1);
+}
+void launch_delete_points_serial(Pool pool, const float* xyz, int n, int* seeds, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_delete_points_serial, dim3(1), dim3(64), 0, s, pool, xyz, n, seeds);
+}
+
+// ================================================================ misc
+__global__ void k_fill_i32(int* p, int v, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+void launch_fill_i32(int* p, int v, long long n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, v, n);
+}
+
+__global__ void k_pack_points(const float* __restrict__ xyz, const uint32_t* __restrict__ ids, uint32_t base, int n,
+                              float4* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t id = ids ? ids[i] : base + (uint32_t)i;
+    out[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], bitsf(id));
+}
+void launch_pack_points(const float* xyz, const uint32_t* ids, uint32_t base, int n, float4* out, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_pack_points, dim3(blocks_for(n, 256)), dim3(256), 0, s, xyz, ids, base, n, out);
+}
+
+}  // namespace ikd

@@ -1,0 +1,298 @@
+// ikd_voxel.hip — the voxel downsample of Add_Points (ikd_Tree.cpp:422-448)
+// as one ordered fold per voxel.
+//
+// For every input point the reference takes the voxel box
+// lo = floor(p/ds)*ds, hi = lo+ds (float, cpp:424-429) and its centre
+// m = lo + (hi-lo)/2.0 (double, narrowed to float, cpp:430-432), collects the
+// live points S of the box (Search_by_range, pre-order), keeps the point
+// nearest to m (the new point unless some s is strictly nearer, the first
+// such s on ties), and when |S| > 1 or that point is same_point() with the new
+// point, deletes S (downsample labels) and inserts the kept point, counting one.
+//
+// When no point of the batch or of the map lies in the boxes of two different
+// voxels of the batch, voxels are independent and each one is a sequential
+// fold over its own points (in input order, kept by a stable sort): after its
+// first insertion a voxel holds exactly one live point c, and a later point p
+// replaces c iff dist(p,m) <= dist(c,m), or re-inserts c iff it is
+// same_point() with it.  Only the final state is materialised (one box delete
+// of the original S, one insert of c); the intermediate nodes the reference
+// creates and deletes again only change size().
+//
+// For power-of-two ds every box is exact and the boxes tile space, so the
+// independence holds by construction.  For other ds the boxes can overlap or
+// miss their own points by an ulp; the fold then reports the first input index
+// at which independence breaks and the host runs the batch in ordered chunks.
+#include "ikd_internal.h"
+
+namespace ikd {
+
+static inline int blocks_for(long long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+constexpr int VOX_BIAS = 1 << 20;
+
+__device__ __forceinline__ unsigned long long vox_key(int ix, int iy, int iz) {
+    return ((unsigned long long)(uint32_t)(ix + VOX_BIAS) << 42) | ((unsigned long long)(uint32_t)(iy + VOX_BIAS) << 21) |
+           (unsigned long long)(uint32_t)(iz + VOX_BIAS);
+}
+
+__global__ void k_voxel_keys(const float* __restrict__ xyz, const uint32_t* __restrict__ ids, uint32_t id_base, int n,
+                             float ds, unsigned long long* __restrict__ keys, uint32_t* __restrict__ vals,
+                             float4* __restrict__ pts, int* __restrict__ bad) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+    float fx = floorf(x / ds), fy = floorf(y / ds), fz = floorf(z / ds);
+    const float lim = (float)VOX_BIAS;
+    if (!(fabsf(fx) < lim && fabsf(fy) < lim && fabsf(fz) < lim)) {
+        atomicOr(bad, 1);
+        fx = fy = fz = 0.0f;
+    }
+    keys[i] = vox_key((int)fx, (int)fy, (int)fz);
+    vals[i] = (uint32_t)i;
+    uint32_t id = ids ? ids[i] : id_base + (uint32_t)i;
+    pts[i] = make_float4(x, y, z, bitsf(id));
+}
+
+__global__ void k_voxel_flags(const unsigned long long* __restrict__ keys, int n, int* __restrict__ flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
+}
+
+__global__ void k_voxel_starts(const int* __restrict__ flag, const int* __restrict__ scan, int n,
+                               int* __restrict__ vstart, DevHdr* hdr) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (flag[i]) vstart[scan[i]] = i;
+    if (i == n - 1) {
+        int nv = scan[i] + flag[i];
+        vstart[nv] = n;
+        hdr->group_count = nv;
+    }
+}
+
+// first sorted position of a key, or -1
+__device__ int find_key(const unsigned long long* keys, int n, unsigned long long k) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (keys[mid] < k) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < n && keys[lo] == k) ? lo : -1;
+}
+
+struct VoxGeo {
+    float fidx[3];  // floor(p/ds) of the voxel
+    float ds;
+};
+
+// Offsets o in {-1,0,1}^3 \ {0} whose voxel box also holds point (x,y,z);
+// returns the smallest first-input-index among such voxels present in the
+// batch (or INT_MAX).
+__device__ int foreign_voxel_first_index(const VoxGeo& g, float x, float y, float z,
+                                         const unsigned long long* keys, const uint32_t* order, int n) {
+    float p[3] = {x, y, z};
+    bool in[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+            float lo = (g.fidx[a] + (float)(o - 1)) * g.ds;
+            float hi = lo + g.ds;
+            in[a][o] = lo <= p[a] && p[a] < hi;
+        }
+    }
+    int best = 0x7fffffff;
+    for (int ox = 0; ox < 3; ++ox) {
+        if (!in[0][ox]) continue;
+        for (int oy = 0; oy < 3; ++oy) {
+            if (!in[1][oy]) continue;
+            for (int oz = 0; oz < 3; ++oz) {
+                if (!in[2][oz] || (ox == 1 && oy == 1 && oz == 1)) continue;
+                unsigned long long k = vox_key((int)g.fidx[0] + ox - 1, (int)g.fidx[1] + oy - 1, (int)g.fidx[2] + oz - 1);
+                int pos = find_key(keys, n, k);
+                if (pos >= 0) best = min(best, (int)order[pos]);
+            }
+        }
+    }
+    return best;
+}
+
+__global__ void k_voxel_fold(Pool pool, const float4* __restrict__ pts, const uint32_t* __restrict__ order,
+                             const unsigned long long* __restrict__ keys, const int* __restrict__ vstart, int n,
+                             int bound, float ds, int check, float* __restrict__ del_boxes, int* __restrict__ del_flag,
+                             float4* __restrict__ ins_pts, int* __restrict__ ins_flag, DevHdr* hdr,
+                             long long max_steps) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= bound || v >= hdr->group_count) return;
+    int j0 = vstart[v], j1 = vstart[v + 1];
+    float4 P0 = pts[order[j0]];
+    VoxGeo g;
+    g.ds = ds;
+    g.fidx[0] = floorf(P0.x / ds);
+    g.fidx[1] = floorf(P0.y / ds);
+    g.fidx[2] = floorf(P0.z / ds);
+    float box[6];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        box[a] = g.fidx[a] * ds;
+        box[3 + a] = box[a] + ds;
+    }
+    float mx = (float)((double)box[0] + (double)(box[3] - box[0]) / 2.0);
+    float my = (float)((double)box[1] + (double)(box[4] - box[1]) / 2.0);
+    float mz = (float)((double)box[2] + (double)(box[5] - box[2]) / 2.0);
+    int first_idx = (int)order[j0];
+    int split = 0x7fffffff;  // 2*index (+1 when the split is not a singleton)
+
+    // ---- S0: live map points in the box, pre-order (own, left, right) ----
+    int n0 = 0, best_s = -1;
+    float best_sd = INFINITY;
+    uint32_t lk0 = fbits(pool.hot[SENTINEL].f[3]);
+    if ((lk0 & LK_HASL) && !(lk0 & LK_TDL) && !box_disjoint(box, pool.hot[SENTINEL].f + 4)) {
+        int stack[2 * MAX_DEPTH + 4];
+        int sp = 0;
+        stack[sp++] = ROOT;
+        long long steps = 0;
+        while (sp > 0) {
+            if (++steps > max_steps) {
+                atomicOr(&hdr->overflow, 4);
+                break;
+            }
+            int u = stack[--sp];
+            const float* f = pool.hot[u].f;
+            uint32_t lk = fbits(f[3]);
+            if (!(lk & LK_PDEL) && box_contains(box, f[0], f[1], f[2])) {
+                ++n0;
+                float d = calc_dist(f[0], f[1], f[2], mx, my, mz);
+                if (d < best_sd) {
+                    best_sd = d;
+                    best_s = u;
+                }
+                if (check) {
+                    int other = foreign_voxel_first_index(g, f[0], f[1], f[2], keys, order, n);
+                    if (other != 0x7fffffff) split = min(split, 2 * max(other, first_idx) + 1);
+                }
+            }
+            int pair = (int)lk_pair(lk);
+            // push right first so the left subtree is visited first
+#pragma unroll
+            for (int s = 1; s >= 0; --s) {
+                if (!(lk & lk_has(s)) || (lk & lk_td(s))) continue;
+                if (box_disjoint(box, f + 4 + 6 * s)) continue;
+                if (sp < 2 * MAX_DEPTH + 4) stack[sp++] = 2 * pair + s;
+                else atomicOr(&hdr->overflow, 4);
+            }
+        }
+    }
+
+    // ---- the ordered fold over this voxel's batch points ----
+    bool have_new = false;
+    float cx = 0, cy = 0, cz = 0;
+    uint32_t cid = 0;
+    float ex = 0, ey = 0, ez = 0;  // the single original point when |S0| == 1
+    if (best_s >= 0) {
+        const float* f = pool.hot[best_s].f;
+        ex = f[0]; ey = f[1]; ez = f[2];
+    }
+    uint32_t eid = best_s >= 0 ? pool.ids[best_s] : 0u;
+    int added = 0;
+    for (int j = j0; j < j1; ++j) {
+        int idx = (int)order[j];
+        float4 P = pts[idx];
+        if (check) {
+            if (!box_contains(box, P.x, P.y, P.z)) split = min(split, 2 * idx);
+            int other = foreign_voxel_first_index(g, P.x, P.y, P.z, keys, order, n);
+            if (other != 0x7fffffff) split = min(split, 2 * idx);
+        }
+        float dp = calc_dist(P.x, P.y, P.z, mx, my, mz);
+        float bx = P.x, by = P.y, bz = P.z;
+        uint32_t bid = fbits(P.w);
+        int s_count;
+        if (j == j0) {
+            s_count = n0;
+            if (n0 > 0 && best_sd < dp) {
+                bx = ex; by = ey; bz = ez; bid = eid;
+            }
+        } else {
+            s_count = 1;
+            float qx = have_new ? cx : ex, qy = have_new ? cy : ey, qz = have_new ? cz : ez;
+            uint32_t qid = have_new ? cid : eid;
+            float de = calc_dist(qx, qy, qz, mx, my, mz);
+            if (de < dp) {
+                bx = qx; by = qy; bz = qz; bid = qid;
+            }
+        }
+        if (s_count > 1 || same_point(P.x, P.y, P.z, bx, by, bz)) {
+            have_new = true;
+            cx = bx; cy = by; cz = bz; cid = bid;
+            ++added;
+        }
+    }
+    if (split != 0x7fffffff) atomicMin(&hdr->split_index, split);
+    atomicAdd(&hdr->added, added);
+    del_flag[v] = (have_new && n0 > 0) ? 1 : 0;
+    ins_flag[v] = have_new ? 1 : 0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) del_boxes[6 * v + a] = box[a];
+    ins_pts[v] = make_float4(cx, cy, cz, bitsf(cid));
+}
+
+// stream compaction of the flagged voxel boxes / points
+__global__ void k_compact_boxes(const int* __restrict__ flag, const int* __restrict__ scan, const float* __restrict__ in,
+                                int bound, DevHdr* hdr, float* __restrict__ out, int* __restrict__ out_count) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    int nv = hdr->group_count;
+    if (v >= bound || v >= nv) return;
+    if (v == nv - 1) *out_count = scan[v] + flag[v];
+    if (!flag[v]) return;
+    int o = scan[v];
+#pragma unroll
+    for (int a = 0; a < 6; ++a) out[6 * o + a] = in[6 * v + a];
+}
+
+__global__ void k_compact_pts(const int* __restrict__ flag, const int* __restrict__ scan, const float4* __restrict__ in,
+                              int bound, DevHdr* hdr, float4* __restrict__ out, int* __restrict__ out_count) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    int nv = hdr->group_count;
+    if (v >= bound || v >= nv) return;
+    if (v == nv - 1) *out_count = scan[v] + flag[v];
+    if (!flag[v]) return;
+    out[scan[v]] = in[v];
+}
+
+void launch_voxel_keys2(const float* xyz, const uint32_t* ids, uint32_t id_base, int n, float ds,
+                        unsigned long long* keys, uint32_t* vals, float4* pts, int* bad, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_voxel_keys, dim3(blocks_for(n, 256)), dim3(256), 0, s, xyz, ids, id_base, n, ds, keys, vals,
+                       pts, bad);
+}
+void launch_voxel_flags(const unsigned long long* keys, int n, int* flag, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_voxel_flags, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n, flag);
+}
+void launch_voxel_starts(const int* flag, const int* scan, int n, int* vstart, DevHdr* hdr, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_voxel_starts, dim3(blocks_for(n, 256)), dim3(256), 0, s, flag, scan, n, vstart, hdr);
+}
+void launch_voxel_fold2(Pool pool, const float4* pts, const uint32_t* order, const unsigned long long* keys,
+                        const int* vstart, int n, int bound, float ds, int check, float* del_boxes, int* del_flag,
+                        float4* ins_pts, int* ins_flag, DevHdr* hdr, long long max_steps, hipStream_t s) {
+    if (bound <= 0) return;
+    hipLaunchKernelGGL(k_voxel_fold, dim3(blocks_for(bound, 64)), dim3(64), 0, s, pool, pts, order, keys, vstart, n,
+                       bound, ds, check, del_boxes, del_flag, ins_pts, ins_flag, hdr, max_steps);
+}
+void launch_compact_boxes(const int* flag, const int* scan, const float* in, int bound, DevHdr* hdr, float* out,
+                          int* out_count, hipStream_t s) {
+    if (bound <= 0) return;
+    hipLaunchKernelGGL(k_compact_boxes, dim3(blocks_for(bound, 256)), dim3(256), 0, s, flag, scan, in, bound, hdr, out,
+                       out_count);
+}
+void launch_compact_pts(const int* flag, const int* scan, const float4* in, int bound, DevHdr* hdr, float4* out,
+                        int* out_count, hipStream_t s) {
+    if (bound <= 0) return;
+    hipLaunchKernelGGL(k_compact_pts, dim3(blocks_for(bound, 256)), dim3(256), 0, s, flag, scan, in, bound, hdr, out,
+                       out_count);
+}
+
+}  // namespace ikd

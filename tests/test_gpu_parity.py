@@ -1,0 +1,266 @@
+"""GPU parity: the HIP path through the C ABI against numpy brute force and the
+CPU oracle, on the same seeded inputs.
+
+Bars (what "equal" means here):
+- kNN: the device returns the k smallest live points under (d2, x, y, z, id);
+  against brute force with that order, squared distances, ids and counts are
+  bit-exact.  Against the oracle (the reference's own heap and traversal) the
+  squared distances are bit-exact and the ids equal except for entries tied
+  at the k-th distance (the reference keeps whichever it met first).
+- box search, box delete, Add_Points, Delete_Points, flatten: exact multisets
+  of points and exact counts.
+- radius search: exact outside a 4-ulp band around the sphere (the
+  reference's sqrt-based subtree rule can resolve band points either way).
+"""
+import importlib
+
+import numpy as np
+import pytest
+
+from common import (INF, assert_same_multiset, brute_knn, brute_radius, grid_boxes, grid_cloud, in_box,
+                    knn_equivalent, random_boxes, uniform_cloud)
+from oracle import OracleTree
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ikd():
+    return importlib.import_module("ikd-tree_amd")
+
+
+def _pair(ikd, pts, ids=None, params=(0.5, 0.6, 0.2)):
+    g = ikd.KD_TREE(*params)
+    o = OracleTree(*params)
+    g.Build(pts, ids)
+    o.Build(pts, ids)
+    return g, o
+
+
+def _check_knn(g, o, pts_live, ids_live, q, k, max_dist, strict_vs_oracle=True):
+    gx, gd, gi, gc = g.knn(q, k, max_dist)
+    bd, bi, bc = brute_knn(pts_live, ids_live, q, k, max_dist, tie="xyz")
+    assert np.array_equal(gc, bc), "counts vs brute force"
+    for i in range(q.shape[0]):
+        c = gc[i]
+        assert np.array_equal(gd[i, :c].view(np.uint32), bd[i, :c].view(np.uint32)), f"d2 q{i}"
+        assert np.array_equal(gi[i, :c], bi[i, :c]), f"ids q{i}"
+    # returned coordinates are the stored points
+    lut = {int(i): p for i, p in zip(ids_live, pts_live)}
+    for i in range(0, q.shape[0], 17):
+        for j in range(gc[i]):
+            assert np.array_equal(gx[i, j], lut[int(gi[i, j])])
+    od = o.knn(q, k, max_dist)
+    assert knn_equivalent(gd, gi, gc, od[1], od[2], od[3], strict_ties=strict_vs_oracle) == []
+
+
+@pytest.mark.parametrize("k,max_dist", [(1, INF), (5, INF), (5, 5.0), (8, 2.0), (20, INF), (64, 3.0)])
+def test_build_knn_uniform(ikd, k, max_dist):
+    rng = np.random.default_rng(100 + k)
+    pts = uniform_cloud(rng, 60000)
+    ids = np.arange(pts.shape[0], dtype=np.uint32)
+    q = uniform_cloud(rng, 3000)
+    g, o = _pair(ikd, pts, ids)
+    assert g.validnum() == 60000 and g.size() == 60000
+    _check_knn(g, o, pts, ids, q, k, max_dist)
+    assert np.array_equal(g.tree_range(), o.tree_range())
+
+
+def test_build_structure_matches_oracle(ikd):
+    """Tie-free keys: the device build is the reference's tree (same root
+    statistics, same range) and Morton-ordered batches do not change results."""
+    rng = np.random.default_rng(7)
+    pts = uniform_cloud(rng, 5000)
+    g, o = _pair(ikd, pts)
+    q = uniform_cloud(rng, 6000)  # >= 4096 queries takes the Morton-sorted path
+    _, d1, i1, c1 = g.knn(q, 5)
+    _, d2, i2, c2 = g.knn(q[:100], 5)
+    assert np.array_equal(d1[:100], d2) and np.array_equal(i1[:100], i2)
+
+
+def test_knn_grid_ties(ikd):
+    rng = np.random.default_rng(9)
+    pts = grid_cloud(rng, 20000, extent=16)  # duplicates and exact distance ties
+    ids = np.arange(pts.shape[0], dtype=np.uint32)
+    q = grid_cloud(rng, 1500, extent=16) + np.float32(0.125)
+    g, o = _pair(ikd, pts, ids)
+    _check_knn(g, o, pts, ids, q, 10, INF, strict_vs_oracle=False)
+
+
+def test_empty_and_tiny_trees(ikd):
+    g = ikd.KD_TREE()
+    q = np.zeros((5, 3), np.float32)
+    _, d, i, c = g.knn(q, 3)
+    assert np.all(c == 0) and np.all(np.isinf(d))
+    assert g.validnum() == 0 and g.size() == 0
+    g.Build(np.array([[1, 2, 3]], np.float32))
+    _, d, i, c = g.knn(q, 3)
+    assert np.all(c == 1) and np.all(d[:, 0] == np.float32(14.0))
+    g.Build(np.zeros((0, 3), np.float32))
+    assert g.validnum() == 0
+
+
+def test_argument_errors(ikd):
+    g = ikd.KD_TREE()
+    g.Build(uniform_cloud(np.random.default_rng(1), 100))
+    with pytest.raises(ikd.IkdError) as e:
+        g.knn(np.zeros((1, 3), np.float32), 0)
+    assert e.value.code == ikd.IKD_ERR_ARG
+    with pytest.raises(ikd.IkdError) as e:
+        g.knn(np.zeros((1, 3), np.float32), 65)
+    assert e.value.code == ikd.IKD_ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("grid", [False, True])
+def test_box_search(ikd, grid):
+    rng = np.random.default_rng(20 + grid)
+    if grid:
+        pts = grid_cloud(rng, 30000, extent=20)
+        boxes = grid_boxes(rng, 200, extent=20, smax=16)
+    else:
+        pts = uniform_cloud(rng, 30000)
+        boxes = random_boxes(rng, 200, smin=1, smax=20)
+    g, o = _pair(ikd, pts)
+    off, xyz, _ = g.box_search_batch(boxes)
+    for b in range(boxes.shape[0]):
+        got = xyz[off[b]:off[b + 1]]
+        assert_same_multiset(got, pts[in_box(pts, boxes[b])], f"box {b}")
+        if b % 20 == 0:
+            assert_same_multiset(got, o.Box_Search(boxes[b])[0], f"box {b} vs oracle")
+
+
+def test_radius_search(ikd):
+    rng = np.random.default_rng(30)
+    pts = uniform_cloud(rng, 30000)
+    ids = np.arange(30000, dtype=np.uint32)
+    g, o = _pair(ikd, pts, ids)
+    c = uniform_cloud(rng, 300)
+    r = (0.2 + 6.0 * rng.random(300)).astype(np.float32)
+    off, _, gid = g.radius_search_batch(c, r)
+    for i in range(300):
+        got = np.zeros(30000, bool)
+        got[gid[off[i]:off[i + 1]]] = True
+        inside, band = brute_radius(pts, c[i], float(r[i]))
+        assert np.array_equal(got[~band], inside[~band]), f"radius query {i}"
+        if i % 30 == 0:
+            _, oid = o.Radius_Search(c[i], float(r[i]))
+            og = np.zeros(30000, bool)
+            og[oid] = True
+            assert np.array_equal(got[~band], og[~band])
+
+
+@pytest.mark.parametrize("grid", [False, True])
+def test_delete_point_boxes(ikd, grid):
+    rng = np.random.default_rng(40 + grid)
+    if grid:
+        pts = grid_cloud(rng, 40000, extent=20)
+        boxes = grid_boxes(rng, 10, extent=20, smax=24)
+    else:
+        pts = uniform_cloud(rng, 40000)
+        boxes = random_boxes(rng, 10)
+    ids = np.arange(pts.shape[0], dtype=np.uint32)
+    g, o = _pair(ikd, pts, ids)
+    gone = np.zeros(pts.shape[0], bool)
+    for b in boxes:
+        gone |= in_box(pts, b)
+    n_g = g.Delete_Point_Boxes(boxes)
+    n_o = o.Delete_Point_Boxes(boxes)
+    assert n_g == n_o == int(gone.sum())
+    assert g.validnum() == o.validnum() == int((~gone).sum())
+    assert_same_multiset(g.flatten()[0], pts[~gone], "live set after box delete")
+    q = uniform_cloud(rng, 1500)
+    _check_knn(g, o, pts[~gone], ids[~gone], q, 5, INF, strict_vs_oracle=not grid)
+    # deleting again removes nothing
+    assert g.Delete_Point_Boxes(boxes) == 0
+
+
+def test_delete_everything_then_insert(ikd):
+    rng = np.random.default_rng(45)
+    pts = uniform_cloud(rng, 5000)
+    g, o = _pair(ikd, pts)
+    box = np.array([[-60, -60, -60, 60, 60, 60]], np.float32)
+    assert g.Delete_Point_Boxes(box) == 5000
+    assert g.validnum() == 0
+    extra = uniform_cloud(rng, 300)
+    assert g.Add_Points(extra, False) == 0
+    o.Delete_Point_Boxes(box)
+    o.Add_Points(extra, False)
+    assert g.validnum() == 300
+    assert_same_multiset(g.flatten()[0], extra, "reinsert after full delete")
+
+
+def test_add_points_without_downsample(ikd):
+    rng = np.random.default_rng(50)
+    base = uniform_cloud(rng, 20000)
+    ids = np.arange(40000, dtype=np.uint32)
+    g, o = _pair(ikd, base, ids[:20000])
+    allp = [base]
+    for it in range(4):
+        extra = uniform_cloud(rng, 5000, -60, 60)
+        assert g.Add_Points(extra, False, ids[20000 + 5000 * it:20000 + 5000 * (it + 1)]) == 0
+        o.Add_Points(extra, False, ids[20000 + 5000 * it:20000 + 5000 * (it + 1)])
+        allp.append(extra)
+    allp = np.concatenate(allp)
+    assert g.validnum() == 40000
+    assert_same_multiset(g.flatten()[0], allp, "live set after inserts")
+    q = uniform_cloud(rng, 1500, -60, 60)
+    _check_knn(g, o, allp, ids, q, 5, INF)
+
+
+@pytest.mark.parametrize("ds", [0.5, 1.0, 0.2])
+def test_add_points_downsample_vs_oracle(ikd, ds):
+    rng = np.random.default_rng(60)
+    base = uniform_cloud(rng, 8000, -10, 10)
+    g, o = _pair(ikd, base, params=(0.5, 0.6, ds))
+    for it in range(5):
+        batch = uniform_cloud(rng, 3000, -11, 11)
+        batch[::40] = base[rng.integers(0, 8000, batch[::40].shape[0])]  # exact duplicates
+        a_g = g.Add_Points(batch, True)
+        a_o = o.Add_Points(batch, True)
+        assert a_g == a_o, f"Add_Points return, batch {it}"
+        assert g.validnum() == o.validnum()
+        assert_same_multiset(g.flatten()[0], o.flatten()[0], f"downsampled live set, batch {it}")
+
+
+def test_delete_points(ikd):
+    rng = np.random.default_rng(70)
+    pts = uniform_cloud(rng, 20000)
+    g, o = _pair(ikd, pts)
+    sel = rng.choice(20000, 6000, replace=False)
+    g.Delete_Points(pts[sel])
+    o.Delete_Points(pts[sel])
+    keep = np.ones(20000, bool)
+    keep[sel] = False
+    assert g.validnum() == o.validnum() == 14000
+    assert_same_multiset(g.flatten()[0], pts[keep], "live set after Delete_Points")
+    # duplicates inside one batch take the ordered path
+    dup = np.concatenate([pts[keep][:10], pts[keep][:10]])
+    g.Delete_Points(dup)
+    o.Delete_Points(dup)
+    assert g.validnum() == o.validnum() == 13990
+
+
+def test_mixed_sequence_vs_oracle(ikd):
+    rng = np.random.default_rng(80)
+    g, o = _pair(ikd, uniform_cloud(rng, 20000, -30, 30), params=(0.5, 0.6, 0.5))
+    next_id = 20000
+    for it in range(8):
+        b = uniform_cloud(rng, 2000, -32, 32)
+        bid = np.arange(next_id, next_id + 2000, dtype=np.uint32)
+        next_id += 2000
+        assert g.Add_Points(b, it % 2 == 0, bid) == o.Add_Points(b, it % 2 == 0, bid)
+        boxes = random_boxes(rng, 3, -30, 30, 2, 15)
+        assert g.Delete_Point_Boxes(boxes) == o.Delete_Point_Boxes(boxes)
+        live = o.flatten()[0]
+        if it % 3 == 2:
+            dp = live[rng.choice(live.shape[0], 500, replace=False)]
+            g.Delete_Points(dp)
+            o.Delete_Points(dp)
+        assert g.validnum() == o.validnum()
+        gl, gid = g.flatten()
+        ol, oid = o.flatten()
+        assert_same_multiset(gl, ol, f"live set step {it}")
+        q = uniform_cloud(rng, 500, -32, 32)
+        _, gd, gi, gc = g.knn(q, 5, 5.0)
+        od = o.knn(q, 5, 5.0)
+        assert knn_equivalent(gd, gi, gc, od[1], od[2], od[3]) == []
