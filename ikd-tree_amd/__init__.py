@@ -40,8 +40,10 @@ EXPORTS = (
     "ikd_nearest_search_device", "ikd_nearest_search_counters", "ikd_box_search", "ikd_radius_search",
     "ikd_add_points", "ikd_add_points_device", "ikd_delete_points", "ikd_delete_point_boxes", "ikd_size",
     "ikd_validnum", "ikd_tree_range", "ikd_root_alpha", "ikd_flatten", "ikd_synchronize", "ikd_stream",
-    "ikd_pool_stats", "ikd_last_error",
+    "ikd_pool_stats", "ikd_set_tuning", "ikd_last_error",
 )
+IKD_TUNE_QUERY_ORDER = 1
+IKD_TUNE_KNN_STACK = 2
 
 
 class IkdError(RuntimeError):
@@ -93,6 +95,7 @@ def load_library():
         "ikd_synchronize": [vp],
         "ikd_stream": [vp, ctypes.POINTER(vp)],
         "ikd_pool_stats": [vp, vp, vp, vp, vp],
+        "ikd_set_tuning": [vp, i32, i32],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -302,6 +305,9 @@ class KD_TREE:
                                              ctypes.byref(reb)))
         return {"used_nodes": used.value, "capacity_nodes": cap.value, "max_depth": depth.value,
                 "rebuilds": reb.value}
+
+    def set_tuning(self, key: int, value: int):
+        self._check(self._lib.ikd_set_tuning(self._h, key, value))
 
     def synchronize(self):
         self._check(self._lib.ikd_synchronize(self._h))

@@ -151,6 +151,8 @@ struct ikd_tree_s {
     long long rebuilds = 0;
     int max_depth = 0;
     long long pool_used = 4;
+    int query_order = 0;  // 1: Morton-sort large query batches before the kNN launch
+    int knn_stack = 0;    // 0: pick from the tree depth; else 32 or 64
     // scratch
     DevBuf b[80];
 };
@@ -849,7 +851,7 @@ int knn_dev(ikd_tree_t t, const float* d_q, int nq, int k, double max_dist, floa
     a.boxes = d_boxes;
     a.overflow = &t->d_hdr->overflow;
     a.max_steps = (int)std::min<long long>(t->pool_used + 16, INT_MAX);
-    if (nq >= 4096) {
+    if (t->query_order && nq >= 4096) {
         unsigned long long *mk, *mk2;
         uint32_t *mv, *mv2;
         CHK(ensure(t, B_MORTK, nq, &mk));
@@ -861,6 +863,7 @@ int knn_dev(ikd_tree_t t, const float* d_q, int nq, int k, double max_dist, floa
         a.order = mv2;
     }
     int stack = t->max_depth + 2 <= 32 ? 32 : 64;
+    if (t->knn_stack > stack) stack = t->knn_stack;
     if (launch_knn(a, stack, 64, t->st) != 0) return IKD_ERR_UNSUPPORTED;
     HIPCHK(hipGetLastError());
     return IKD_OK;
@@ -1238,6 +1241,21 @@ int ikd_stream(ikd_tree_t t, void** out) {
     if (!out) return IKD_ERR_ARG;
     *out = (void*)t->st;
     return IKD_OK;
+}
+
+int ikd_set_tuning(ikd_tree_t t, int key, int value) {
+    CHECK_T();
+    switch (key) {
+        case IKD_TUNE_QUERY_ORDER:
+            t->query_order = value ? 1 : 0;
+            return IKD_OK;
+        case IKD_TUNE_KNN_STACK:
+            if (value != 0 && value != 32 && value != 64) return IKD_ERR_ARG;
+            t->knn_stack = value;
+            return IKD_OK;
+        default:
+            return IKD_ERR_ARG;
+    }
 }
 
 int ikd_pool_stats(ikd_tree_t t, int64_t* used, int64_t* capacity, int32_t* max_depth, int64_t* rebuilds) {

@@ -133,6 +133,15 @@ int ikd_stream(ikd_tree_t t, void** out_stream);
 int ikd_pool_stats(ikd_tree_t t, int64_t* used_nodes, int64_t* capacity_nodes, int32_t* max_depth,
                    int64_t* rebuilds);
 
+/* Launch tuning (no counterpart in the reference; results never depend on it):
+ *  IKD_TUNE_QUERY_ORDER 1 sorts batches of >= 4096 kNN queries into
+ *                       Morton order before the launch, 0 (default) keeps input order;
+ *  IKD_TUNE_KNN_STACK   0 (default: from the tree depth), 32 or 64 LDS stack
+ *                       entries per lane. */
+#define IKD_TUNE_QUERY_ORDER 1
+#define IKD_TUNE_KNN_STACK 2
+int ikd_set_tuning(ikd_tree_t t, int key, int value);
+
 /* Text of the last error on this tree (never NULL). */
 const char* ikd_last_error(ikd_tree_t t);
 
