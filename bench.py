@@ -54,9 +54,12 @@ def main():
     ap.add_argument("--max-dist", type=float, default=5.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-scans", type=int, default=2, help="scans of queries in the CPU sample")
-    ap.add_argument("--query-order", type=int, default=0, help="1: Morton-sort queries before the kNN launch")
     ap.add_argument("--batch-scans", type=int, default=8, help="scans per launch for the batched form")
     ap.add_argument("--frames", type=int, default=200, help="scans in the mapping-pipeline phase")
+    ap.add_argument("--lanes", type=int, default=0, help="force lanes per query (0: autotuned)")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch shape or --lanes)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (10 launches), to calibrate HBM byte counters")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -67,8 +70,13 @@ def main():
 
     import torch
 
+    if args.copy_calibration:
+        copy_calibration(torch, args.copy_calibration)
+        return
+
     ikd = importlib.import_module("ikd-tree_amd")
     scenes = importlib.import_module("ikd-tree_amd.scenes")
+    autotune = importlib.import_module("ikd-tree_amd.autotune")
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
 
@@ -87,7 +95,6 @@ def main():
     scans = [scene.scan(f + i) for i in range(total_steps)]
 
     tree = ikd.KD_TREE(0.5, 0.6, 0.5)
-    tree.set_tuning(ikd.IKD_TUNE_QUERY_ORDER, args.query_order)
     t0 = time.perf_counter()
     tree.Build(seed_pts)
     build_s = time.perf_counter() - t0
@@ -101,6 +108,14 @@ def main():
     ext = torch.cuda.ExternalStream(tree.stream())
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(total_steps)]
     torch.cuda.synchronize()
+
+    if args.no_tune:
+        tune_best, tune_ms = (args.lanes if args.lanes else -1, 0, 0), {}
+    else:
+        tune_best, tune_ms = autotune.autotune_knn(tree, qd[0], k, args.max_dist)
+    if args.lanes:
+        tree.set_tuning(ikd.IKD_TUNE_LANES_PER_QUERY, args.lanes)
+        tune_best = (args.lanes,) + tuple(tune_best[1:])
 
     def knn_step(i):
         ev[i][0].record(ext)
@@ -149,6 +164,7 @@ def main():
               torch.empty((big.shape[0], k), dtype=torch.float32, device=dev),
               torch.empty((big.shape[0], k), dtype=torch.int32, device=dev),
               torch.empty((big.shape[0],), dtype=torch.int32, device=dev))
+        btune_best, btune_ms = autotune.autotune_knn(tree, big, k, args.max_dist)
         tree.knn_device(big, k, args.max_dist, *bo)
         tree.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -163,10 +179,12 @@ def main():
                                      sum(scans[j].shape[0] for j in timed[:m + 1])], outs[timed[m]][1])
                    for m in range(nb))
         batched = {"scans_per_launch": nb, "queries_per_launch": int(big.shape[0]), "launch_ms": ms,
-                   "qps": big.shape[0] / (ms / 1e3), "identical_to_plain": bool(same)}
+                   "qps": big.shape[0] / (ms / 1e3), "identical_to_plain": bool(same),
+                   "tuned": {"lanes_per_query": btune_best[0], "stack": btune_best[1], "morton": btune_best[2]},
+                   "sweep_ms": {f"lanes{a}_stack{b}_order{c}": v for (a, b, c), v in btune_ms.items()}}
 
     # ================================================ phase B: mapping pipeline
-    pipe = pipeline_latency(ikd, torch, dev, scene, args.frames, k, args.max_dist)
+    pipe = pipeline_latency(ikd, torch, dev, scene, args.frames, k, args.max_dist) if args.frames > 1 else None
 
     result = {
         "metric": METRIC,
@@ -188,7 +206,8 @@ def main():
             "scan_points_mean": nq_total / args.steps,
             "k": k,
             "max_dist_m": args.max_dist,
-            "query_order": "morton" if args.query_order else "scan order",
+            "tuned": {"lanes_per_query": tune_best[0], "stack": tune_best[1], "morton": tune_best[2]},
+            "sweep_ms": {f"lanes{a}_stack{b}_order{c}": v for (a, b, c), v in tune_ms.items()},
         },
         "roofline": {
             "bound": "hbm",
@@ -197,7 +216,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved_gbs / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": f"k_knn<{knn_bucket(k)},32>",
+            "kernel": knn_kernel_name(k, tune_best[0]),
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": avg_launch_s * 1e3,
             "bytes_per_query": alg_bytes / nq_total,
@@ -216,6 +235,28 @@ def main():
         result["cpu_baseline"] = cpu_baseline(seed_pts, [scans[i] for i in timed][: args.cpu_scans], k,
                                               args.max_dist)
     print(json.dumps(result))
+
+
+def copy_calibration(torch, mib):
+    """A plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
+    counters per dispatch give the scale factor from counter units to bytes."""
+    n = mib * (1 << 20) // 4
+    a = torch.arange(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(10):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 10}))
+
+
+def knn_kernel_name(k, lanes):
+    if lanes == 1:
+        return f"k_knn<{knn_bucket(k)},32>"
+    if lanes == 0:
+        return f"k_knn_persist<{knn_bucket(k)},32>"
+    if lanes < 0:
+        return f"k_knn*<{knn_bucket(k)}> (lanes per query from the batch size)"
+    return f"k_knn_team<{knn_bucket(k)},{lanes},32>"
 
 
 def knn_bucket(k):

@@ -40,10 +40,16 @@ EXPORTS = (
     "ikd_nearest_search_device", "ikd_nearest_search_counters", "ikd_box_search", "ikd_radius_search",
     "ikd_add_points", "ikd_add_points_device", "ikd_delete_points", "ikd_delete_point_boxes", "ikd_size",
     "ikd_validnum", "ikd_tree_range", "ikd_root_alpha", "ikd_flatten", "ikd_synchronize", "ikd_stream",
-    "ikd_pool_stats", "ikd_set_tuning", "ikd_last_error",
+    "ikd_pool_stats", "ikd_set_tuning", "ikd_dump_nodes", "ikd_last_error",
+    "ikd_batch_create", "ikd_batch_destroy", "ikd_batch_nearest_search_device", "ikd_batch_synchronize",
+    "ikd_batch_stream", "ikd_batch_set_tuning", "ikd_batch_last_error",
+    "ikd_now_ns", "ikd_server_create", "ikd_server_add_map", "ikd_server_submit", "ikd_server_wait",
+    "ikd_server_pending", "ikd_server_destroy",
 )
 IKD_TUNE_QUERY_ORDER = 1
 IKD_TUNE_KNN_STACK = 2
+IKD_TUNE_LANES_PER_QUERY = 3
+IKD_TUNE_SMALL_BUILD = 4
 
 
 class IkdError(RuntimeError):
@@ -96,6 +102,13 @@ def load_library():
         "ikd_stream": [vp, ctypes.POINTER(vp)],
         "ikd_pool_stats": [vp, vp, vp, vp, vp],
         "ikd_set_tuning": [vp, i32, i32],
+        "ikd_dump_nodes": [vp, vp, i64, vp],
+        "ikd_batch_create": [ctypes.POINTER(vp)],
+        "ikd_batch_destroy": [vp],
+        "ikd_batch_nearest_search_device": [vp, vp, i32, vp, vp, i64, vp, i32, vp, f64, vp, vp, vp, vp],
+        "ikd_batch_synchronize": [vp],
+        "ikd_batch_stream": [vp, ctypes.POINTER(vp)],
+        "ikd_batch_set_tuning": [vp, i32, i32],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -103,6 +116,8 @@ def load_library():
         fn.restype = i32
     lib.ikd_last_error.argtypes = [vp]
     lib.ikd_last_error.restype = ctypes.c_char_p
+    lib.ikd_batch_last_error.argtypes = [vp]
+    lib.ikd_batch_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
 
@@ -309,10 +324,75 @@ class KD_TREE:
     def set_tuning(self, key: int, value: int):
         self._check(self._lib.ikd_set_tuning(self._h, key, value))
 
+    def dump_nodes(self):
+        """Pre-order rows (oracle DUMP_DTYPE layout) of the device tree."""
+        n = ctypes.c_int64(0)
+        self._check(self._lib.ikd_dump_nodes(self._h, None, 0, ctypes.byref(n)))
+        dt = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("id", "<u4"), ("axis", "<i4"),
+                       ("tree_size", "<i4"), ("invalid", "<i4"), ("flags", "<i4"), ("range", "<f4", (6,)),
+                       ("left", "<i4"), ("right", "<i4")])
+        rows = np.zeros(max(n.value, 1), dt)
+        self._check(self._lib.ikd_dump_nodes(self._h, _ptr(rows), n.value, ctypes.byref(n)))
+        return rows[:n.value]
+
     def synchronize(self):
         self._check(self._lib.ikd_synchronize(self._h))
 
     def stream(self) -> int:
         s = ctypes.c_void_p()
         self._check(self._lib.ikd_stream(self._h, ctypes.byref(s)))
+        return s.value or 0
+
+
+class Batch:
+    """Many trees' kNN queries in one launch (the batched Nearest_Search).
+
+    ``nearest_search_device(trees, qtree, q, k_max, k=None, max_dist=inf, outs)``
+    runs query i against ``trees[qtree[i]]`` with its own k (``k`` an int32
+    tensor, or None for k_max) and max_dist (a float64 tensor, or one float).
+    Results are identical to one ``KD_TREE.knn_device`` call per tree."""
+
+    def __init__(self):
+        self._lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self._lib.ikd_batch_create(ctypes.byref(h))
+        if rc != IKD_OK:
+            raise IkdError(rc, "batch create failed")
+        self._h = h
+
+    def _check(self, rc):
+        if rc != IKD_OK:
+            raise IkdError(rc, self._lib.ikd_batch_last_error(self._h).decode())
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.ikd_batch_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tuning(self, key: int, value: int):
+        self._check(self._lib.ikd_batch_set_tuning(self._h, key, value))
+
+    def nearest_search_device(self, trees, qtree, q, k_max: int, out_xyz, out_d2, out_ids, out_counts, k=None,
+                              max_dist=float("inf")):
+        handles = (ctypes.c_void_p * len(trees))(*[t._h.value for t in trees])
+        per_q = not isinstance(max_dist, (int, float))
+        self._check(self._lib.ikd_batch_nearest_search_device(
+            self._h, handles, len(trees), _tensor_ptr(qtree), _tensor_ptr(q), q.shape[0],
+            None if k is None else _tensor_ptr(k), int(k_max), _tensor_ptr(max_dist) if per_q else None,
+            float("inf") if per_q else float(max_dist), _tensor_ptr(out_xyz), _tensor_ptr(out_d2),
+            _tensor_ptr(out_ids), _tensor_ptr(out_counts)))
+
+    def synchronize(self):
+        self._check(self._lib.ikd_batch_synchronize(self._h))
+
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        self._check(self._lib.ikd_batch_stream(self._h, ctypes.byref(s)))
         return s.value or 0

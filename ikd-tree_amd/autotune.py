@@ -1,0 +1,57 @@
+"""Launch autotuner for the batched kNN kernel.
+
+Sweeps the lane schedule (0: persistent lanes that fetch queries; 1, 2, 4, 8:
+lanes per query, i.e. 64, 32, 16, 8 queries per wavefront), the per-lane LDS
+stack depth (32, 64) and the query order (input order or Morton order) on a representative device-resident batch, timing each
+configuration with HIP events on the tree's stream.  Every configuration
+returns identical results (the selection order is total), so the tuner only
+changes speed; ``verify=True`` checks that on the sweep batch.
+"""
+from __future__ import annotations
+
+import itertools
+
+
+def autotune_knn(tree, queries, k: int, max_dist: float = float("inf"), reps: int = 3, verify: bool = True,
+                 lanes=(0, 1, 2, 4, 8), stacks=(0, 64), orders=(0, 1)):
+    """Time the kNN launch on ``queries`` (a contiguous cuda float32 tensor [n, 3])
+    for every configuration, apply the fastest to ``tree`` and return
+    (best_config, {config: ms})."""
+    import torch
+    import importlib
+
+    ikd = importlib.import_module("ikd-tree_amd")
+    dev = queries.device
+    n = queries.shape[0]
+    outs = (torch.empty((n, k, 3), dtype=torch.float32, device=dev),
+            torch.empty((n, k), dtype=torch.float32, device=dev),
+            torch.empty((n, k), dtype=torch.int32, device=dev),
+            torch.empty((n,), dtype=torch.int32, device=dev))
+    ref = None
+    ext = torch.cuda.ExternalStream(tree.stream())
+    timings = {}
+    for ln, st, od in itertools.product(lanes, stacks, orders):
+        tree.set_tuning(ikd.IKD_TUNE_LANES_PER_QUERY, ln)
+        tree.set_tuning(ikd.IKD_TUNE_KNN_STACK, st)
+        tree.set_tuning(ikd.IKD_TUNE_QUERY_ORDER, od)
+        tree.knn_device(queries, k, max_dist, *outs)  # warm
+        tree.synchronize()
+        if verify:
+            cur = (outs[1].clone(), outs[2].clone(), outs[3].clone())
+            if ref is None:
+                ref = cur
+            elif not all(torch.equal(x, y) for x, y in zip(ref, cur)):
+                raise AssertionError(f"kNN results changed under tuning lanes={ln} stack={st} order={od}")
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(ext)
+        for _ in range(reps):
+            tree.knn_device(queries, k, max_dist, *outs)
+        e1.record(ext)
+        tree.synchronize()
+        timings[(ln, st, od)] = e0.elapsed_time(e1) / reps
+    best = min(timings, key=timings.get)
+    tree.set_tuning(ikd.IKD_TUNE_LANES_PER_QUERY, best[0])
+    tree.set_tuning(ikd.IKD_TUNE_KNN_STACK, best[1])
+    tree.set_tuning(ikd.IKD_TUNE_QUERY_ORDER, best[2])
+    return best, timings

@@ -83,13 +83,16 @@ __global__ void k_group_maxlen(const int4* __restrict__ seg, DevHdr* hdr, int bo
     atomicMax(out, seg[g].y - seg[g].x + 1);
 }
 
-// (TreeSize, invalid, depth, parent) of the listed nodes
-__global__ void k_node_info(Pool pool, const int* __restrict__ nodes, int n, int4* __restrict__ out) {
+// (node, live points, depth, parent) of the topmost violators, plus the node
+// count of the subtree before its rebuild
+__global__ void k_node_info(Pool pool, const int* __restrict__ top, int n, int4* __restrict__ out,
+                            int* __restrict__ nodes_before) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int u = nodes[i];
+    int u = top[2 * i];
     int4 c = pool.cnt[u];
-    out[i] = make_int4(c.x, c.y, c.w, pool.par[u]);
+    out[i] = make_int4(u, c.x - c.y, c.w, pool.par[u]);
+    nodes_before[i] = top[2 * i + 1];
 }
 
 __global__ void k_seg_of_from_keys(const unsigned long long* __restrict__ keys, int n, int* __restrict__ seg_of) {
@@ -152,9 +155,25 @@ struct ikd_tree_s {
     int max_depth = 0;
     long long pool_used = 4;
     int query_order = 0;  // 1: Morton-sort large query batches before the kNN launch
-    int knn_stack = 0;    // 0: pick from the tree depth; else 32 or 64
+    int knn_stack = 0;    // minimum LDS stack entries per lane (0: the tree depth + 2)
+    int knn_lanes = -1;   // lanes per query: -1 picks from the batch size; 0 persistent; else 1, 2, 4, 8, 16
+    int small_build = 1;  // build subtrees of <= SMALL_MAX points with one workgroup each
+    hipEvent_t ev = nullptr;  // orders batch launches against this tree's stream
     // scratch
     DevBuf b[80];
+};
+
+// A multi-tree launch context: its own stream, scratch and error slot.  A
+// launch waits for every tree it reads (their streams' pending work) and each
+// of those trees' later work waits for the launch.
+struct ikd_batch_s {
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    std::string err;
+    int knn_lanes = -1;
+    int knn_stack = 0;
+    int* h_flag = nullptr;  // pinned mirror of the overflow flag
+    DevBuf b[8];
 };
 
 namespace {
@@ -164,7 +183,7 @@ enum Buf {
     B_SID2, B_SIDE, B_SEG0, B_SEG1, B_SEGD0, B_SEGD1, B_SEGAX, B_SEGMID, B_SEGCH, B_SEGFL, B_SEGSC, B_POSFL, B_POSSC,
     B_SEEDS, B_DIRTY, B_VIOL, B_TOP, B_INFO, B_FRONT0, B_FRONT1, B_EKEY0, B_EKEY1, B_RECS, B_GFLAG, B_GSCAN, B_ATTACH,
     B_PFLAG, B_PSCAN, B_ISEG, B_ISEGD, B_IO0, B_IO1, B_IO2, B_IO3, B_IO4, B_VSTART, B_DELBOX, B_DELFL, B_INSPT,
-    B_INSFL, B_CBOX, B_CPTS, B_MISC, B_MORTK, B_MORTV, B_MORTK2, B_MORTV2, B_CNT64, B_OFF64, B_ADDX, B_ADDI, B_COUNT
+    B_INSFL, B_CBOX, B_CPTS, B_MISC, B_MORTK, B_MORTV, B_MORTK2, B_MORTV2, B_CNT64, B_OFF64, B_ADDX, B_ADDI, B_PNEXT, B_SPAIRS, B_SPSCAN, B_COUNT
 };
 static_assert(B_COUNT <= 80, "scratch table");
 
@@ -182,8 +201,8 @@ static_assert(B_COUNT <= 80, "scratch table");
         if (r_ != IKD_OK) return r_; \
     } while (0)
 
-template <class T>
-int ensure(ikd_tree_t t, int slot, long long count, T** out) {
+template <class T, class Owner>
+int ensure(Owner* t, int slot, long long count, T** out) {
     size_t need = (size_t)std::max<long long>(count, 1) * sizeof(T);
     DevBuf& d = t->b[slot];
     if (d.bytes < need) {
@@ -352,6 +371,18 @@ int run_build(ikd_tree_t t, const float4* pts, const int* seg_of, int m, const i
     if (m <= 0 || S <= 0) return IKD_OK;
     CHK(reserve_nodes(t, 2ll * m + 4));
     hipStream_t st = t->st;
+    if (maxlen <= SMALL_MAX && t->small_build) {
+        // every subtree fits one workgroup: one launch builds them all
+        int *pairs, *pscan;
+        CHK(ensure(t, B_SPAIRS, S, &pairs));
+        CHK(ensure(t, B_SPSCAN, S, &pscan));
+        launch_small_pairs(d_seg, S, pairs, st);
+        CHK(exscan_i32(t, pairs, pscan, S));
+        launch_build_small(pts, d_seg, d_seg_depth, pscan, S, t->pool, t->d_hdr, st);
+        launch_small_commit(pairs, pscan, S, t->d_hdr, st);
+        HIPCHK(hipGetLastError());
+        return IKD_OK;
+    }
     unsigned long long *keya, *keyb;
     uint32_t* vala;
     CHK(ensure(t, B_KEYA, m, &keya));
@@ -486,49 +517,68 @@ int enumerate_live(ikd_tree_t t, const std::vector<int>& roots, long long total_
 }
 
 // ------------------------------------------------- refit, criterion, rebuild
-int refit(ikd_tree_t t, const int* seeds, int nseeds, int stop_at_tdel) {
-    hipStream_t st = t->st;
-    int *dirty, *viol, *top;
+// Refit = mark the ancestors of every seed (one or more batches of seeds),
+// then Update them bottom-up one depth level per launch, then list the
+// topmost Criterion_Check violators.
+int refit_begin(ikd_tree_t t) {
     long long cap = std::max<long long>(t->pool_used, 16);
+    int *dirty, *viol, *top;
     CHK(ensure(t, B_DIRTY, cap, &dirty));
-    CHK(ensure(t, B_VIOL, cap, &viol));
-    CHK(ensure(t, B_TOP, cap, &top));
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, t->d_hdr, 1);
-    launch_mark_dirty(t->pool, seeds, nullptr, nseeds, stop_at_tdel, dirty, t->d_hdr, st);
-    for (int d = t->max_depth; d >= 0; --d) launch_refit_level(t->pool, t->prm, dirty, t->d_hdr, d, viol, st);
-    launch_find_top(t->pool, viol, (int)cap, t->d_hdr, top, st);
-    launch_clear_viol(t->pool, viol, (int)cap, t->d_hdr, st);
+    CHK(ensure(t, B_VIOL, 2 * cap, &viol));
+    CHK(ensure(t, B_TOP, 2 * cap, &top));
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, t->st, t->d_hdr, 1);
+    return IKD_OK;
+}
+int refit_mark(ikd_tree_t t, const int* seeds, const int* nseeds_dev, int bound, int stop_at_tdel) {
+    int* dirty;
+    CHK(ensure(t, B_DIRTY, std::max<long long>(t->pool_used, 16), &dirty));
+    launch_mark_dirty(t->pool, seeds, nseeds_dev, bound, stop_at_tdel, dirty, t->d_hdr, t->st);
+    return IKD_OK;
+}
+int refit_finish(ikd_tree_t t) {
+    long long cap = std::max<long long>(t->pool_used, 16);
+    int *dirty, *viol, *top;
+    CHK(ensure(t, B_DIRTY, cap, &dirty));
+    CHK(ensure(t, B_VIOL, 2 * cap, &viol));
+    CHK(ensure(t, B_TOP, 2 * cap, &top));
+    for (int d = t->max_depth; d >= 0; --d) launch_refit_level(t->pool, t->prm, dirty, t->d_hdr, d, viol, t->st);
+    launch_find_top(t->pool, viol, (int)cap, t->d_hdr, top, t->st);
+    launch_clear_viol(t->pool, viol, (int)cap, t->d_hdr, t->st);
     HIPCHK(hipGetLastError());
     return IKD_OK;
+}
+int refit(ikd_tree_t t, const int* seeds, int nseeds, int stop_at_tdel) {
+    CHK(refit_begin(t));
+    CHK(refit_mark(t, seeds, nullptr, nseeds, stop_at_tdel));
+    return refit_finish(t);
 }
 
 int rebuild_tops(ikd_tree_t t) {
     hipStream_t st = t->st;
-    for (int iter = 0; iter < 64; ++iter) {
+    for (int iter = 0; iter < 8; ++iter) {
         CHK(sync_hdr(t));
         int ntop = t->h_hdr->top_count;
         if (ntop == 0) return IKD_OK;
         int* top;
         int4* info;
-        CHK(ensure(t, B_TOP, ntop, &top));
+        int* before;
+        CHK(ensure(t, B_TOP, 2 * ntop, &top));
         CHK(ensure(t, B_INFO, ntop, &info));
-        hipLaunchKernelGGL(k_node_info, dim3(nblk(ntop)), dim3(BS), 0, st, t->pool, top, ntop, info);
-        std::vector<int> htop(ntop);
+        CHK(ensure(t, B_SEEDS, ntop, &before));
+        hipLaunchKernelGGL(k_node_info, dim3(nblk(ntop)), dim3(BS), 0, st, t->pool, top, ntop, info, before);
         std::vector<int4> hinfo(ntop);
-        HIPCHK(hipMemcpyAsync(htop.data(), top, ntop * sizeof(int), hipMemcpyDeviceToHost, st));
+        std::vector<int> hbefore(ntop);
         HIPCHK(hipMemcpyAsync(hinfo.data(), info, ntop * sizeof(int4), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(hbefore.data(), before, ntop * sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        std::vector<int> roots, seeds;
+        std::vector<int> roots;
         std::vector<int4> segs;
         std::vector<int> segd;
         long long total_nodes = 0, total_live = 0;
         int maxlen = 0;
         bool root_rebuilt = false;
         for (int i = 0; i < ntop; ++i) {
-            int u = htop[i];
-            int live = hinfo[i].x - hinfo[i].y;
-            int parent = hinfo[i].w;
-            seeds.push_back(parent);
+            int u = hinfo[i].x, live = hinfo[i].y, depth = hinfo[i].z, parent = hinfo[i].w;
             if (u == ROOT) root_rebuilt = true;
             if (live <= 0) {
                 // Rebuild of an all-deleted subtree leaves an empty slot (BuildTree of
@@ -538,9 +588,9 @@ int rebuild_tops(ikd_tree_t t) {
                 continue;
             }
             segs.push_back(make_int4((int)total_live, (int)(total_live + live - 1), u, parent));
-            segd.push_back(hinfo[i].z);
+            segd.push_back(depth);
             roots.push_back(u);
-            total_nodes += hinfo[i].x;
+            total_nodes += hbefore[i];
             total_live += live;
             maxlen = std::max(maxlen, live);
         }
@@ -561,12 +611,9 @@ int rebuild_tops(ikd_tree_t t) {
             CHK(run_build(t, pts, segof, (int)total_live, dseg, dsegd, (int)segs.size(), maxlen));
         }
         t->rebuilds += ntop;
-        // ancestors of the rebuilt subtrees shrink: refit and re-check them
-        int* dseeds;
-        CHK(ensure(t, B_SEEDS, (long long)seeds.size(), &dseeds));
-        HIPCHK(hipMemcpyAsync(dseeds, seeds.data(), seeds.size() * sizeof(int), hipMemcpyHostToDevice, st));
-        CHK(sync_hdr(t));
-        CHK(refit(t, dseeds, (int)seeds.size(), 0));
+        // The refit already handed every ancestor the post-rebuild counts, so
+        // the rebuilt subtrees leave nothing to re-check.
+        hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, t->d_hdr, 1);
     }
     t->err = "rebuild did not converge";
     return IKD_ERR_INTERNAL;
@@ -610,9 +657,18 @@ int compact_if_needed(ikd_tree_t t, bool force) {
 }
 
 // ---------------------------------------------------------------- insert
-// Inserts m packed points (device, float4 x,y,z,id).  The caller refits.
-int insert_points(ikd_tree_t t, const float4* pts, int m) {
-    if (m <= 0) return IKD_OK;
+// Inserts m packed points (device, float4 x,y,z,id), then refits the paths
+// to the attach points together with extra_seeds (nodes whose own labels the
+// caller changed; their count is read from *n_extra_dev on the device).
+int insert_points(ikd_tree_t t, const float4* pts, int m, const int* extra_seeds = nullptr,
+                  const int* n_extra_dev = nullptr, int extra_bound = 0) {
+    if (m <= 0) {
+        if (extra_bound <= 0) return IKD_OK;
+        CHK(refit_begin(t));
+        CHK(refit_mark(t, extra_seeds, n_extra_dev, extra_bound, 0));
+        CHK(refit_finish(t));
+        return rebuild_tops(t);
+    }
     hipStream_t st = t->st;
     CHK(reserve_nodes(t, 4ll * m + 16));
     unsigned long long *ka, *kb;
@@ -656,7 +712,10 @@ int insert_points(ikd_tree_t t, const float4* pts, int m) {
     CHK(ensure(t, B_SEEDS, S, &seeds));
     hipLaunchKernelGGL(k_seeds_from_attach, dim3(nblk(S)), dim3(BS), 0, st, attach, t->d_hdr, S, seeds);
     CHK(sync_hdr(t));
-    CHK(refit(t, seeds, S, 0));
+    CHK(refit_begin(t));
+    CHK(refit_mark(t, seeds, nullptr, S, 0));
+    if (extra_bound > 0) CHK(refit_mark(t, extra_seeds, n_extra_dev, extra_bound, 0));
+    CHK(refit_finish(t));
     return rebuild_tops(t);
 }
 
@@ -712,47 +771,51 @@ int add_points_ds_chunk(ikd_tree_t t, const float* d_xyz, const uint32_t* d_ids,
     CHK(sort_pairs(t, ka, kb, va, vb, n, 63));
     launch_voxel_flags(kb, n, flag, st);
     CHK(exscan_i32(t, flag, scan, n));
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, t->d_hdr, 8 | 16);
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, t->d_hdr, 16);
     launch_voxel_starts(flag, scan, n, vstart, t->d_hdr, st);
     float* delbox;
     float4* inspt;
-    int *delfl, *insfl;
+    int *delfl, *insfl, *dseeds;
+    long long dcap = std::max<long long>(t->pool_used, 16);
     CHK(ensure(t, B_DELBOX, 6ll * n, &delbox));
     CHK(ensure(t, B_DELFL, n, &delfl));
     CHK(ensure(t, B_INSPT, n, &inspt));
     CHK(ensure(t, B_INSFL, n, &insfl));
-    launch_voxel_fold2(t->pool, pts, vb, kb, vstart, n, n, t->prm.box_length, check, delbox, delfl, inspt, insfl,
-                       t->d_hdr, t->pool_used + 16, st);
-    int hbad = 0;
-    HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
-    CHK(sync_hdr(t));
-    if (hbad) {
-        t->err = "Add_Points: a voxel index floor(p/box_length) is beyond +-2^20";
-        return IKD_ERR_UNSUPPORTED;
+    CHK(ensure(t, B_RECS, dcap, &dseeds));
+    // With conflict checks on, a first pass only decides whether the batch is
+    // independent; the deleting pass runs only when it is.
+    const int passes = (check && apply) ? 2 : 1;
+    for (int pass = 0; pass < passes; ++pass) {
+        const bool deleting = apply && pass == passes - 1;
+        hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, t->d_hdr, 2 | 8);
+        launch_voxel_fold2(t->pool, pts, vb, kb, vstart, n, n, t->prm.box_length, check, delbox, delfl, inspt, insfl,
+                           t->d_hdr, t->pool_used + 16, deleting ? dseeds : nullptr, (int)dcap, st);
+        int hbad = 0;
+        HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
+        CHK(sync_hdr(t));
+        if (hbad) {
+            t->err = "Add_Points: a voxel index floor(p/box_length) is beyond +-2^20";
+            return IKD_ERR_UNSUPPORTED;
+        }
+        *split_out = t->h_hdr->split_index;
+        *added_out = t->h_hdr->added;
+        if (!apply || t->h_hdr->split_index != INT_MAX) return IKD_OK;
     }
-    *split_out = t->h_hdr->split_index;
-    *added_out = t->h_hdr->added;
-    if (!apply || t->h_hdr->split_index != INT_MAX) return IKD_OK;
     int nv = t->h_hdr->group_count;
-    // compact: boxes to delete (downsample labels), then points to insert
-    float* cbox;
+    int ndel = t->h_hdr->rec_count;
+    // compact the points to insert (voxel order), then insert them and refit
+    // the insert paths and the deleted points' paths in one pass
     float4* cpts;
     int* counts;
-    CHK(ensure(t, B_CBOX, 6ll * nv, &cbox));
     CHK(ensure(t, B_CPTS, nv, &cpts));
     CHK(ensure(t, B_CNT64, 2, &counts));
     launch_fill_i32(counts, 0, 2, st);
-    CHK(exscan_i32(t, delfl, scan, nv));
-    launch_compact_boxes(delfl, scan, delbox, nv, t->d_hdr, cbox, counts, st);
-    int hc[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(&hc[0], counts, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    if (hc[0] > 0) CHK(delete_boxes_dev(t, cbox, hc[0], 1));
     CHK(exscan_i32(t, insfl, scan, nv));
     launch_compact_pts(insfl, scan, inspt, nv, t->d_hdr, cpts, counts + 1, st);
-    HIPCHK(hipMemcpyAsync(&hc[1], counts + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    int hc = 0;
+    HIPCHK(hipMemcpyAsync(&hc, counts + 1, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    return insert_points(t, cpts, hc[1]);
+    return insert_points(t, cpts, hc, dseeds, nullptr, ndel);
 }
 
 bool exact_voxels(float ds) {
@@ -815,6 +878,15 @@ int add_points_dev(ikd_tree_t t, const float* d_xyz, const uint32_t* d_ids, int 
     return compact_if_needed(t, false);
 }
 
+// Lanes per query so that a launch has about 2^17 lanes in flight: a small
+// batch (one scan) shares each query among a team of lanes, a large batch
+// keeps one lane per query (the most work-efficient form).
+int auto_lanes(int nq) {
+    int lanes = 1;
+    while (lanes < 16 && (long long)nq * lanes < (1ll << 17)) lanes *= 2;
+    return lanes;
+}
+
 float max_dist_threshold(double max_dist) {
     double m = max_dist * max_dist;
     if (std::isnan(m)) return -INFINITY;
@@ -842,6 +914,7 @@ int knn_dev(ikd_tree_t t, const float* d_q, int nq, int k, double max_dist, floa
     a.order = nullptr;
     a.nq = nq;
     a.k = k;
+    a.out_stride = k;
     a.thr = max_dist_threshold(max_dist);
     a.out_xyz = d_xyz;
     a.out_d2 = d_d2;
@@ -862,9 +935,14 @@ int knn_dev(ikd_tree_t t, const float* d_q, int nq, int k, double max_dist, floa
         CHK(sort_pairs(t, mk, mk2, mv, mv2, nq, 30));
         a.order = mv2;
     }
-    int stack = t->max_depth + 2 <= 32 ? 32 : 64;
+    // a lane defers at most one far child per level above its node
+    int stack = t->max_depth + 2;
     if (t->knn_stack > stack) stack = t->knn_stack;
-    if (launch_knn(a, stack, 64, t->st) != 0) return IKD_ERR_UNSUPPORTED;
+    if (stack > MAX_DEPTH) stack = MAX_DEPTH;
+    int lanes = t->knn_lanes >= 0 ? t->knn_lanes : auto_lanes(nq);
+    int* next;
+    CHK(ensure(t, B_PNEXT, 8, &next));
+    if (launch_knn(a, k, lanes, stack, t->st, next, 256 * 10) != 0) return IKD_ERR_UNSUPPORTED;
     HIPCHK(hipGetLastError());
     return IKD_OK;
 }
@@ -875,7 +953,7 @@ int range_dev(ikd_tree_t t, int mode, const float* d_boxes, const float* d_cente
     long long *cnt, *off;
     CHK(ensure(t, B_CNT64, nq + 1, &cnt));
     CHK(ensure(t, B_OFF64, nq + 1, &off));
-    HIPCHK(hipMemsetAsync(cnt + nq, 0, sizeof(long long), st));
+    launch_fill_i32(reinterpret_cast<int*>(cnt + nq), 0, 2, st);  // the scan's trailing zero
     RangeArgs a{};
     a.max_steps = t->pool_used + 16;
     a.pool = t->pool;
@@ -936,8 +1014,9 @@ int ikd_create(float delete_param, float balance_param, float box_length, ikd_tr
         hipMalloc(&t->d_hdr, sizeof(DevHdr)) != hipSuccess ||
         hipHostMalloc(&t->h_hdr, sizeof(DevHdr), hipHostMallocDefault) != hipSuccess ||
         hipMalloc(&t->d_root, sizeof(RootInfo)) != hipSuccess ||
-        hipHostMalloc(&t->h_root, sizeof(RootInfo), hipHostMallocDefault) != hipSuccess) {
-        delete t;
+        hipHostMalloc(&t->h_root, sizeof(RootInfo), hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&t->ev, hipEventDisableTiming) != hipSuccess) {
+        ikd_destroy(t);
         return IKD_ERR_HIP;
     }
     int rc = reset_pool(t);
@@ -961,6 +1040,7 @@ int ikd_destroy(ikd_tree_t t) {
     if (t->h_hdr) hipHostFree(t->h_hdr);
     if (t->d_root) hipFree(t->d_root);
     if (t->h_root) hipHostFree(t->h_root);
+    if (t->ev) hipEventDestroy(t->ev);
     if (t->st) hipStreamDestroy(t->st);
     delete t;
     return IKD_OK;
@@ -1231,6 +1311,78 @@ int ikd_flatten(ikd_tree_t t, float* out_xyz, uint32_t* out_ids, int64_t capacit
     return sync_hdr(t);
 }
 
+// Pre-order dump of the live structure (diagnostic; the row layout is the
+// oracle's: x, y, z, id, axis, TreeSize, invalid, flags, range[6], left, right).
+// Lazy labels are resolved on the way down, as flatten() does.
+int ikd_dump_nodes(ikd_tree_t t, void* out_rows, int64_t capacity, int64_t* out_n) {
+    CHECK_T();
+    if (!out_n) return IKD_ERR_ARG;
+    CHK(sync_hdr(t));
+    long long used = t->pool_used;
+    std::vector<NodeHot> hot(used);
+    std::vector<uint32_t> ids(used), meta(used);
+    std::vector<int4> cnt(used);
+    HIPCHK(hipMemcpy(hot.data(), t->pool.hot, used * sizeof(NodeHot), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ids.data(), t->pool.ids, used * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(meta.data(), t->pool.meta, used * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cnt.data(), t->pool.cnt, used * sizeof(int4), hipMemcpyDeviceToHost));
+    struct Row {
+        float x, y, z;
+        uint32_t id;
+        int32_t axis, tree_size, invalid, flags;
+        float range[6];
+        int32_t left, right;
+    };
+    std::vector<Row> rows;
+    uint32_t lk0 = fbits(hot[SENTINEL].f[3]);
+    if (lk0 & LK_HASL) {
+        struct Item {
+            int node, parent_row, side;
+            bool dead;
+            const float* box;
+        };
+        std::vector<Item> stack;
+        stack.push_back({ROOT, -1, 0, (lk0 & LK_TDL) != 0, hot[SENTINEL].f + 4});
+        while (!stack.empty()) {
+            Item it = stack.back();
+            stack.pop_back();
+            int u = it.node;
+            const float* f = hot[u].f;
+            uint32_t lk = fbits(f[3]);
+            uint32_t m = meta[u];
+            bool dead = it.dead || (m & M_TDEL);
+            Row r{};
+            r.x = f[0];
+            r.y = f[1];
+            r.z = f[2];
+            r.id = ids[u];
+            r.axis = (int)(m & M_AXIS);
+            r.tree_size = cnt[u].x;
+            r.invalid = dead ? cnt[u].x : cnt[u].y;
+            bool pd = dead || (m & M_PDEL);
+            r.flags = (pd ? 1 : 0) | (dead ? 2 : 0) | ((m & M_PDS) ? 4 : 0) | ((m & M_TDS) ? 8 : 0);
+            for (int i = 0; i < 6; ++i) r.range[i] = it.box[i];
+            r.left = r.right = -1;
+            int me = (int)rows.size();
+            rows.push_back(r);
+            if (it.parent_row >= 0) {
+                if (it.side == 0) rows[it.parent_row].left = me;
+                else rows[it.parent_row].right = me;
+            }
+            int pair = (int)lk_pair(lk);
+            for (int sd = 1; sd >= 0; --sd)
+                if (lk & lk_has(sd))
+                    stack.push_back({2 * pair + sd, me, sd, dead || (lk & lk_td(sd)), f + 4 + 6 * sd});
+        }
+    }
+    *out_n = (int64_t)rows.size();
+    if (out_rows) {
+        if (capacity < (int64_t)rows.size()) return IKD_ERR_CAPACITY;
+        std::memcpy(out_rows, rows.data(), rows.size() * sizeof(Row));
+    }
+    return IKD_OK;
+}
+
 int ikd_synchronize(ikd_tree_t t) {
     CHECK_T();
     return sync_hdr(t);
@@ -1250,8 +1402,15 @@ int ikd_set_tuning(ikd_tree_t t, int key, int value) {
             t->query_order = value ? 1 : 0;
             return IKD_OK;
         case IKD_TUNE_KNN_STACK:
-            if (value != 0 && value != 32 && value != 64) return IKD_ERR_ARG;
+            if (value < 0 || value > MAX_DEPTH) return IKD_ERR_ARG;
             t->knn_stack = value;
+            return IKD_OK;
+        case IKD_TUNE_SMALL_BUILD:
+            t->small_build = value ? 1 : 0;
+            return IKD_OK;
+        case IKD_TUNE_LANES_PER_QUERY:
+            if (value < -1 || value > 16 || (value > 0 && (value & (value - 1)))) return IKD_ERR_ARG;
+            t->knn_lanes = value;
             return IKD_OK;
         default:
             return IKD_ERR_ARG;
@@ -1271,6 +1430,175 @@ int ikd_pool_stats(ikd_tree_t t, int64_t* used, int64_t* capacity, int32_t* max_
 const char* ikd_last_error(ikd_tree_t t) {
     if (!t) return "null tree handle";
     return t->err.c_str();
+}
+
+// ------------------------------------------------------------ multi-tree batches
+enum BatchBuf { BB_TREES = 0, BB_THR, BB_NEXT, BB_FLAG };
+
+int ikd_batch_create(ikd_batch_t* out) {
+    if (!out) return IKD_ERR_ARG;
+    *out = nullptr;
+    ikd_batch_t b = new ikd_batch_s();
+    if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev, hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc(&b->h_flag, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        ikd_batch_destroy(b);
+        return IKD_ERR_HIP;
+    }
+    int* flag;
+    int rc = ensure(b, BB_FLAG, 1, &flag);
+    if (rc == IKD_OK) {
+        launch_fill_i32(flag, 0, 1, b->st);
+        if (hipStreamSynchronize(b->st) != hipSuccess) rc = IKD_ERR_HIP;
+    }
+    if (rc != IKD_OK) {
+        ikd_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return IKD_OK;
+}
+
+int ikd_batch_destroy(ikd_batch_t b) {
+    if (!b) return IKD_ERR_ARG;
+    if (b->st) hipStreamSynchronize(b->st);
+    for (auto& d : b->b)
+        if (d.p) hipFree(d.p);
+    if (b->h_flag) hipHostFree(b->h_flag);
+    if (b->ev) hipEventDestroy(b->ev);
+    if (b->st) hipStreamDestroy(b->st);
+    delete b;
+    return IKD_OK;
+}
+
+int ikd_batch_set_tuning(ikd_batch_t b, int key, int value) {
+    if (!b) return IKD_ERR_ARG;
+    switch (key) {
+        case IKD_TUNE_KNN_STACK:
+            if (value < 0 || value > MAX_DEPTH) return IKD_ERR_ARG;
+            b->knn_stack = value;
+            return IKD_OK;
+        case IKD_TUNE_LANES_PER_QUERY:
+            if (value < -1 || value > 16 || (value > 0 && (value & (value - 1)))) return IKD_ERR_ARG;
+            b->knn_lanes = value;
+            return IKD_OK;
+        default:
+            return IKD_ERR_ARG;
+    }
+}
+
+int ikd_batch_nearest_search_device(ikd_batch_t b, const ikd_tree_t* trees, int ntrees, const int32_t* d_query_tree,
+                                    const float* d_queries, int64_t nq, const int32_t* d_k, int k_max,
+                                    const double* d_max_dist, double max_dist, float* d_out_xyz, float* d_out_d2,
+                                    uint32_t* d_out_ids, int32_t* d_out_counts) {
+    if (!b) return IKD_ERR_ARG;
+    if (!trees || ntrees <= 0 || nq < 0 || nq >= (1ll << 31)) {
+        b->err = "batch: bad tree list or query count";
+        return IKD_ERR_ARG;
+    }
+    if (k_max < 1) {
+        b->err = "batch: k_max must be >= 1";
+        return IKD_ERR_ARG;
+    }
+    if (k_max > 64) {
+        b->err = "batch: k above 64 is not supported";
+        return IKD_ERR_UNSUPPORTED;
+    }
+    if (nq == 0) return IKD_OK;
+    if (!d_query_tree || !d_queries || !d_out_xyz || !d_out_d2 || !d_out_ids || !d_out_counts) {
+        b->err = "batch: null device pointer";
+        return IKD_ERR_ARG;
+    }
+    std::vector<TreeRef> refs((size_t)ntrees);
+    int depth = 0;
+    long long steps = 16;
+    for (int i = 0; i < ntrees; ++i) {
+        ikd_tree_t t = trees[i];
+        if (!t) {
+            b->err = "batch: null tree handle";
+            return IKD_ERR_ARG;
+        }
+        if (t->max_depth + 2 > MAX_DEPTH) {
+            int rc = compact_if_needed(t, true);
+            if (rc != IKD_OK) {
+                b->err = "batch: " + t->err;
+                return rc;
+            }
+        }
+        refs[i] = TreeRef{t->pool.hot, t->pool.ids};
+        depth = std::max(depth, t->max_depth);
+        steps = std::max(steps, t->pool_used + 16);
+    }
+    auto* t = b;  // HIPCHK reports into b->err
+    TreeRef* d_refs;
+    int *next, *flag;
+    CHK(ensure(b, BB_TREES, ntrees, &d_refs));
+    CHK(ensure(b, BB_NEXT, 8, &next));
+    CHK(ensure(b, BB_FLAG, 1, &flag));
+    for (int i = 0; i < ntrees; ++i) {
+        HIPCHK(hipEventRecord(trees[i]->ev, trees[i]->st));
+        HIPCHK(hipStreamWaitEvent(b->st, trees[i]->ev, 0));
+    }
+    HIPCHK(hipMemcpyAsync(d_refs, refs.data(), ntrees * sizeof(TreeRef), hipMemcpyHostToDevice, b->st));
+    KnnArgs a{};
+    a.trees = d_refs;
+    a.qtree = d_query_tree;
+    a.ntrees = ntrees;
+    a.q = d_queries;
+    a.nq = (int)nq;
+    a.k = k_max;
+    a.qk = d_k;
+    a.thr = max_dist_threshold(max_dist);
+    if (d_max_dist) {
+        float* thr;
+        CHK(ensure(b, BB_THR, nq, &thr));
+        launch_thresholds(d_max_dist, (int)nq, thr, b->st);
+        a.qthr = thr;
+    }
+    a.out_stride = k_max;
+    a.out_xyz = d_out_xyz;
+    a.out_d2 = d_out_d2;
+    a.out_ids = d_out_ids;
+    a.out_counts = d_out_counts;
+    a.overflow = flag;
+    a.max_steps = (int)std::min<long long>(steps, INT_MAX);
+    int stack = std::min(std::max(depth + 2, b->knn_stack), MAX_DEPTH);
+    int lanes = b->knn_lanes >= 0 ? b->knn_lanes : auto_lanes((int)nq);
+    if (launch_knn(a, k_max, lanes, stack, b->st, next, 256 * 10) != 0) {
+        b->err = "batch: unsupported launch shape";
+        return IKD_ERR_UNSUPPORTED;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(b->ev, b->st));
+    for (int i = 0; i < ntrees; ++i) HIPCHK(hipStreamWaitEvent(trees[i]->st, b->ev, 0));
+    return IKD_OK;
+}
+
+int ikd_batch_synchronize(ikd_batch_t b) {
+    if (!b) return IKD_ERR_ARG;
+    auto* t = b;
+    int* flag;
+    CHK(ensure(b, BB_FLAG, 1, &flag));
+    HIPCHK(hipMemcpyAsync(b->h_flag, flag, sizeof(int), hipMemcpyDeviceToHost, b->st));
+    HIPCHK(hipStreamSynchronize(b->st));
+    if (*b->h_flag) {
+        launch_fill_i32(flag, 0, 1, b->st);
+        HIPCHK(hipStreamSynchronize(b->st));
+        b->err = "batch: device traversal stack overflow";
+        return IKD_ERR_INTERNAL;
+    }
+    return IKD_OK;
+}
+
+int ikd_batch_stream(ikd_batch_t b, void** out_stream) {
+    if (!b || !out_stream) return IKD_ERR_ARG;
+    *out_stream = (void*)b->st;
+    return IKD_OK;
+}
+
+const char* ikd_batch_last_error(ikd_batch_t b) {
+    if (!b) return "null batch handle";
+    return b->err.c_str();
 }
 
 }  // extern "C"

@@ -1,0 +1,301 @@
+// ikd_build_small.hip — one-launch build of many small subtrees.
+//
+// The same construction as ikd_build.hip (BuildTree, ikd_Tree.cpp:573-622:
+// longest axis with ties to the lower axis, the element of rank mid-l with
+// mid = (l+r)>>1 as the node, stable presorted partitions so that equal keys
+// keep their input order), but one 256-thread workgroup builds one whole
+// segment of at most SMALL_MAX points in LDS: a bitonic sort per axis, then
+// one level per block-synchronous step.  Segments, pairs and children are
+// numbered in the same order as the global build, so a subtree built here is
+// the one the level-synchronous global build makes for the same input.
+//
+// Node allocation is deterministic: segment g takes the pair range that
+// starts at pair_used + exclusive_scan(pairs)[g], where pairs(m) is the number
+// of nodes with children in the median tree of m points.
+#include "ikd_internal.h"
+
+namespace ikd {
+
+constexpr int SB = 256;  // threads per workgroup
+constexpr int SMAX_SEG = SMALL_MAX / 2 + 1;
+
+// Nodes with at least one child in the median-split tree of m points.  A node
+// of size x >= 2 leaves children of sizes (x-1)>>1 and x-1-((x-1)>>1), so a
+// level holds at most two distinct sizes.
+__device__ int median_tree_pairs(int m) {
+    int sz[2] = {m, -1};
+    long long cnt[2] = {1, 0};
+    long long internal = 0;
+    while (cnt[0] > 0 || cnt[1] > 0) {
+        int nsz[2] = {-1, -1};
+        long long ncnt[2] = {0, 0};
+        for (int i = 0; i < 2; ++i) {
+            if (cnt[i] == 0 || sz[i] < 2) continue;
+            internal += cnt[i];
+            int parts[2] = {(sz[i] - 1) >> 1, sz[i] - 1 - ((sz[i] - 1) >> 1)};
+            for (int p = 0; p < 2; ++p) {
+                int x = parts[p];
+                if (x < 2) continue;
+                int slot = (nsz[0] == x || nsz[0] < 0) ? 0 : 1;
+                nsz[slot] = x;
+                ncnt[slot] += cnt[i];
+            }
+        }
+        sz[0] = nsz[0];
+        sz[1] = nsz[1];
+        cnt[0] = ncnt[0];
+        cnt[1] = ncnt[1];
+    }
+    return (int)internal;
+}
+
+__global__ void k_small_pairs(const int4* __restrict__ seg, int S, int* __restrict__ pairs) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= S) return;
+    pairs[g] = median_tree_pairs(seg[g].y - seg[g].x + 1);
+}
+
+__global__ void k_small_commit(const int* __restrict__ pairs, const int* __restrict__ scan, int S, DevHdr* hdr) {
+    hdr->pair_used += scan[S - 1] + pairs[S - 1];
+}
+
+__device__ __forceinline__ void write_child_box_s(NodeHot* hot, int parent, int side, const float* b) {
+    float* f = hot[parent].f;
+    if (side == 0) {
+        *reinterpret_cast<float4*>(f + 4) = make_float4(b[0], b[1], b[2], b[3]);
+        *reinterpret_cast<float2*>(f + 8) = make_float2(b[4], b[5]);
+    } else {
+        *reinterpret_cast<float2*>(f + 10) = make_float2(b[0], b[1]);
+        *reinterpret_cast<float4*>(f + 12) = make_float4(b[2], b[3], b[4], b[5]);
+    }
+}
+
+// Exclusive scan of one int per thread across the workgroup; *total gets the sum.
+__device__ int block_exscan(int v, int* tmp, int* total) {
+    const int t = threadIdx.x;
+    tmp[t] = v;
+    __syncthreads();
+    for (int off = 1; off < SB; off <<= 1) {
+        int x = t >= off ? tmp[t - off] : 0;
+        __syncthreads();
+        tmp[t] += x;
+        __syncthreads();
+    }
+    *total = tmp[SB - 1];
+    int ex = tmp[t] - v;
+    __syncthreads();
+    return ex;
+}
+
+__global__ __launch_bounds__(SB) void k_build_small(const float4* __restrict__ pts, const int4* __restrict__ seg,
+                                                     const int* __restrict__ seg_depth,
+                                                     const int* __restrict__ pair_scan, Pool pool, DevHdr* hdr) {
+    __shared__ float px[SMALL_MAX], py[SMALL_MAX], pz[SMALL_MAX];
+    __shared__ unsigned long long skey[SMALL_MAX];  // sort keys, then running counts
+    __shared__ unsigned short lst[2][3][SMALL_MAX];
+    __shared__ short sid[2][SMALL_MAX];
+    __shared__ unsigned char side[SMALL_MAX];
+    __shared__ short seg_l[2][SMAX_SEG], seg_r[2][SMAX_SEG];
+    __shared__ int seg_node[2][SMAX_SEG], seg_par[2][SMAX_SEG];
+    __shared__ unsigned char seg_axis[SMAX_SEG];
+    __shared__ short seg_mid[SMAX_SEG], seg_cl[SMAX_SEG], seg_cr[SMAX_SEG];
+    __shared__ int scan_tmp[SB];
+
+    const int g = blockIdx.x;
+    const int t = threadIdx.x;
+    const int4 sg = seg[g];
+    const int l0 = sg.x;
+    const int m = sg.y - sg.x + 1;
+    if (m <= 0 || m > SMALL_MAX) return;  // the host routes larger segments elsewhere
+    const int depth0 = seg_depth[g];
+    int next_pair = hdr->pair_used + pair_scan[g];
+
+    for (int i = t; i < m; i += SB) {
+        float4 p = pts[l0 + i];
+        px[i] = p.x;
+        py[i] = p.y;
+        pz[i] = p.z;
+    }
+    int P = 1;
+    while (P < m) P <<= 1;
+    __syncthreads();
+    // three stable sorted index lists: key (coordinate, input position)
+    for (int a = 0; a < 3; ++a) {
+        const float* c = a == 0 ? px : (a == 1 ? py : pz);
+        for (int i = t; i < P; i += SB)
+            skey[i] = i < m ? (((unsigned long long)float_key(c[i]) << 32) | (unsigned)i) : ~0ull;
+        __syncthreads();
+        for (int k = 2; k <= P; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = t; i < P; i += SB) {
+                    int ixj = i ^ j;
+                    if (ixj > i) {
+                        unsigned long long x = skey[i], y = skey[ixj];
+                        bool up = (i & k) == 0;
+                        if ((x > y) == up) {
+                            skey[i] = y;
+                            skey[ixj] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int i = t; i < m; i += SB) lst[0][a][i] = (unsigned short)(skey[i] & 0xffffu);
+        __syncthreads();
+    }
+    for (int i = t; i < m; i += SB) sid[0][i] = 0;
+    if (t == 0) {
+        seg_l[0][0] = 0;
+        seg_r[0][0] = (short)(m - 1);
+        seg_node[0][0] = sg.z;
+        seg_par[0][0] = sg.w;
+    }
+    __syncthreads();
+    unsigned int* run = reinterpret_cast<unsigned int*>(skey);
+    const int chunk = (m + SB - 1) / SB;
+    int ns = 1;
+    int cur = 0;
+    for (int level = 0; ns > 0; ++level) {
+        const int depth = depth0 + level;
+        const int per = (ns + SB - 1) / SB;  // contiguous segments per thread keep the order
+        int my_pairs = 0, my_children = 0;
+        for (int q = 0; q < per; ++q) {
+            const int s = t * per + q;
+            if (s >= ns) break;
+            const int l = seg_l[cur][s], r = seg_r[cur][s];
+            float ext[3];
+            ext[0] = px[lst[cur][0][r]] - px[lst[cur][0][l]];
+            ext[1] = py[lst[cur][1][r]] - py[lst[cur][1][l]];
+            ext[2] = pz[lst[cur][2][r]] - pz[lst[cur][2][l]];
+            int axis = 0;
+            if (ext[1] > ext[axis]) axis = 1;
+            if (ext[2] > ext[axis]) axis = 2;
+            const int mid = (l + r) >> 1;
+            seg_axis[s] = (unsigned char)axis;
+            seg_mid[s] = (short)mid;
+            const int hl = mid > l, hr = mid < r;
+            my_pairs += (hl || hr) ? 1 : 0;
+            my_children += hl + hr;
+        }
+        int tot_pairs, tot_children;
+        int pair_off = block_exscan(my_pairs, scan_tmp, &tot_pairs);
+        int child_off = block_exscan(my_children, scan_tmp, &tot_children);
+        for (int q = 0; q < per; ++q) {
+            const int s = t * per + q;
+            if (s >= ns) break;
+            const int l = seg_l[cur][s], r = seg_r[cur][s], mid = seg_mid[s], axis = seg_axis[s];
+            const int node = seg_node[cur][s], parent = seg_par[cur][s];
+            const int hl = mid > l, hr = mid < r;
+            const int pair = (hl || hr) ? next_pair + pair_off++ : 0;
+            const int piv = lst[cur][axis][mid];
+            const uint32_t link = ((uint32_t)pair << LK_PAIR_SHIFT) | (hl ? LK_HASL : 0u) | (hr ? LK_HASR : 0u);
+            const float4 P4 = pts[l0 + piv];
+            *reinterpret_cast<float4*>(pool.hot[node].f) = make_float4(P4.x, P4.y, P4.z, bitsf(link));
+            pool.ids[node] = fbits(P4.w);
+            pool.par[node] = parent;
+            pool.cnt[node] = make_int4(r - l + 1, 0, 0, depth);
+            pool.meta[node] = (uint32_t)axis;
+            float box[6] = {px[lst[cur][0][l]], py[lst[cur][1][l]], pz[lst[cur][2][l]],
+                            px[lst[cur][0][r]], py[lst[cur][1][r]], pz[lst[cur][2][r]]};
+            write_child_box_s(pool.hot, parent, node & 1, box);
+            int cl = -1, cr = -1;
+            if (hl) {
+                cl = child_off++;
+                seg_l[cur ^ 1][cl] = (short)l;
+                seg_r[cur ^ 1][cl] = (short)(mid - 1);
+                seg_node[cur ^ 1][cl] = 2 * pair;
+                seg_par[cur ^ 1][cl] = node;
+            }
+            if (hr) {
+                cr = child_off++;
+                seg_l[cur ^ 1][cr] = (short)(mid + 1);
+                seg_r[cur ^ 1][cr] = (short)r;
+                seg_node[cur ^ 1][cr] = 2 * pair + 1;
+                seg_par[cur ^ 1][cr] = node;
+            }
+            seg_cl[s] = (short)cl;
+            seg_cr[s] = (short)cr;
+        }
+        if (t == 0) atomicMax(&hdr->max_depth, depth);
+        next_pair += tot_pairs;
+        __syncthreads();
+        if (tot_children == 0) break;
+        // side of every live position, read off its segment's axis list
+        for (int j = t; j < m; j += SB) {
+            const int s = sid[cur][j];
+            if (s < 0) continue;
+            const int mid = seg_mid[s];
+            side[lst[cur][seg_axis[s]][j]] = (unsigned char)(j < mid ? 0 : (j == mid ? 2 : 1));
+        }
+        __syncthreads();
+        // stable partition of each list: running counts of left / right
+        // entries before every position, then offsets inside the segment
+        for (int b = 0; b < 3; ++b) {
+            int cl = 0, cr = 0;
+            for (int q = 0; q < chunk; ++q) {
+                const int j = t * chunk + q;
+                if (j >= m) break;
+                if (sid[cur][j] < 0) continue;
+                const unsigned char sd = side[lst[cur][b][j]];
+                cl += sd == 0;
+                cr += sd == 1;
+            }
+            int tl_, tr_;
+            int a0 = block_exscan(cl, scan_tmp, &tl_);
+            int a1 = block_exscan(cr, scan_tmp, &tr_);
+            for (int q = 0; q < chunk; ++q) {
+                const int j = t * chunk + q;
+                if (j >= m) break;
+                run[2 * j] = (unsigned)a0;
+                run[2 * j + 1] = (unsigned)a1;
+                if (sid[cur][j] < 0) continue;
+                const unsigned char sd = side[lst[cur][b][j]];
+                a0 += sd == 0;
+                a1 += sd == 1;
+            }
+            __syncthreads();
+            for (int q = 0; q < chunk; ++q) {
+                const int j = t * chunk + q;
+                if (j >= m) break;
+                const int s = sid[cur][j];
+                if (s < 0) {
+                    if (b == 0) sid[cur ^ 1][j] = -1;
+                    continue;
+                }
+                const int p = lst[cur][b][j];
+                const unsigned char sd = side[p];
+                if (sd == 2) {
+                    if (b == 0) sid[cur ^ 1][seg_mid[s]] = -1;
+                    continue;
+                }
+                const int l = seg_l[cur][s], mid = seg_mid[s];
+                const int np = sd == 0 ? l + (int)(run[2 * j] - run[2 * l])
+                                       : mid + 1 + (int)(run[2 * j + 1] - run[2 * l + 1]);
+                lst[cur ^ 1][b][np] = (unsigned short)p;
+                if (b == 0) sid[cur ^ 1][np] = sd == 0 ? seg_cl[s] : seg_cr[s];
+            }
+            __syncthreads();
+        }
+        ns = tot_children;
+        cur ^= 1;
+    }
+}
+
+static inline int blocks_for(long long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+void launch_small_pairs(const int4* seg, int S, int* pairs, hipStream_t s) {
+    if (S <= 0) return;
+    hipLaunchKernelGGL(k_small_pairs, dim3(blocks_for(S, 256)), dim3(256), 0, s, seg, S, pairs);
+}
+void launch_build_small(const float4* pts, const int4* seg, const int* seg_depth, const int* pair_scan, int S, Pool pool,
+                        DevHdr* hdr, hipStream_t s) {
+    if (S <= 0) return;
+    hipLaunchKernelGGL(k_build_small, dim3(S), dim3(SB), 0, s, pts, seg, seg_depth, pair_scan, pool, hdr);
+}
+void launch_small_commit(const int* pairs, const int* pair_scan, int S, DevHdr* hdr, hipStream_t s) {
+    if (S <= 0) return;
+    hipLaunchKernelGGL(k_small_commit, dim3(1), dim3(1), 0, s, pairs, pair_scan, S, hdr);
+}
+
+}  // namespace ikd

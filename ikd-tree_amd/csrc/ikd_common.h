@@ -59,6 +59,7 @@ constexpr int SENTINEL = 0;
 constexpr int ROOT = 2;
 constexpr int MAX_DEPTH = 64;  // deepest tree the traversal stacks accept
 constexpr int WALK_LIMIT = 4096;  // bound on any root-to-leaf or leaf-to-root walk
+constexpr int SMALL_MAX = 1024;   // largest subtree built by one workgroup in LDS
 
 // ---- tunables from the reference (ikd_Tree.h:13-18) ----------------------
 constexpr double EPSS = 1e-6;

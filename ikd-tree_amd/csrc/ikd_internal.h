@@ -76,6 +76,11 @@ void launch_build_side(const BuildLevel& L, DevHdr* hdr, hipStream_t s);
 void launch_build_flags(const BuildLevel& L, hipStream_t s);
 void launch_build_scatter(const BuildLevel& L, hipStream_t s);
 
+void launch_small_pairs(const int4* seg, int S, int* pairs, hipStream_t s);
+void launch_build_small(const float4* pts, const int4* seg, const int* seg_depth, const int* pair_scan, int S, Pool pool,
+                        DevHdr* hdr, hipStream_t s);
+void launch_small_commit(const int* pairs, const int* pair_scan, int S, DevHdr* hdr, hipStream_t s);
+
 // ---------------------------------------------------------------- refit
 void launch_mark_dirty(Pool pool, const int* seeds, const int* nseeds_dev, int bound, int stop_at_tdel,
                        int* dirty, DevHdr* hdr, hipStream_t s);
@@ -117,21 +122,33 @@ void launch_voxel_flags(const unsigned long long* keys, int n, int* flag, hipStr
 void launch_voxel_starts(const int* flag, const int* scan, int n, int* vstart, DevHdr* hdr, hipStream_t s);
 void launch_voxel_fold2(Pool pool, const float4* pts, const uint32_t* order, const unsigned long long* keys,
                         const int* vstart, int n, int bound, float ds, int check, float* del_boxes, int* del_flag,
-                        float4* ins_pts, int* ins_flag, DevHdr* hdr, long long max_steps, hipStream_t s);
+                        float4* ins_pts, int* ins_flag, DevHdr* hdr, long long max_steps, int* del_out, int del_cap,
+                        hipStream_t s);
 void launch_compact_boxes(const int* flag, const int* scan, const float* in, int bound, DevHdr* hdr, float* out,
                           int* out_count, hipStream_t s);
 void launch_compact_pts(const int* flag, const int* scan, const float4* in, int bound, DevHdr* hdr, float4* out,
                         int* out_count, hipStream_t s);
 
 // ---------------------------------------------------------------- queries
-struct KnnArgs {
+struct TreeRef {
     const NodeHot* hot;
     const uint32_t* ids;
+};
+
+struct KnnArgs {
+    const NodeHot* hot;     // single-tree launch
+    const uint32_t* ids;
+    const TreeRef* trees;   // multi-tree launch: tree table, with qtree[q] per query (else null)
+    const int* qtree;
+    int ntrees;             // entries of trees[]; a query naming no valid tree finds nothing
     const float* q;         // packed xyz
     const uint32_t* order;  // optional query permutation (Morton order), may be null
     int nq;
-    int k;
-    float thr;              // largest float <= max_dist^2 (double)
+    int k;                  // uniform k, or the bound on per-query qk[q] (clamped to [0, k])
+    float thr;              // largest float <= max_dist^2 (double); or per-query qthr[q]
+    const int* qk;
+    const float* qthr;
+    int out_stride;         // result slots per query
     float* out_xyz;
     float* out_d2;
     uint32_t* out_ids;
@@ -140,8 +157,16 @@ struct KnnArgs {
     unsigned int* boxes;    // optional: child boxes evaluated per query
     int* overflow;
     int max_steps;          // visit bound per query (the pool size): a corrupt tree cannot hang a lane
+    int stack;              // LDS stack entries per lane (set by launch_knn)
 };
-int launch_knn(const KnnArgs& a, int stack, int block, hipStream_t s);
+// lanes_per_query 0 (persistent lanes), 1 (one lane per query) or 2, 4, 8, 16 (a team of lanes
+// shares one query's traversal); stack = LDS entries per lane (1-lane) or
+// per-lane share of the team stack.
+int launch_knn(const KnnArgs& a, int kmax, int lanes_per_query, int stack, hipStream_t s, int* persist_next,
+               int persist_waves);
+
+// per-query thresholds: thr[i] = largest float <= max_dist[i]^2 (double); NaN gives -inf
+void launch_thresholds(const double* max_dist, int n, float* thr, hipStream_t s);
 
 void launch_morton(const float* q, int nq, const NodeHot* hot, unsigned long long* keys, uint32_t* vals,
                    hipStream_t s);

@@ -22,6 +22,8 @@
 // max_dist: the reference compares in double (cpp:872, 887).  The host
 // passes thr = the largest float <= max_dist*max_dist, for which
 // (double)d <= max_dist*max_dist  <=>  d <= thr  for every float d.
+#include <algorithm>
+
 #include "ikd_internal.h"
 
 namespace ikd {
@@ -46,106 +48,229 @@ __device__ __forceinline__ bool after(const NodeHot* hot, const uint32_t* ids, f
     return tie_less(hot, ids, n, en);
 }
 
-template <int KM, int STACK>
-__global__ __launch_bounds__(64) void k_knn(KnnArgs a) {
-    __shared__ int st_n[STACK][64];
-    __shared__ float st_d[STACK][64];
-    const int lane = threadIdx.x;
-    const int gq = blockIdx.x * 64 + lane;
-    if (gq >= a.nq) return;
-    const int qi = a.order ? (int)a.order[gq] : gq;
-    const float qx = a.q[3 * qi], qy = a.q[3 * qi + 1], qz = a.q[3 * qi + 2];
-    const NodeHot* __restrict__ hot = a.hot;
-    const float thr = a.thr;
-    const int k = a.k;
+// Sorted best-k list of one lane: bd_[0] is the current k-th candidate,
+// slots past k hold (-inf, -2) padding.
+template <int KM>
+struct BestList {
+    float d[KM];
+    int n[KM];
+    __device__ __forceinline__ void init(int k) {
+#pragma unroll
+        for (int i = 0; i < KM; ++i) {
+            d[i] = i < k ? INFINITY : -INFINITY;
+            n[i] = i < k ? -1 : -2;
+        }
+    }
+    // offer (dist, node); keeps the k smallest in (d2, x, y, z, id) order
+    __device__ __forceinline__ void offer(const NodeHot* hot, const uint32_t* ids, float dist, int node) {
+        if (!after(hot, ids, d[0], n[0], dist, node)) return;
+        bool gnext[KM];
+#pragma unroll
+        for (int i = 0; i < KM; ++i)
+            gnext[i] = (i + 1 < KM) ? after(hot, ids, d[i + 1], n[i + 1], dist, node) : false;
+#pragma unroll
+        for (int i = 0; i < KM; ++i) {
+            bool gcur = (i == 0) ? true : gnext[i - 1];
+            if (gnext[i]) {
+                d[i] = d[i + 1 < KM ? i + 1 : i];
+                n[i] = n[i + 1 < KM ? i + 1 : i];
+            } else if (gcur) {
+                d[i] = dist;
+                n[i] = node;
+            }
+        }
+    }
+};
 
-    // descending best list: slot 0 is the current k-th candidate
-    float bd_[KM];
-    int bn_[KM];
+struct QueryCtx {
+    const NodeHot* hot;
+    const uint32_t* ids;
+    int qo;      // query index in the caller's arrays
+    int k;
+    float thr;
+    float qx, qy, qz;
+};
+
+__device__ __forceinline__ QueryCtx load_query(const KnnArgs& a, int qi) {
+    QueryCtx c;
+    c.qo = a.order ? (int)a.order[qi] : qi;
+    c.k = a.qk ? min(max(a.qk[c.qo], 0), a.k) : a.k;
+    if (a.trees) {
+        const int ti = a.qtree[c.qo];
+        const bool ok = ti >= 0 && ti < a.ntrees;
+        const TreeRef t = a.trees[ok ? ti : 0];
+        c.hot = t.hot;
+        c.ids = t.ids;
+        if (!ok) c.k = 0;
+    } else {
+        c.hot = a.hot;
+        c.ids = a.ids;
+    }
+    c.thr = a.qthr ? a.qthr[c.qo] : a.thr;
+    c.qx = a.q[3 * c.qo];
+    c.qy = a.q[3 * c.qo + 1];
+    c.qz = a.q[3 * c.qo + 2];
+    return c;
+}
+
+__device__ __forceinline__ void write_slot(const KnnArgs& a, const QueryCtx& c, int j, float d, int node) {
+    const long long o = (long long)c.qo * a.out_stride + j;
+    if (node >= 0) {
+        const float* f = c.hot[node].f;
+        a.out_xyz[3 * o] = f[0];
+        a.out_xyz[3 * o + 1] = f[1];
+        a.out_xyz[3 * o + 2] = f[2];
+        a.out_d2[o] = d;
+        a.out_ids[o] = c.ids[node];
+    } else {
+        a.out_xyz[3 * o] = 0.0f;
+        a.out_xyz[3 * o + 1] = 0.0f;
+        a.out_xyz[3 * o + 2] = 0.0f;
+        a.out_d2[o] = INFINITY;
+        a.out_ids[o] = 0xffffffffu;
+    }
+}
+
+// Ascending output: the list holds the k best in descending order with the
+// real entries in its last `found` slots.  Slots [found, out_stride) of the
+// query's row are written empty.
+template <int KM>
+__device__ __forceinline__ void finish_query(const KnnArgs& a, const QueryCtx& c, const BestList<KM>& best,
+                                             unsigned int visits, unsigned int boxes) {
+    int found = 0;
+#pragma unroll
+    for (int i = 0; i < KM; ++i) found += best.n[i] >= 0 ? 1 : 0;
+    a.out_counts[c.qo] = found;
 #pragma unroll
     for (int i = 0; i < KM; ++i) {
-        bd_[i] = i < k ? INFINITY : -INFINITY;
-        bn_[i] = i < k ? -1 : -2;
+        if (i < c.k) write_slot(a, c, c.k - 1 - i, best.d[i], best.n[i]);
     }
+    for (int j = c.k; j < a.out_stride; ++j) write_slot(a, c, j, INFINITY, -1);
+    if (a.visits) a.visits[c.qo] = visits;
+    if (a.boxes) a.boxes[c.qo] = boxes;
+}
+
+// One visit of node n: offer its point, then compute both children's box
+// distances.  Returns the near child (or -1) and, through far/df, the far one.
+template <int KM>
+__device__ __forceinline__ void visit_node(const QueryCtx& c, BestList<KM>& best, int n, int& near_n, float& dn,
+                                           int& far_n, float& df, unsigned int& boxes) {
+    const float4* rec = reinterpret_cast<const float4*>(c.hot[n].f);
+    const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
+    const uint32_t lk = fbits(r0.w);
+    if (!(lk & LK_PDEL)) {
+        float d = calc_dist(c.qx, c.qy, c.qz, r0.x, r0.y, r0.z);
+        if (d <= c.thr) best.offer(c.hot, c.ids, d, n);
+    }
+    const int pair = (int)lk_pair(lk);
+    const bool hl = (lk & LK_HASL) != 0, hr = (lk & LK_HASR) != 0;
+    boxes += (hl ? 1u : 0u) + (hr ? 1u : 0u);
+    const bool el = hl && !(lk & LK_TDL), er = hr && !(lk & LK_TDR);
+    float bl[6] = {r1.x, r1.y, r1.z, r1.w, r2.x, r2.y};
+    float br[6] = {r2.z, r2.w, r3.x, r3.y, r3.z, r3.w};
+    const float dl = el ? calc_box_dist(bl, c.qx, c.qy, c.qz) : INFINITY;
+    const float dr = er ? calc_box_dist(br, c.qx, c.qy, c.qz) : INFINITY;
+    near_n = -1;
+    far_n = -1;
+    if (el || er) {
+        const bool near_left = el && (!er || dl <= dr);
+        near_n = near_left ? 2 * pair : 2 * pair + 1;
+        dn = near_left ? dl : dr;
+        if (near_left ? er : el) {
+            far_n = near_left ? 2 * pair + 1 : 2 * pair;
+            df = near_left ? dr : dl;
+        }
+    }
+}
+
+// ---- LDS traversal stacks --------------------------------------------------
+// Each lane's deferred far children live in dynamic LDS as [stack][64] node
+// indices followed by [stack][64] 16-bit box distances.  A stored distance is
+// the float's upper half (sign, exponent, 7 mantissa bits): for the
+// non-negative box distances that is a round-down, so a popped entry is
+// admitted whenever the exact distance would be (the visit then offers
+// nothing the exact test would have rejected, it only costs the visit).  Six
+// bytes per entry instead of eight, and the stack depth follows the tree depth
+// at run time, keep more wavefronts resident per CU.
+struct LdsStack {
+    int* n;
+    unsigned short* d;
+};
+__device__ __forceinline__ LdsStack lds_stack(int stack) {
+    extern __shared__ int lds_dyn[];
+    return LdsStack{lds_dyn, reinterpret_cast<unsigned short*>(lds_dyn + stack * 64)};
+}
+__device__ __forceinline__ unsigned short dist_key(float d) { return (unsigned short)(fbits(d) >> 16); }
+__device__ __forceinline__ float dist_val(unsigned short k) { return bitsf((uint32_t)k << 16); }
+static inline size_t lds_stack_bytes(int stack) { return (size_t)stack * 64 * (sizeof(int) + sizeof(unsigned short)); }
+
+// ---- one lane per query -------------------------------------------------
+// Workgroups are dispatched to the 8 XCDs round-robin; map block b to query
+// chunk (b % 8) * per + b / 8 so each XCD (and its L2) works on one
+// contiguous eighth of the batch, i.e. on one spatial region of a scan.
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcd_chunk(int b, int nchunks) {
+    const int per = (nchunks + NUM_XCD - 1) / NUM_XCD;
+    return (b % NUM_XCD) * per + b / NUM_XCD;
+}
+static inline int xcd_grid(int nchunks) {
+    const int per = (nchunks + 7) / 8;
+    return per * 8;
+}
+
+template <int KM>
+__global__ __launch_bounds__(64) void k_knn(KnnArgs a) {
+    const LdsStack st = lds_stack(a.stack);
+    const int lane = threadIdx.x;
+    const int chunk = xcd_chunk(blockIdx.x, (a.nq + 63) / 64);
+    const int gq = chunk * 64 + lane;
+    if (gq >= a.nq) return;
+    const QueryCtx c = load_query(a, gq);
+    BestList<KM> best;
+    best.init(c.k);
     unsigned int visits = 0, boxes = 0;
 
-    const uint32_t lk0 = fbits(hot[SENTINEL].f[3]);
+    const uint32_t lk0 = fbits(c.hot[SENTINEL].f[3]);
     int sp = 0;
     int n = -1;
     float bd = 0.0f;
     if ((lk0 & LK_HASL) && !(lk0 & LK_TDL)) {
         n = ROOT;
-        bd = calc_box_dist(hot[SENTINEL].f + 4, qx, qy, qz);
+        bd = calc_box_dist(c.hot[SENTINEL].f + 4, c.qx, c.qy, c.qz);
     }
     while (true) {
         if ((int)visits > a.max_steps) {
             atomicOr(a.overflow, 2);
             break;
         }
-        if (n >= 0 && bd <= thr) {
-            const float4* rec = reinterpret_cast<const float4*>(hot[n].f);
-            const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
-            const uint32_t lk = fbits(r0.w);
+        if (n >= 0 && bd <= c.thr) {
             ++visits;
-            if (!(lk & LK_PDEL)) {
-                float d = calc_dist(qx, qy, qz, r0.x, r0.y, r0.z);
-                if (d <= thr && after(hot, a.ids, bd_[0], bn_[0], d, n)) {
-                    // drop slot 0, insert (d, n) keeping descending order
-                    bool gnext[KM];
-#pragma unroll
-                    for (int i = 0; i < KM; ++i) gnext[i] = (i + 1 < KM) ? after(hot, a.ids, bd_[i + 1], bn_[i + 1], d, n) : false;
-#pragma unroll
-                    for (int i = 0; i < KM; ++i) {
-                        bool gcur = (i == 0) ? true : gnext[i - 1];
-                        if (gnext[i]) {
-                            bd_[i] = bd_[i + 1 < KM ? i + 1 : i];
-                            bn_[i] = bn_[i + 1 < KM ? i + 1 : i];
-                        } else if (gcur) {
-                            bd_[i] = d;
-                            bn_[i] = n;
-                        }
-                    }
+            int nn, fn;
+            float dn = 0.0f, df = 0.0f;
+            visit_node<KM>(c, best, n, nn, dn, fn, df, boxes);
+            const float top = best.d[0];
+            if (fn >= 0 && df <= top && df <= c.thr) {
+                if (sp < a.stack) {
+                    st.n[sp * 64 + lane] = fn;
+                    st.d[sp * 64 + lane] = dist_key(df);
+                    ++sp;
+                } else {
+                    atomicOr(a.overflow, 1);
                 }
             }
-            const int pair = (int)lk_pair(lk);
-            const bool hl = (lk & LK_HASL) != 0, hr = (lk & LK_HASR) != 0;
-            boxes += (hl ? 1u : 0u) + (hr ? 1u : 0u);
-            const bool el = hl && !(lk & LK_TDL), er = hr && !(lk & LK_TDR);
-            float bl[6] = {r1.x, r1.y, r1.z, r1.w, r2.x, r2.y};
-            float br[6] = {r2.z, r2.w, r3.x, r3.y, r3.z, r3.w};
-            const float dl = el ? calc_box_dist(bl, qx, qy, qz) : INFINITY;
-            const float dr = er ? calc_box_dist(br, qx, qy, qz) : INFINITY;
-            if (el || er) {
-                const bool near_left = el && (!er || dl <= dr);
-                const int nn = near_left ? 2 * pair : 2 * pair + 1;
-                const float dn = near_left ? dl : dr;
-                const bool has_far = near_left ? er : el;
-                const int fn = near_left ? 2 * pair + 1 : 2 * pair;
-                const float df = near_left ? dr : dl;
-                const float top = bd_[0];
-                if (has_far && df <= top && df <= thr) {
-                    if (sp < STACK) {
-                        st_n[sp][lane] = fn;
-                        st_d[sp][lane] = df;
-                        ++sp;
-                    } else {
-                        atomicOr(a.overflow, 1);
-                    }
-                }
-                if (dn <= top && dn <= thr) {
-                    n = nn;
-                    bd = dn;
-                    continue;
-                }
+            if (nn >= 0 && dn <= top && dn <= c.thr) {
+                n = nn;
+                bd = dn;
+                continue;
             }
         }
         // backtrack to the deepest deferred child that can still improve
         n = -1;
         while (sp > 0) {
             --sp;
-            float d = st_d[sp][lane];
-            if (d <= bd_[0]) {
-                n = st_n[sp][lane];
+            const float d = dist_val(st.d[sp * 64 + lane]);
+            if (d <= best.d[0]) {
+                n = st.n[sp * 64 + lane];
                 bd = d;
                 break;
             }
@@ -153,60 +278,340 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs a) {
         if (n < 0) break;
     }
 
-    // ascending output: real entries sit in slots [k-found, k-1]
-    int found = 0;
-#pragma unroll
-    for (int i = 0; i < KM; ++i) found += bn_[i] >= 0 ? 1 : 0;
-    a.out_counts[qi] = found;
-#pragma unroll
-    for (int i = 0; i < KM; ++i) {
-        if (i >= k) break;
-        const int j = k - 1 - i;
-        const long long o = (long long)qi * k + j;
-        const int node = bn_[i];
-        if (node >= 0) {
-            const float* f = hot[node].f;
-            a.out_xyz[3 * o] = f[0];
-            a.out_xyz[3 * o + 1] = f[1];
-            a.out_xyz[3 * o + 2] = f[2];
-            a.out_d2[o] = bd_[i];
-            a.out_ids[o] = a.ids[node];
-        } else {
-            a.out_xyz[3 * o] = 0.0f;
-            a.out_xyz[3 * o + 1] = 0.0f;
-            a.out_xyz[3 * o + 2] = 0.0f;
-            a.out_d2[o] = INFINITY;
-            a.out_ids[o] = 0xffffffffu;
+    finish_query<KM>(a, c, best, visits, boxes);
+}
+
+// ---- a team of T lanes per query ------------------------------------------
+// Each lane of a team runs its own depth-first walk with its own deque of
+// deferred far children (a ring buffer in LDS holding at most one entry per
+// level of its current walk, so never more than the tree depth).  Whenever a
+// lane is idle, a busy teammate donates the bottom (shallowest) entry of its
+// deque.  Each lane keeps its own best-k list; the pruning bound is the
+// smallest k-th distance of the team, which is never below the true k-th
+// distance, so nothing of the answer is pruned.  At the end the T lists are
+// merged by rank in the total order.  Every node is reached through exactly
+// one parent, so no point is offered twice and the merge sees distinct
+// candidates.
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int KM, int T>
+__global__ __launch_bounds__(64) void k_knn_team(KnnArgs a) {
+    constexpr int TEAMS = 64 / T;
+    const LdsStack st = lds_stack(a.stack);
+    const int smask = a.stack - 1;  // team launches use a power-of-two stack
+    __shared__ int dn_n[TEAMS][T];
+    __shared__ float dn_d[TEAMS][T];
+    __shared__ float mg_d[TEAMS][T * KM];
+    __shared__ int mg_n[TEAMS][T * KM];
+    const int lane = threadIdx.x;
+    const int team = lane / T, tl = lane % T;
+    const unsigned long long team_mask = (T == 64) ? ~0ull : (((1ull << T) - 1ull) << (team * T));
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int qi = xcd_chunk(blockIdx.x, (a.nq + TEAMS - 1) / TEAMS) * TEAMS + team;
+    const bool valid = qi < a.nq;
+    QueryCtx c = load_query(a, valid ? qi : 0);
+    if (!valid) c.k = 1;
+    BestList<KM> best;
+    best.init(c.k);
+    unsigned int visits = 0, boxes = 0;
+    int head = 0, tail = 0;  // deque [head, tail) modulo a.stack
+
+    int n = -1;
+    float bd = 0.0f;
+    if (valid && tl == 0) {
+        const uint32_t lk0 = fbits(c.hot[SENTINEL].f[3]);
+        if ((lk0 & LK_HASL) && !(lk0 & LK_TDL)) {
+            n = ROOT;
+            bd = calc_box_dist(c.hot[SENTINEL].f + 4, c.qx, c.qy, c.qz);
         }
     }
-    if (a.visits) a.visits[qi] = visits;
-    if (a.boxes) a.boxes[qi] = boxes;
+    float B = INFINITY;  // team bound: min over the team's k-th distances
+    int iter = 0;
+    while (true) {
+        // an idle lane resumes the deepest entry of its own deque
+        while (n < 0 && tail != head) {
+            --tail;
+            const int s = tail & smask;
+            const float d = dist_val(st.d[s * 64 + lane]);
+            if (d <= fminf(B, best.d[0])) {
+                n = st.n[s * 64 + lane];
+                bd = d;
+            }
+        }
+        // lanes still idle receive the bottom entry of a busy teammate
+        const unsigned long long imask = __ballot(valid && n < 0) & team_mask;
+        const unsigned long long dmask = __ballot(tail != head) & team_mask;
+        if (imask != 0 && dmask != 0) {
+            const int ni = __popcll(imask), nd = __popcll(dmask);
+            if (tail != head) {
+                const int r = __popcll(dmask & below);
+                if (r < ni) {
+                    const int s = head & smask;
+                    dn_n[team][r] = st.n[s * 64 + lane];
+                    dn_d[team][r] = dist_val(st.d[s * 64 + lane]);
+                    ++head;
+                }
+            }
+            wave_lds_fence();
+            if (valid && n < 0) {
+                const int r = __popcll(imask & below);
+                if (r < nd) {
+                    n = dn_n[team][r];
+                    bd = dn_d[team][r];
+                }
+            }
+            wave_lds_fence();
+        }
+        if (__ballot(n >= 0 || tail != head) == 0) break;
+        if (++iter > a.max_steps) {
+            if (lane == 0) atomicOr(a.overflow, 2);
+            break;
+        }
+
+        if (n >= 0) {
+            float bound = fminf(B, best.d[0]);
+            if (bd > c.thr || bd > bound) {
+                n = -1;
+            } else {
+                ++visits;
+                int nn, fn;
+                float dn = 0.0f, df = 0.0f;
+                visit_node<KM>(c, best, n, nn, dn, fn, df, boxes);
+                bound = fminf(B, best.d[0]);
+                if (fn >= 0 && df <= bound && df <= c.thr) {
+                    if (tail - head < a.stack) {
+                        const int s = tail & smask;
+                        st.n[s * 64 + lane] = fn;
+                        st.d[s * 64 + lane] = dist_key(df);
+                        ++tail;
+                    } else {
+                        atomicOr(a.overflow, 1);
+                    }
+                }
+                if (nn >= 0 && dn <= bound && dn <= c.thr) {
+                    n = nn;
+                    bd = dn;
+                } else {
+                    n = -1;
+                }
+            }
+        }
+        float v = best.d[0];
+#pragma unroll
+        for (int off = T / 2; off >= 1; off >>= 1) v = fminf(v, __shfl_xor(v, off, T));
+        B = v;
+    }
+
+    // ---- merge the team's lists by rank in (d2, x, y, z, id) order
+#pragma unroll
+    for (int i = 0; i < KM; ++i) {
+        mg_d[team][tl * KM + i] = best.d[i];
+        mg_n[team][tl * KM + i] = best.n[i];
+    }
+    wave_lds_fence();
+    int real = 0;
+    for (int cidx = tl; cidx < T * KM; cidx += T) real += mg_n[team][cidx] >= 0 ? 1 : 0;
+#pragma unroll
+    for (int off = T / 2; off >= 1; off >>= 1) real += __shfl_xor(real, off, T);
+    unsigned int vsum = visits, bsum = boxes;
+#pragma unroll
+    for (int off = T / 2; off >= 1; off >>= 1) {
+        vsum += __shfl_xor(vsum, off, T);
+        bsum += __shfl_xor(bsum, off, T);
+    }
+    if (!valid) return;
+    const int found = min(real, c.k);
+    for (int cidx = tl; cidx < T * KM; cidx += T) {
+        const int node = mg_n[team][cidx];
+        if (node < 0) continue;
+        const float d = mg_d[team][cidx];
+        int rank = 0;
+        for (int j = 0; j < T * KM; ++j) {
+            const int nj = mg_n[team][j];
+            if (nj < 0 || j == cidx) continue;
+            const float dj = mg_d[team][j];
+            if (dj < d || (dj == d && tie_less(c.hot, c.ids, nj, node))) ++rank;
+        }
+        if (rank < c.k) write_slot(a, c, rank, d, node);
+    }
+    for (int j = found + tl; j < a.out_stride; j += T) write_slot(a, c, j, INFINITY, -1);
+    if (tl == 0) {
+        a.out_counts[c.qo] = found;
+        if (a.visits) a.visits[c.qo] = vsum;
+        if (a.boxes) a.boxes[c.qo] = bsum;
+    }
+}
+
+// ---- persistent lanes -----------------------------------------------------
+// A fixed grid (a multiple of 8 workgroups, so workgroup b stays on XCD
+// b % 8) walks the batch; a lane that finishes its query takes the next one
+// of its XCD's contiguous eighth of the batch.  Idle lanes of a wavefront
+// claim their queries with one atomic per wavefront.
+template <int KM>
+__global__ __launch_bounds__(64) void k_knn_persist(KnnArgs a, int* __restrict__ next) {
+    const LdsStack st = lds_stack(a.stack);
+    const int lane = threadIdx.x;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int xcd = blockIdx.x % NUM_XCD;
+    const int per = (a.nq + NUM_XCD - 1) / NUM_XCD;
+    const int lo = min(xcd * per, a.nq), hi = min(lo + per, a.nq);
+    QueryCtx c;
+    BestList<KM> best;
+    bool has = false;
+    int n = -1, sp = 0;
+    float bd = 0.0f;
+    unsigned int visits = 0, boxes = 0;
+    bool drained = false;
+    while (true) {
+        const unsigned long long imask = __ballot(!has);
+        if (imask != 0 && !drained) {
+            const int leader = __ffsll((long long)imask) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(&next[xcd], __popcll(imask));
+            base = __shfl(base, leader);
+            if (base + lo >= hi) drained = true;
+            if (!has) {
+                const int qi = lo + base + __popcll(imask & below);
+                if (qi < hi) {
+                    c = load_query(a, qi);
+                    best.init(c.k);
+                    visits = boxes = 0;
+                    sp = 0;
+                    n = -1;
+                    const uint32_t lk0 = fbits(c.hot[SENTINEL].f[3]);
+                    if ((lk0 & LK_HASL) && !(lk0 & LK_TDL)) {
+                        n = ROOT;
+                        bd = calc_box_dist(c.hot[SENTINEL].f + 4, c.qx, c.qy, c.qz);
+                    }
+                    has = true;
+                }
+            }
+        }
+        if (__ballot(has) == 0) break;
+        if (!has) continue;
+        if ((int)visits > a.max_steps) {
+            atomicOr(a.overflow, 2);
+            n = -1;
+            sp = 0;
+        }
+        if (n >= 0 && bd <= c.thr) {
+            ++visits;
+            int nn, fn;
+            float dn = 0.0f, df = 0.0f;
+            visit_node<KM>(c, best, n, nn, dn, fn, df, boxes);
+            const float top = best.d[0];
+            if (fn >= 0 && df <= top && df <= c.thr) {
+                if (sp < a.stack) {
+                    st.n[sp * 64 + lane] = fn;
+                    st.d[sp * 64 + lane] = dist_key(df);
+                    ++sp;
+                } else {
+                    atomicOr(a.overflow, 1);
+                }
+            }
+            if (nn >= 0 && dn <= top && dn <= c.thr) {
+                n = nn;
+                bd = dn;
+                continue;
+            }
+        }
+        n = -1;
+        while (sp > 0) {
+            --sp;
+            const float d = dist_val(st.d[sp * 64 + lane]);
+            if (d <= best.d[0]) {
+                n = st.n[sp * 64 + lane];
+                bd = d;
+                break;
+            }
+        }
+        if (n >= 0) continue;
+        finish_query<KM>(a, c, best, visits, boxes);
+        has = false;
+    }
 }
 
 template <int KM>
-static void launch_knn_km(const KnnArgs& a, int stack, hipStream_t s) {
-    dim3 grid(blocks_for(a.nq, 64)), block(64);
-    if (stack <= 32) hipLaunchKernelGGL((k_knn<KM, 32>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_knn<KM, 64>), grid, block, 0, s, a);
+static int launch_knn_km(const KnnArgs& a0, int lanes, int stack, hipStream_t s, int* persist_next,
+                         int persist_waves) {
+    KnnArgs a = a0;
+    dim3 block(64);
+    if (lanes >= 2) {  // team deques are ring buffers: round the stack up to a power of two
+        int p = 1;
+        while (p < stack) p <<= 1;
+        stack = p;
+    }
+    a.stack = stack;
+    const size_t lds = lds_stack_bytes(stack);
+    switch (lanes) {
+        case 1:
+            hipLaunchKernelGGL((k_knn<KM>), dim3(xcd_grid(blocks_for(a.nq, 64))), block, lds, s, a);
+            return 0;
+        case 0: {  // persistent lanes; the 8 per-XCD query counters were zeroed by the caller
+            int waves = std::min(blocks_for(a.nq, 64), persist_waves);
+            hipLaunchKernelGGL((k_knn_persist<KM>), dim3(xcd_grid(waves)), block, lds, s, a, persist_next);
+            return 0;
+        }
+        case 2:
+            hipLaunchKernelGGL((k_knn_team<KM, 2>), dim3(xcd_grid(blocks_for(a.nq, 32))), block, lds, s, a);
+            return 0;
+        case 4:
+            hipLaunchKernelGGL((k_knn_team<KM, 4>), dim3(xcd_grid(blocks_for(a.nq, 16))), block, lds, s, a);
+            return 0;
+        case 8:
+            hipLaunchKernelGGL((k_knn_team<KM, 8>), dim3(xcd_grid(blocks_for(a.nq, 8))), block, lds, s, a);
+            return 0;
+        case 16:
+            hipLaunchKernelGGL((k_knn_team<KM, 16>), dim3(xcd_grid(blocks_for(a.nq, 4))), block, lds, s, a);
+            return 0;
+        default:
+            return -1;
+    }
 }
 
-// kmax_bucket unused: the bucket is chosen from k; stack is the LDS stack depth.
-int launch_knn(const KnnArgs& a, int stack, int block, hipStream_t s) {
-    (void)block;
+int launch_knn(const KnnArgs& a, int kmax, int lanes, int stack, hipStream_t s, int* persist_next,
+               int persist_waves) {
     if (a.nq <= 0) return 0;
-    int k = a.k;
-    if (k <= 1) launch_knn_km<1>(a, stack, s);
-    else if (k <= 2) launch_knn_km<2>(a, stack, s);
-    else if (k <= 4) launch_knn_km<4>(a, stack, s);
-    else if (k <= 5) launch_knn_km<5>(a, stack, s);
-    else if (k <= 8) launch_knn_km<8>(a, stack, s);
-    else if (k <= 10) launch_knn_km<10>(a, stack, s);
-    else if (k <= 16) launch_knn_km<16>(a, stack, s);
-    else if (k <= 20) launch_knn_km<20>(a, stack, s);
-    else if (k <= 32) launch_knn_km<32>(a, stack, s);
-    else if (k <= 64) launch_knn_km<64>(a, stack, s);
-    else return -1;
-    return 0;
+    if (lanes == 0) launch_fill_i32(persist_next, 0, NUM_XCD, s);
+#define IKD_KNN(KMV) return launch_knn_km<KMV>(a, lanes, stack, s, persist_next, persist_waves)
+    if (kmax <= 1) IKD_KNN(1);
+    if (kmax <= 2) IKD_KNN(2);
+    if (kmax <= 4) IKD_KNN(4);
+    if (kmax <= 5) IKD_KNN(5);
+    if (kmax <= 8) IKD_KNN(8);
+    if (kmax <= 10) IKD_KNN(10);
+    if (kmax <= 16) IKD_KNN(16);
+    if (kmax <= 20) IKD_KNN(20);
+    if (kmax <= 32) IKD_KNN(32);
+#undef IKD_KNN
+    if (kmax <= 64) return launch_knn_km<64>(a, lanes <= 1 ? lanes : (lanes > 4 ? 4 : lanes), stack, s, persist_next,
+                                             persist_waves);
+    return -1;
+}
+
+// ------------------------------------------------ per-query max_dist thresholds
+// The device form of the host conversion: m = max_dist^2 in double, the float
+// nearest to m, stepped down one ulp when it lies above m.
+__global__ void k_thresholds(const double* __restrict__ md, int n, float* __restrict__ thr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double m = md[i] * md[i];
+    float f;
+    if (m != m) {
+        f = -INFINITY;
+    } else {
+        f = __double2float_rn(m);
+        if ((double)f > m) f = bitsf(fbits(f) - 1u);  // f > m >= 0: one ulp toward zero
+    }
+    thr[i] = f;
+}
+
+void launch_thresholds(const double* max_dist, int n, float* thr, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_thresholds, dim3(blocks_for(n, 256)), dim3(256), 0, s, max_dist, n, thr);
 }
 
 // ------------------------------------------------------------- Morton order

@@ -1,0 +1,271 @@
+// ikd_serve.hip — request loop serving many device-resident maps at once.
+//
+// A request is one sensor scan for one map: Nearest_Search of every scan point
+// against the map (ikd_Tree.cpp:366-397), then Add_Points of the scan
+// (cpp:413-489), then optionally Delete_Point_Boxes (cpp:535-556) for a
+// sliding-window trim.  The server runs `slots` worker threads; each takes
+// the oldest pending request whose map is idle and whose device memory fits
+// the HBM budget, so
+//   - requests for one map run one at a time in submission order (one writer
+//     per tree), which makes every result identical to running the same
+//     requests alone, one after another;
+//   - requests for different maps run concurrently, each on its map's stream,
+//     so their kernels overlap on the device;
+//   - a request is admitted as soon as a slot and memory free up.
+// The server only uses the public C ABI of the tree library and the HIP
+// runtime; it owns per-slot device buffers for scans and kNN results.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/ikd_tree_amd.h"
+
+namespace {
+
+int64_t now_ns() {
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+struct SlotBufs {
+    float* xyz = nullptr;  // scan on the device
+    float* out_xyz = nullptr;
+    float* out_d2 = nullptr;
+    uint32_t* out_ids = nullptr;
+    int32_t* out_cnt = nullptr;
+    long long cap_pts = 0, cap_res = 0;
+};
+
+// device bytes a request needs beyond its map: the scan, the kNN results and
+// the tree's insert scratch (about 160 bytes per inserted point)
+int64_t request_bytes(const ikd_request_t* r) {
+    const int64_t n = r->n, k = std::max(r->k, 0);
+    return n * 12 + n * k * 20 + n * 4 + (r->downsample >= 0 ? n * 160 : 0) + (1 << 20);
+}
+
+}  // namespace
+
+struct ikd_server_s {
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::vector<ikd_tree_t> maps;
+    std::vector<char> busy;
+    std::deque<ikd_request_t*> pending;
+    std::deque<ikd_request_t*> done;
+    std::vector<std::thread> workers;
+    std::vector<SlotBufs> bufs;
+    int64_t budget = 0;
+    int64_t in_flight_bytes = 0;
+    int64_t in_flight = 0;
+    bool stop = false;
+    std::string err;
+
+    int grow(SlotBufs& b, long long n, int k, ikd_request_t* r) {
+        if (n > b.cap_pts) {
+            if (b.xyz) hipFree(b.xyz);
+            if (b.out_cnt) hipFree(b.out_cnt);
+            b.xyz = nullptr;
+            b.out_cnt = nullptr;
+            long long c = std::max(n, b.cap_pts + b.cap_pts / 2);
+            if (hipMalloc(&b.xyz, c * 12) != hipSuccess || hipMalloc(&b.out_cnt, c * 4) != hipSuccess) {
+                b.cap_pts = 0;
+                r->status = IKD_ERR_NOMEM;
+                return IKD_ERR_NOMEM;
+            }
+            b.cap_pts = c;
+        }
+        long long res = n * std::max(k, 1);
+        if (res > b.cap_res) {
+            if (b.out_xyz) hipFree(b.out_xyz);
+            if (b.out_d2) hipFree(b.out_d2);
+            if (b.out_ids) hipFree(b.out_ids);
+            b.out_xyz = b.out_d2 = nullptr;
+            b.out_ids = nullptr;
+            long long c = std::max(res, b.cap_res + b.cap_res / 2);
+            if (hipMalloc(&b.out_xyz, c * 12) != hipSuccess || hipMalloc(&b.out_d2, c * 4) != hipSuccess ||
+                hipMalloc(&b.out_ids, c * 4) != hipSuccess) {
+                b.cap_res = 0;
+                r->status = IKD_ERR_NOMEM;
+                return IKD_ERR_NOMEM;
+            }
+            b.cap_res = c;
+        }
+        return IKD_OK;
+    }
+
+    // the request's work, on its map's stream
+    int run(SlotBufs& b, ikd_request_t* r) {
+        ikd_tree_t t = maps[r->map];
+        void* sv = nullptr;
+        int rc = ikd_stream(t, &sv);
+        if (rc != IKD_OK) return rc;
+        hipStream_t st = (hipStream_t)sv;
+        const long long n = r->n;
+        if (n > 0) {
+            rc = grow(b, n, r->k, r);
+            if (rc != IKD_OK) return rc;
+            if (hipMemcpyAsync(b.xyz, r->xyz, n * 12, hipMemcpyHostToDevice, st) != hipSuccess) return IKD_ERR_HIP;
+        }
+        if (r->k > 0 && n > 0) {
+            rc = ikd_nearest_search_device(t, b.xyz, n, r->k, r->max_dist, b.out_xyz, b.out_d2, b.out_ids,
+                                           b.out_cnt);
+            if (rc != IKD_OK) return rc;
+            const long long res = n * r->k;
+            if (r->out_d2 && hipMemcpyAsync(r->out_d2, b.out_d2, res * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+                return IKD_ERR_HIP;
+            if (r->out_ids && hipMemcpyAsync(r->out_ids, b.out_ids, res * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+                return IKD_ERR_HIP;
+            if (r->out_xyz && hipMemcpyAsync(r->out_xyz, b.out_xyz, res * 12, hipMemcpyDeviceToHost, st) != hipSuccess)
+                return IKD_ERR_HIP;
+            if (r->out_counts && hipMemcpyAsync(r->out_counts, b.out_cnt, n * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+                return IKD_ERR_HIP;
+        }
+        if (r->downsample >= 0 && n > 0) {
+            int32_t added = 0;
+            rc = ikd_add_points_device(t, b.xyz, nullptr, n, r->downsample, &added);
+            if (rc != IKD_OK) return rc;
+            r->added = added;
+        }
+        if (r->n_trim > 0) {
+            int32_t del = 0;
+            rc = ikd_delete_point_boxes(t, r->trim_boxes, r->n_trim, &del);
+            if (rc != IKD_OK) return rc;
+            r->trimmed = del;
+        }
+        return ikd_synchronize(t);
+    }
+
+    void worker(int slot) {
+        SlotBufs& b = bufs[slot];
+        std::unique_lock<std::mutex> lk(mu);
+        while (true) {
+            ikd_request_t* r = nullptr;
+            int64_t need = 0;
+            while (true) {
+                if (stop && pending.empty()) return;
+                for (auto it = pending.begin(); it != pending.end(); ++it) {
+                    ikd_request_t* c = *it;
+                    if (busy[c->map]) continue;
+                    const int64_t nb = request_bytes(c);
+                    // admit when it fits; an idle server always admits its oldest runnable request
+                    if (in_flight_bytes + nb > budget && in_flight > 0) continue;
+                    r = c;
+                    need = nb;
+                    pending.erase(it);
+                    break;
+                }
+                if (r) break;
+                cv_work.wait(lk);
+            }
+            busy[r->map] = 1;
+            in_flight_bytes += need;
+            ++in_flight;
+            lk.unlock();
+            r->t_start_ns = now_ns();
+            int rc = run(b, r);
+            r->t_done_ns = now_ns();
+            if (rc != IKD_OK && r->status == IKD_OK) r->status = rc;
+            lk.lock();
+            busy[r->map] = 0;
+            in_flight_bytes -= need;
+            --in_flight;
+            done.push_back(r);
+            cv_done.notify_all();
+            cv_work.notify_all();
+        }
+    }
+};
+
+extern "C" {
+
+int64_t ikd_now_ns(void) { return now_ns(); }
+
+int ikd_server_create(int slots, int64_t hbm_budget_bytes, ikd_server_t* out) {
+    if (!out || slots < 1 || slots > 64 || hbm_budget_bytes <= 0) return IKD_ERR_ARG;
+    ikd_server_t s = new ikd_server_s();
+    s->budget = hbm_budget_bytes;
+    s->bufs.resize(slots);
+    for (int i = 0; i < slots; ++i) s->workers.emplace_back([s, i] { s->worker(i); });
+    *out = s;
+    return IKD_OK;
+}
+
+int ikd_server_add_map(ikd_server_t s, ikd_tree_t tree, int32_t* out_map) {
+    if (!s || !tree || !out_map) return IKD_ERR_ARG;
+    std::lock_guard<std::mutex> g(s->mu);
+    s->maps.push_back(tree);
+    s->busy.push_back(0);
+    *out_map = (int32_t)(s->maps.size() - 1);
+    return IKD_OK;
+}
+
+int ikd_server_submit(ikd_server_t s, ikd_request_t* r) {
+    if (!s || !r) return IKD_ERR_ARG;
+    std::lock_guard<std::mutex> g(s->mu);
+    if (r->map < 0 || r->map >= (int32_t)s->maps.size() || r->n < 0 || r->k > 64 || (r->n > 0 && !r->xyz) ||
+        (r->n_trim > 0 && !r->trim_boxes) || s->stop) {
+        s->err = "server: bad request";
+        return IKD_ERR_ARG;
+    }
+    r->status = IKD_OK;
+    r->added = r->trimmed = 0;
+    r->t_start_ns = r->t_done_ns = 0;
+    s->pending.push_back(r);
+    s->cv_work.notify_one();
+    return IKD_OK;
+}
+
+int ikd_server_wait(ikd_server_t s, ikd_request_t** out, int32_t max_out, int32_t timeout_ms, int32_t* out_n) {
+    if (!s || !out || !out_n || max_out < 1) return IKD_ERR_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->done.empty() && timeout_ms != 0) {
+        auto pred = [s] { return !s->done.empty() || (s->pending.empty() && s->in_flight == 0); };
+        if (timeout_ms < 0) s->cv_done.wait(lk, pred);
+        else s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), pred);
+    }
+    int n = 0;
+    while (n < max_out && !s->done.empty()) {
+        out[n++] = s->done.front();
+        s->done.pop_front();
+    }
+    *out_n = n;
+    return IKD_OK;
+}
+
+int ikd_server_pending(ikd_server_t s, int64_t* out_pending, int64_t* out_in_flight) {
+    if (!s) return IKD_ERR_ARG;
+    std::lock_guard<std::mutex> g(s->mu);
+    if (out_pending) *out_pending = (int64_t)s->pending.size();
+    if (out_in_flight) *out_in_flight = s->in_flight;
+    return IKD_OK;
+}
+
+int ikd_server_destroy(ikd_server_t s) {
+    if (!s) return IKD_ERR_ARG;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop = true;
+        s->cv_work.notify_all();
+    }
+    for (auto& w : s->workers) w.join();
+    for (auto& b : s->bufs) {
+        if (b.xyz) hipFree(b.xyz);
+        if (b.out_xyz) hipFree(b.out_xyz);
+        if (b.out_d2) hipFree(b.out_d2);
+        if (b.out_ids) hipFree(b.out_ids);
+        if (b.out_cnt) hipFree(b.out_cnt);
+    }
+    delete s;
+    return IKD_OK;
+}
+
+}  // extern "C"

@@ -130,7 +130,14 @@ __device__ void refit_node(Pool pool, Params prm, int u, int* viol, DevHdr* hdr)
         if (del_eval > prm.delete_param || bal_eval > prm.balance_param || bal_eval < 1.0f - prm.balance_param) {
             nm |= M_VIOL;
             int pos = atomicAdd(&hdr->viol_count, 1);
-            viol[pos] = u;
+            viol[2 * pos] = u;
+            viol[2 * pos + 1] = size;
+            // The subtree will be rebuilt from its live points: hand the parent
+            // the post-rebuild counts now, so one bottom-up pass already sees
+            // the cascade the rebuild causes above (the reference re-checks the
+            // parent after each rebuild on its way up, cpp:860-863).
+            pool.cnt[u] = make_int4(size - inv, 0, 0, depth);
+            nm &= ~M_TDS;
         }
     }
     pool.meta[u] = nm;
@@ -158,7 +165,7 @@ __global__ void k_refit_level(Pool pool, Params prm, const int* __restrict__ dir
 __global__ void k_find_top(Pool pool, const int* __restrict__ viol, DevHdr* hdr, int* __restrict__ top) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= hdr->viol_count) return;
-    int v = viol[i];
+    int v = viol[2 * i];
     int u = v;
     for (int step = 0; u != ROOT; ++step) {
         u = pool.par[u];
@@ -170,13 +177,14 @@ __global__ void k_find_top(Pool pool, const int* __restrict__ viol, DevHdr* hdr,
     }
     atomicOr(&pool.meta[v], M_TOP);
     int pos = atomicAdd(&hdr->top_count, 1);
-    top[pos] = v;
+    top[2 * pos] = v;
+    top[2 * pos + 1] = viol[2 * i + 1];  // nodes in the subtree before the rebuild
 }
 
 __global__ void k_clear_viol(Pool pool, const int* __restrict__ viol, DevHdr* hdr) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= hdr->viol_count) return;
-    atomicAnd(&pool.meta[viol[i]], ~M_VIOL);
+    atomicAnd(&pool.meta[viol[2 * i]], ~M_VIOL);
 }
 
 void launch_mark_dirty(Pool pool, const int* seeds, const int* nseeds_dev, int bound, int stop_at_tdel,

@@ -123,7 +123,7 @@ __global__ void k_voxel_fold(Pool pool, const float4* __restrict__ pts, const ui
                              const unsigned long long* __restrict__ keys, const int* __restrict__ vstart, int n,
                              int bound, float ds, int check, float* __restrict__ del_boxes, int* __restrict__ del_flag,
                              float4* __restrict__ ins_pts, int* __restrict__ ins_flag, DevHdr* hdr,
-                             long long max_steps) {
+                             long long max_steps, int* __restrict__ del_out, int del_cap) {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= bound || v >= hdr->group_count) return;
     int j0 = vstart[v], j1 = vstart[v + 1];
@@ -236,6 +236,40 @@ __global__ void k_voxel_fold(Pool pool, const float4* __restrict__ pts, const ui
 #pragma unroll
     for (int a = 0; a < 6; ++a) del_boxes[6 * v + a] = box[a];
     ins_pts[v] = make_float4(cx, cy, cz, bitsf(cid));
+
+    // The voxel's original points S0 are deleted (downsample labels) when a
+    // new point replaces them: Delete_by_range(voxel, downsample) of cpp:446.
+    // Voxel boxes are disjoint here, so each S0 node is written by one lane.
+    if (del_out && have_new && n0 > 0 && split == 0x7fffffff) {
+        int stack[2 * MAX_DEPTH + 4];
+        int sp = 0;
+        stack[sp++] = ROOT;
+        long long steps = 0;
+        while (sp > 0) {
+            if (++steps > max_steps) {
+                atomicOr(&hdr->overflow, 4);
+                break;
+            }
+            int u = stack[--sp];
+            float* f = pool.hot[u].f;
+            uint32_t lk = fbits(f[3]);
+            if (!(lk & LK_PDEL) && box_contains(box, f[0], f[1], f[2])) {
+                atomicOr(&pool.meta[u], M_PDEL | M_PDS);
+                atomicOr(reinterpret_cast<uint32_t*>(&f[3]), LK_PDEL);
+                int pos = atomicAdd(&hdr->rec_count, 1);
+                if (pos < del_cap) del_out[pos] = u;
+                else atomicOr(&hdr->overflow, 4);
+            }
+            int pair = (int)lk_pair(lk);
+#pragma unroll
+            for (int s2 = 1; s2 >= 0; --s2) {
+                if (!(lk & lk_has(s2)) || (lk & lk_td(s2))) continue;
+                if (box_disjoint(box, f + 4 + 6 * s2)) continue;
+                if (sp < 2 * MAX_DEPTH + 4) stack[sp++] = 2 * pair + s2;
+                else atomicOr(&hdr->overflow, 4);
+            }
+        }
+    }
 }
 
 // stream compaction of the flagged voxel boxes / points
@@ -277,10 +311,11 @@ void launch_voxel_starts(const int* flag, const int* scan, int n, int* vstart, D
 }
 void launch_voxel_fold2(Pool pool, const float4* pts, const uint32_t* order, const unsigned long long* keys,
                         const int* vstart, int n, int bound, float ds, int check, float* del_boxes, int* del_flag,
-                        float4* ins_pts, int* ins_flag, DevHdr* hdr, long long max_steps, hipStream_t s) {
+                        float4* ins_pts, int* ins_flag, DevHdr* hdr, long long max_steps, int* del_out, int del_cap,
+                        hipStream_t s) {
     if (bound <= 0) return;
     hipLaunchKernelGGL(k_voxel_fold, dim3(blocks_for(bound, 64)), dim3(64), 0, s, pool, pts, order, keys, vstart, n,
-                       bound, ds, check, del_boxes, del_flag, ins_pts, ins_flag, hdr, max_steps);
+                       bound, ds, check, del_boxes, del_flag, ins_pts, ins_flag, hdr, max_steps, del_out, del_cap);
 }
 void launch_compact_boxes(const int* flag, const int* scan, const float* in, int bound, DevHdr* hdr, float* out,
                           int* out_count, hipStream_t s) {

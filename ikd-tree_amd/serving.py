@@ -1,0 +1,209 @@
+"""Serving many device-resident maps: Python side of the request loop in
+``csrc/ikd_serve.hip`` (``ikd_server_*`` in ``include/ikd_tree_amd.h``).
+
+A request is one scan for one map: kNN of every scan point against the map,
+then Add_Points of the scan, then an optional box trim.  The native server
+runs ``slots`` worker threads that admit requests as slots and HBM budget free
+up; requests for one map run in submission order, requests for different maps
+overlap on the device.
+
+``simulate`` drives the server with Poisson arrivals from many simulated
+robots (each with its own map and LiDAR-like scene) in real time and reports
+p50/p99 of completion minus arrival, next to throughput.
+"""
+from __future__ import annotations
+
+import ctypes
+import importlib
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+_pkg = importlib.import_module("ikd-tree_amd")
+
+
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("map", ctypes.c_int32), ("n", ctypes.c_int32), ("xyz", ctypes.c_void_p),
+        ("k", ctypes.c_int32), ("max_dist", ctypes.c_double), ("downsample", ctypes.c_int32),
+        ("n_trim", ctypes.c_int32), ("trim_boxes", ctypes.c_void_p),
+        ("out_xyz", ctypes.c_void_p), ("out_d2", ctypes.c_void_p), ("out_ids", ctypes.c_void_p),
+        ("out_counts", ctypes.c_void_p), ("t_arrival_ns", ctypes.c_int64), ("tag", ctypes.c_int64),
+        ("t_start_ns", ctypes.c_int64), ("t_done_ns", ctypes.c_int64),
+        ("added", ctypes.c_int32), ("trimmed", ctypes.c_int32), ("status", ctypes.c_int32),
+    ]
+
+
+def _lib():
+    lib = _pkg.load_library()
+    if not getattr(lib, "_serve_sigs", False):
+        vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+        lib.ikd_now_ns.argtypes = []
+        lib.ikd_now_ns.restype = i64
+        for name, args in {
+            "ikd_server_create": [i32, i64, ctypes.POINTER(vp)],
+            "ikd_server_add_map": [vp, vp, ctypes.POINTER(i32)],
+            "ikd_server_submit": [vp, ctypes.POINTER(Request)],
+            "ikd_server_wait": [vp, ctypes.POINTER(ctypes.POINTER(Request)), i32, i32, ctypes.POINTER(i32)],
+            "ikd_server_pending": [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)],
+            "ikd_server_destroy": [vp],
+        }.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = i32
+        lib._serve_sigs = True
+    return lib
+
+
+def now_ns() -> int:
+    return int(_lib().ikd_now_ns())
+
+
+class Ticket:
+    """One submitted request: keeps its buffers alive and exposes the results."""
+
+    def __init__(self, map_index, xyz, k, max_dist, downsample, trim, tag):
+        self.xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        n = self.xyz.shape[0]
+        self.k = int(k)
+        kk = max(self.k, 0)
+        self.out_xyz = np.zeros((n, kk, 3), np.float32)
+        self.out_d2 = np.zeros((n, kk), np.float32)
+        self.out_ids = np.zeros((n, kk), np.uint32)
+        self.out_counts = np.zeros(n, np.int32)
+        self.trim = None if trim is None else np.ascontiguousarray(trim, np.float32).reshape(-1, 6)
+        r = Request()
+        r.map = map_index
+        r.n = n
+        r.xyz = self.xyz.ctypes.data
+        r.k = self.k
+        r.max_dist = float(max_dist)
+        r.downsample = int(downsample)
+        r.n_trim = 0 if self.trim is None else self.trim.shape[0]
+        r.trim_boxes = None if self.trim is None else self.trim.ctypes.data
+        if kk > 0 and n > 0:
+            r.out_xyz = self.out_xyz.ctypes.data
+            r.out_d2 = self.out_d2.ctypes.data
+            r.out_ids = self.out_ids.ctypes.data
+            r.out_counts = self.out_counts.ctypes.data
+        r.tag = int(tag)
+        self.req = r
+
+    @property
+    def latency_ms(self) -> float:
+        return (self.req.t_done_ns - self.req.t_arrival_ns) / 1e6
+
+    @property
+    def status(self) -> int:
+        return int(self.req.status)
+
+
+class Server:
+    def __init__(self, slots: int = 8, hbm_budget_bytes: int = 64 << 30):
+        self._lib = _lib()
+        h = ctypes.c_void_p()
+        rc = self._lib.ikd_server_create(slots, int(hbm_budget_bytes), ctypes.byref(h))
+        if rc != 0:
+            raise _pkg.IkdError(rc, "server create failed")
+        self._h = h
+        self._tickets = {}
+        self._maps = []
+
+    def add_map(self, tree) -> int:
+        m = ctypes.c_int32(0)
+        rc = self._lib.ikd_server_add_map(self._h, tree._h, ctypes.byref(m))
+        if rc != 0:
+            raise _pkg.IkdError(rc, "add_map failed")
+        self._maps.append(tree)
+        return int(m.value)
+
+    def submit(self, map_index: int, xyz, k: int = 5, max_dist: float = 5.0, downsample: int = 1, trim=None,
+               tag: int = 0, arrival_ns: int | None = None) -> Ticket:
+        t = Ticket(map_index, xyz, k, max_dist, downsample, trim, tag)
+        t.req.t_arrival_ns = now_ns() if arrival_ns is None else int(arrival_ns)
+        rc = self._lib.ikd_server_submit(self._h, ctypes.byref(t.req))
+        if rc != 0:
+            raise _pkg.IkdError(rc, "submit rejected")
+        self._tickets[ctypes.addressof(t.req)] = t
+        return t
+
+    def wait(self, timeout_ms: int = -1, max_out: int = 64):
+        arr = (ctypes.POINTER(Request) * max_out)()
+        n = ctypes.c_int32(0)
+        rc = self._lib.ikd_server_wait(self._h, arr, max_out, int(timeout_ms), ctypes.byref(n))
+        if rc != 0:
+            raise _pkg.IkdError(rc, "wait failed")
+        return [self._tickets.pop(ctypes.addressof(arr[i].contents)) for i in range(n.value)]
+
+    def outstanding(self) -> int:
+        return len(self._tickets)
+
+    def drain(self):
+        out = []
+        while self._tickets:
+            out.extend(self.wait(-1))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.ikd_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def simulate(server: Server, map_ids, scan_fn, rate_hz: float = 10.0, duration_s: float = 5.0, k: int = 5,
+             max_dist: float = 5.0, trim_fn=None, trim_every: int = 10, seed: int = 1005):
+    """Real-time Poisson arrivals: robot r's scans arrive at rate_hz (exponential
+    gaps) for duration_s seconds; scan_fn(r, j) gives robot r's j-th scan and
+    trim_fn(r, j) its trim boxes (every trim_every-th request).  Returns
+    latencies (ms) and throughput."""
+    rng = np.random.default_rng(seed)
+    events = []
+    for r, m in enumerate(map_ids):
+        t, j = 0.0, 0
+        while True:
+            t += rng.exponential(1.0 / rate_hz)
+            if t >= duration_s:
+                break
+            events.append((t, r, m, j))
+            j += 1
+    events.sort()
+    keys = [(r, j) for (_, r, _, j) in events]
+    with ThreadPoolExecutor(max_workers=16) as ex:  # scan synthesis releases the GIL
+        scans = dict(zip(keys, ex.map(lambda rj: scan_fn(*rj), keys)))
+    t0 = now_ns() + 50_000_000
+    done = []
+    points = 0
+    for (t, r, m, j) in events:
+        due = t0 + int(t * 1e9)
+        while True:
+            wait_ms = (due - now_ns()) / 1e6
+            if wait_ms <= 0:
+                break
+            got = server.wait(timeout_ms=max(1, int(wait_ms)))
+            done.extend(got)
+            if not got:  # idle server: wait() returned at once
+                time.sleep(min(wait_ms, 1.0) / 1e3)
+        trim = trim_fn(r, j) if (trim_fn is not None and j % trim_every == trim_every - 1) else None
+        s = scans[(r, j)]
+        points += s.shape[0]
+        server.submit(m, s, k, max_dist, 1, trim, tag=r, arrival_ns=due)
+    done.extend(server.drain())
+    t1 = max(d.req.t_done_ns for d in done)
+    lat = np.array([d.latency_ms for d in done])
+    bad = [d for d in done if d.status != 0]
+    return {
+        "requests": len(done),
+        "failed": len(bad),
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "max": float(lat.max()), "mean": float(lat.mean())},
+        "scan_points_per_s": points / ((t1 - t0) / 1e9),
+        "requests_per_s": len(done) / ((t1 - t0) / 1e9),
+        "wall_s": (t1 - t0) / 1e9,
+    }

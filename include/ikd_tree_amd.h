@@ -136,14 +136,105 @@ int ikd_pool_stats(ikd_tree_t t, int64_t* used_nodes, int64_t* capacity_nodes, i
 /* Launch tuning (no counterpart in the reference; results never depend on it):
  *  IKD_TUNE_QUERY_ORDER 1 sorts batches of >= 4096 kNN queries into
  *                       Morton order before the launch, 0 (default) keeps input order;
- *  IKD_TUNE_KNN_STACK   0 (default: from the tree depth), 32 or 64 LDS stack
- *                       entries per lane. */
+ *  IKD_TUNE_KNN_STACK   minimum LDS stack entries per lane, 0..64 (default 0:
+ *                       the tree depth + 2, enough for any query);
+ *  IKD_TUNE_LANES_PER_QUERY -1 (default: from the batch size); 0 persistent
+ *                       lanes (one query per lane at a time, idle lanes fetch
+ *                       the next query); or 1, 2, 4, 8, 16 lanes sharing one
+ *                       query's traversal (64 / lanes queries per wavefront). */
 #define IKD_TUNE_QUERY_ORDER 1
 #define IKD_TUNE_KNN_STACK 2
+#define IKD_TUNE_LANES_PER_QUERY 3
+/*  IKD_TUNE_SMALL_BUILD 1 (default) builds every subtree of <= 1024 points
+ *                       with one workgroup in one launch; 0 always uses the
+ *                       level-synchronous build (identical trees). */
+#define IKD_TUNE_SMALL_BUILD 4
 int ikd_set_tuning(ikd_tree_t t, int key, int value);
+
+/* Pre-order dump of the tree (diagnostic, no counterpart in the reference):
+ * per node x, y, z, id (4 x 4 bytes), axis, TreeSize, invalid_point_num, flags
+ * (bit 0 point_deleted, 1 tree_deleted, 2/3 downsample labels), range[6]
+ * (min xyz, max xyz), left and right row index or -1: 64 bytes per row.
+ * *out_n gets the row count; out_rows may be NULL to query it. */
+int ikd_dump_nodes(ikd_tree_t t, void* out_rows, int64_t capacity, int64_t* out_n);
 
 /* Text of the last error on this tree (never NULL). */
 const char* ikd_last_error(ikd_tree_t t);
+
+/* ---- Multi-tree batches (no counterpart in the reference: the batched form
+ * of Nearest_Search, ikd_Tree.cpp:366-397, over many trees in one launch). ---
+ * A batch owns a stream.  A launch waits for the pending work of every tree
+ * it reads, and every such tree's later work waits for the launch. */
+typedef struct ikd_batch_s* ikd_batch_t;
+int ikd_batch_create(ikd_batch_t* out);
+int ikd_batch_destroy(ikd_batch_t b);
+
+/* Nearest_Search for nq queries spread over trees[0..ntrees): query i searches
+ * trees[d_query_tree[i]] (an index outside the list finds nothing) with
+ * k = d_k[i] clamped to [0, k_max] (d_k NULL: k_max for all) and max_dist =
+ * d_max_dist[i] (d_max_dist NULL: max_dist for all).  Row i of the outputs
+ * has k_max slots: results ascending in [0, count), then empty slots (d2 =
+ * +inf, id = 0xffffffff).  All pointers are device memory; ordered on the
+ * batch stream; results are identical to one ikd_nearest_search_device per
+ * tree.  k_max <= 64. */
+int ikd_batch_nearest_search_device(ikd_batch_t b, const ikd_tree_t* trees, int ntrees, const int32_t* d_query_tree,
+                                    const float* d_queries, int64_t nq, const int32_t* d_k, int k_max,
+                                    const double* d_max_dist, double max_dist, float* d_out_xyz, float* d_out_d2,
+                                    uint32_t* d_out_ids, int32_t* d_out_counts);
+
+/* Wait for the batch stream; IKD_ERR_INTERNAL if a traversal overflowed. */
+int ikd_batch_synchronize(ikd_batch_t b);
+int ikd_batch_stream(ikd_batch_t b, void** out_stream);
+/* IKD_TUNE_LANES_PER_QUERY and IKD_TUNE_KNN_STACK, as for a tree. */
+int ikd_batch_set_tuning(ikd_batch_t b, int key, int value);
+const char* ikd_batch_last_error(ikd_batch_t b);
+
+/* ---- Serving (no counterpart in the reference): a request loop over many
+ * device-resident maps.  A request is one scan for one map: Nearest_Search of
+ * every scan point (ikd_Tree.cpp:366-397), then Add_Points of the scan
+ * (cpp:413-489), then an optional Delete_Point_Boxes trim (cpp:535-556).
+ * `slots` worker threads each take the oldest pending request whose map is
+ * idle and whose device memory fits the HBM budget; requests for one map run
+ * in submission order, one at a time, so each result is identical to running
+ * the same requests alone in that order; requests for different maps overlap
+ * on the device (one stream per map). ---- */
+typedef struct ikd_server_s* ikd_server_t;
+
+typedef struct ikd_request {
+    /* set by the caller; the request and every buffer it names stay owned by
+     * the caller and untouched until ikd_server_wait hands the request back */
+    int32_t map;              /* index returned by ikd_server_add_map */
+    int32_t n;                /* scan points */
+    const float* xyz;         /* host, 3n floats */
+    int32_t k;                /* kNN per scan point before the insert; 0 skips it (k <= 64) */
+    double max_dist;
+    int32_t downsample;       /* Add_Points(scan, downsample); -1 skips the insert */
+    int32_t n_trim;           /* boxes deleted after the insert */
+    const float* trim_boxes;  /* host, 6 n_trim floats */
+    float* out_xyz;           /* host, n*k*3 floats, or NULL */
+    float* out_d2;            /* host, n*k floats, or NULL */
+    uint32_t* out_ids;        /* host, n*k, or NULL */
+    int32_t* out_counts;      /* host, n, or NULL */
+    int64_t t_arrival_ns;     /* caller's ikd_now_ns() at arrival */
+    int64_t tag;              /* caller's */
+    /* set by the server */
+    int64_t t_start_ns, t_done_ns;
+    int32_t added, trimmed, status;
+} ikd_request_t;
+
+/* Monotonic clock of the server (ns). */
+int64_t ikd_now_ns(void);
+int ikd_server_create(int slots, int64_t hbm_budget_bytes, ikd_server_t* out);
+/* Register a map; the tree must outlive the server and get no other calls
+ * while the server runs. */
+int ikd_server_add_map(ikd_server_t s, ikd_tree_t tree, int32_t* out_map);
+int ikd_server_submit(ikd_server_t s, ikd_request_t* req);
+/* Hand back up to max_out finished requests (timeout_ms < 0: wait for one, or
+ * until nothing is pending or running; 0: do not wait). */
+int ikd_server_wait(ikd_server_t s, ikd_request_t** out, int32_t max_out, int32_t timeout_ms, int32_t* out_n);
+int ikd_server_pending(ikd_server_t s, int64_t* out_pending, int64_t* out_in_flight);
+/* Finish every submitted request, then stop the workers. */
+int ikd_server_destroy(ikd_server_t s);
 
 #ifdef __cplusplus
 }

@@ -14,7 +14,7 @@ HEADER = os.path.join(ROOT, "include", "ikd_tree_amd.h")
 
 def _declared():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(ikd_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^(?:int|int64_t|const char\*)\s+(ikd_\w+)\s*\(", text, flags=re.M)))
 
 
 def _pkg():
@@ -60,3 +60,19 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(pkg, "_lib", None)
     with pytest.raises(ImportError):
         pkg.load_library()
+
+
+def _compile_facade(out_path):
+    src = os.path.join(ROOT, "tests", "cpp", "facade_check.cpp")
+    libdir = os.path.join(ROOT, "ikd-tree_amd")
+    cmd = ["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include"), src, "-L", libdir, "-likd_hip",
+           f"-Wl,-rpath,{libdir}", "-o", str(out_path)]
+    import subprocess
+    return subprocess.run(cmd, capture_output=True, text=True)
+
+
+def test_cpp_facade_compiles_and_links(tmp_path):
+    """The header-only KD_TREE<PointType> facade builds with plain g++ against
+    the C ABI, exactly as a reference caller would include ikd_Tree.h."""
+    r = _compile_facade(tmp_path / "facade_check")
+    assert r.returncode == 0, r.stderr

@@ -1,0 +1,147 @@
+"""GPU: many trees' kNN queries in one launch (ikd_batch_nearest_search_device).
+
+Bars: every query's result (counts, squared distances bit for bit, ids) is
+identical to the same query run alone against its own tree through the plain
+entry point, and equal to brute force under the (d2, x, y, z, id) order; uneven
+tree sizes, uneven query counts, mixed k and mixed max_dist in one launch.
+"""
+import importlib
+
+import numpy as np
+import pytest
+
+from common import INF, brute_knn, grid_cloud, uniform_cloud
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def ikd():
+    return importlib.import_module("ikd-tree_amd")
+
+
+def _outs(n, kmax, dev):
+    return (torch.empty((n, kmax, 3), dtype=torch.float32, device=dev),
+            torch.empty((n, kmax), dtype=torch.float32, device=dev),
+            torch.empty((n, kmax), dtype=torch.int32, device=dev),
+            torch.empty((n,), dtype=torch.int32, device=dev))
+
+
+def _mixed_batch(seed, sizes, nqs, ks=(1, 5, 10, 20), dists=(1.0, 5.0, INF), grid_tree=None):
+    rng = np.random.default_rng(seed)
+    clouds = []
+    for i, n in enumerate(sizes):
+        clouds.append(grid_cloud(rng, n) if i == grid_tree else uniform_cloud(rng, n, -30, 30))
+    qt, qs, qk, qd = [], [], [], []
+    for i, m in enumerate(nqs):
+        qs.append(uniform_cloud(rng, m, -32, 32))
+        qt.append(np.full(m, i, np.int32))
+        qk.append(rng.choice(ks, m).astype(np.int32))
+        qd.append(rng.choice(dists, m).astype(np.float64))
+    perm = rng.permutation(sum(nqs))  # interleave the trees' queries
+    cat = lambda xs: np.concatenate(xs)[perm]
+    return clouds, cat(qt), cat(qs), cat(qk), cat(qd)
+
+
+@pytest.mark.parametrize("lanes", [-1, 1, 2, 0])
+def test_batch_matches_single_tree_calls(ikd, lanes):
+    dev = torch.device("cuda:0")
+    sizes = [1, 17, 900, 5000, 40000, 120000]
+    nqs = [30, 200, 1000, 3000, 700, 9000]
+    clouds, qt, q, qk, qd = _mixed_batch(11, sizes, nqs, grid_tree=3)
+    trees = []
+    for c in clouds:
+        t = ikd.KD_TREE(0.5, 0.6, 0.2)
+        t.Build(c)
+        trees.append(t)
+    kmax = 20
+    b = ikd.Batch()
+    b.set_tuning(ikd.IKD_TUNE_LANES_PER_QUERY, lanes)
+    o = _outs(q.shape[0], kmax, dev)
+    b.nearest_search_device(trees, torch.from_numpy(qt).to(dev), torch.from_numpy(q).to(dev), kmax, *o,
+                            k=torch.from_numpy(qk).to(dev), max_dist=torch.from_numpy(qd).to(dev))
+    b.synchronize()
+    bx, bd, bi, bc = (x.cpu().numpy() for x in o)
+    bi = bi.view(np.uint32)
+    for ti, t in enumerate(trees):
+        for k in (1, 5, 10, 20):
+            for md in (1.0, 5.0, INF):
+                sel = np.nonzero((qt == ti) & (qk == k) & (qd == md))[0]
+                if sel.size == 0:
+                    continue
+                sx, sd, si, sc = t.knn(q[sel], k, md)
+                assert np.array_equal(bc[sel], sc), f"counts tree {ti} k {k} md {md}"
+                assert np.array_equal(bd[sel, :k].view(np.uint32), sd.view(np.uint32))
+                assert np.array_equal(bi[sel, :k], si)
+                assert np.array_equal(bx[sel, :k], sx)
+                # slots past the query's k are empty
+                assert np.all(np.isinf(bd[sel, k:])) and np.all(bi[sel, k:] == 0xFFFFFFFF)
+                if ti in (0, 1, 2):  # brute force on the small trees
+                    ids = np.arange(clouds[ti].shape[0], dtype=np.uint32)
+                    rd, ri, rc = brute_knn(clouds[ti], ids, q[sel], k, md, tie="xyz")
+                    assert np.array_equal(sc, rc)
+                    assert np.array_equal(sd.view(np.uint32), rd.view(np.uint32))
+                    assert np.array_equal(si, ri)
+    for t in trees:
+        t.close()
+
+
+def test_batch_uniform_k_and_bad_tree_index(ikd):
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(5)
+    a, c = ikd.KD_TREE(), ikd.KD_TREE()
+    pa, pc = uniform_cloud(rng, 3000), uniform_cloud(rng, 500)
+    a.Build(pa)
+    c.Build(pc)
+    q = uniform_cloud(rng, 400)
+    qt = rng.integers(-1, 3, 400).astype(np.int32)  # -1 and 2 name no tree
+    o = _outs(400, 5, dev)
+    b = ikd.Batch()
+    b.nearest_search_device([a, c], torch.from_numpy(qt).to(dev), torch.from_numpy(q).to(dev), 5, *o,
+                            max_dist=7.5)
+    b.synchronize()
+    bd, bi, bc = o[1].cpu().numpy(), o[2].cpu().numpy().view(np.uint32), o[3].cpu().numpy()
+    bad = (qt < 0) | (qt > 1)
+    assert np.all(bc[bad] == 0) and np.all(np.isinf(bd[bad])) and np.all(bi[bad] == 0xFFFFFFFF)
+    for ti, t in enumerate((a, c)):
+        sel = np.nonzero(qt == ti)[0]
+        _, sd, si, sc = t.knn(q[sel], 5, 7.5)
+        assert np.array_equal(bc[sel], sc)
+        assert np.array_equal(bd[sel].view(np.uint32), sd.view(np.uint32))
+        assert np.array_equal(bi[sel], si)
+
+
+def test_batch_orders_after_tree_updates(ikd):
+    """A launch sees every update queued on the trees before it, without a host sync."""
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(9)
+    t = ikd.KD_TREE(0.5, 0.6, 0.5)
+    t.Build(uniform_cloud(rng, 20000))
+    extra = torch.from_numpy(uniform_cloud(rng, 5000)).to(dev)
+    t.add_points_device(extra, False)
+    q = uniform_cloud(rng, 2000)
+    o = _outs(2000, 5, dev)
+    b = ikd.Batch()
+    b.nearest_search_device([t], torch.zeros(2000, dtype=torch.int32, device=dev), torch.from_numpy(q).to(dev), 5,
+                            *o)
+    b.synchronize()
+    _, sd, si, sc = t.knn(q, 5)
+    assert np.array_equal(o[1].cpu().numpy().view(np.uint32), sd.view(np.uint32))
+    assert np.array_equal(o[2].cpu().numpy().view(np.uint32), si)
+
+
+def test_batch_argument_errors(ikd):
+    dev = torch.device("cuda:0")
+    t = ikd.KD_TREE()
+    t.Build(uniform_cloud(np.random.default_rng(1), 100))
+    b = ikd.Batch()
+    q = torch.zeros((4, 3), dtype=torch.float32, device=dev)
+    qt = torch.zeros(4, dtype=torch.int32, device=dev)
+    with pytest.raises(ikd.IkdError):
+        b.nearest_search_device([t], qt, q, 0, *_outs(4, 1, dev))
+    with pytest.raises(ikd.IkdError):
+        b.nearest_search_device([t], qt, q, 65, *_outs(4, 65, dev))
+    with pytest.raises(ikd.IkdError):
+        b.nearest_search_device([], qt, q, 5, *_outs(4, 5, dev))

@@ -1,0 +1,52 @@
+"""GPU: the C++ drop-in facade (include/ikd_Tree_amd.hpp) driven like the
+reference's demo, checked against the oracle on the same inputs.  The point
+type carries an intensity payload that must round-trip."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from common import assert_same_multiset, in_box, random_boxes, uniform_cloud
+from oracle import OracleTree
+from test_abi import _compile_facade
+
+pytestmark = pytest.mark.gpu
+
+
+def test_cpp_facade_matches_oracle(tmp_path):
+    exe = tmp_path / "facade_check"
+    r = _compile_facade(exe)
+    assert r.returncode == 0, r.stderr
+    rng = np.random.default_rng(555)
+    pts = uniform_cloud(rng, 20000, -20, 20)
+    qs = uniform_cloud(rng, 400, -21, 21)
+    boxes = random_boxes(rng, 4, -20, 20, 3, 10)
+    pts.tofile(tmp_path / "p.f32")
+    qs.tofile(tmp_path / "q.f32")
+    boxes.tofile(tmp_path / "b.f32")
+    out = subprocess.run([str(exe), str(tmp_path / "p.f32"), str(tmp_path / "q.f32"), str(tmp_path / "b.f32"),
+                          str(tmp_path / "o")], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    o = OracleTree(0.5, 0.6, 0.5)
+    o.Build(pts)
+    # per-point Nearest_Search
+    knn = np.fromfile(tmp_path / "o_knn.f32", np.float32).reshape(400, 5, 5)
+    _, od, oid, oc = o.knn(qs, 5)
+    for i in range(400):
+        c = int(oc[i])
+        assert np.array_equal(knn[i, :c, 4], od[i, :c])
+        got = knn[i, :c, 3].astype(np.int64)  # intensity = input index of the returned point
+        assert np.array_equal(np.sort(got), np.sort(oid[i, :c].astype(np.int64)))
+        assert np.array_equal(knn[i, :c, :3], pts[got])
+    box = np.fromfile(tmp_path / "o_box.f32", np.float32).reshape(-1, 4)
+    assert_same_multiset(box[:, :3], pts[in_box(pts, boxes[0])], "facade Box_Search")
+    assert np.array_equal(box[:, :3], pts[box[:, 3].astype(np.int64)])
+    rad = np.fromfile(tmp_path / "o_radius.f32", np.float32).reshape(-1, 4)
+    assert_same_multiset(rad[:, :3], o.Radius_Search(qs[0], 7.5)[0], "facade Radius_Search")
+    n_del = o.Delete_Point_Boxes(boxes)
+    n_add = o.Add_Points(qs, True)
+    line = out.stdout.strip().split()
+    assert int(line[1]) == n_del and int(line[3]) == n_add and int(line[5]) == o.validnum()
+    flat = np.fromfile(tmp_path / "o_flat.f32", np.float32).reshape(-1, 4)
+    assert_same_multiset(flat[:, :3], o.flatten()[0], "facade flatten after updates")

@@ -66,16 +66,66 @@ def test_build_knn_uniform(ikd, k, max_dist):
     assert np.array_equal(g.tree_range(), o.tree_range())
 
 
-def test_build_structure_matches_oracle(ikd):
-    """Tie-free keys: the device build is the reference's tree (same root
-    statistics, same range) and Morton-ordered batches do not change results."""
+def _same_structure(gd, od):
+    assert gd.shape == od.shape, f"node counts {gd.shape[0]} vs {od.shape[0]}"
+    for f in ("id", "axis", "tree_size", "invalid", "left", "right"):
+        assert np.array_equal(gd[f], od[f]), f
+    for f in ("x", "y", "z"):
+        assert np.array_equal(gd[f].view(np.uint32), od[f].view(np.uint32)), f
+    assert np.array_equal(gd["range"].view(np.uint32), od["range"].view(np.uint32)), "range"
+    assert np.array_equal(gd["flags"] & 3, od["flags"] & 3), "flags"
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 100, 1000, 1024, 1025, 5000])
+@pytest.mark.parametrize("small", [1, 0])
+def test_build_structure_matches_oracle(ikd, n, small):
+    """Tie-free keys: the device build (one-workgroup build for <= 1024 points,
+    level-synchronous otherwise) is node for node the reference's BuildTree."""
+    rng = np.random.default_rng(7 + n)
+    pts = uniform_cloud(rng, n)
+    g = ikd.KD_TREE()
+    g.set_tuning(ikd.IKD_TUNE_SMALL_BUILD, small)
+    g.Build(pts)
+    o = OracleTree()
+    o.Build(pts)
+    _same_structure(g.dump_nodes(), o.dump())
+
+
+def test_morton_order_does_not_change_results(ikd):
     rng = np.random.default_rng(7)
     pts = uniform_cloud(rng, 5000)
     g, o = _pair(ikd, pts)
+    g.set_tuning(ikd.IKD_TUNE_QUERY_ORDER, 1)
     q = uniform_cloud(rng, 6000)  # >= 4096 queries takes the Morton-sorted path
     _, d1, i1, c1 = g.knn(q, 5)
     _, d2, i2, c2 = g.knn(q[:100], 5)
     assert np.array_equal(d1[:100], d2) and np.array_equal(i1[:100], i2)
+
+
+@pytest.mark.parametrize("k,max_dist", [(1, INF), (5, 5.0), (10, INF), (20, 2.0)])
+def test_knn_lanes_per_query_invariance(ikd, k, max_dist):
+    """Every lanes-per-query choice (team traversal) returns the same bits."""
+    rng = np.random.default_rng(300 + k)
+    pts = uniform_cloud(rng, 80000)
+    ids = np.arange(pts.shape[0], dtype=np.uint32)
+    q = uniform_cloud(rng, 2500)
+    g, o = _pair(ikd, pts, ids)
+    ref = None
+    for lanes in (1, 0, 2, 4, 8, 16):
+        g.set_tuning(ikd.IKD_TUNE_LANES_PER_QUERY, lanes)
+        out = g.knn(q, k, max_dist)
+        if ref is None:
+            ref = out
+            bd, bi, bc = brute_knn(pts, ids, q, k, max_dist, tie="xyz")
+            assert np.array_equal(out[3], bc)
+            for i in range(q.shape[0]):
+                c = bc[i]
+                assert np.array_equal(out[1][i, :c].view(np.uint32), bd[i, :c].view(np.uint32))
+                assert np.array_equal(out[2][i, :c], bi[i, :c])
+        else:
+            for a, b in zip(out, ref):
+                assert np.array_equal(a.view(np.uint32) if a.dtype == np.float32 else a,
+                                      b.view(np.uint32) if b.dtype == np.float32 else b), f"lanes={lanes}"
 
 
 def test_knn_grid_ties(ikd):

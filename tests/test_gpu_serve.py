@@ -1,0 +1,103 @@
+"""GPU: the request loop over many device-resident maps (ikd_server_*).
+
+Bar: every request's kNN results (counts, squared distances bit for bit, ids,
+points), Add_Points return value and trim count, and every map's final live
+point set, are identical when the requests run mixed and overlapped in the
+server and when the same requests run alone, one after another, through the
+plain entry points.
+"""
+import importlib
+
+import numpy as np
+import pytest
+
+from common import assert_same_multiset
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ikd():
+    return importlib.import_module("ikd-tree_amd")
+
+
+@pytest.fixture(scope="module")
+def serving():
+    return importlib.import_module("ikd-tree_amd.serving")
+
+
+@pytest.fixture(scope="module")
+def scenes():
+    return importlib.import_module("ikd-tree_amd.scenes")
+
+
+def _trim_box(scene, frame, half=60.0):
+    """Six half-open slabs: everything outside a cube of side 2*half around the sensor."""
+    p = scene.pose(frame)
+    big = 1e7
+    lo = p - half
+    hi = p + half
+    return np.array([
+        [-big, -big, -big, lo[0], big, big], [hi[0], -big, -big, big, big, big],
+        [-big, -big, -big, big, lo[1], big], [-big, hi[1], -big, big, big, big],
+        [-big, -big, -big, big, big, lo[2]], [-big, -big, hi[2], big, big, big],
+    ], np.float32)
+
+
+def test_mixed_equals_alone(ikd, serving, scenes):
+    n_maps, per_map = 4, 5
+    sc = [scenes.MappingScene(1005 + m) for m in range(n_maps)]
+    seeds = [np.concatenate([s.scan(f) for f in range(2)]) for s in sc]
+    reqs = []  # submission order, interleaved over the maps
+    for j in range(per_map):
+        for m in range(n_maps):
+            pts = sc[m].scan(2 + j)[: 8000 + 3000 * m]  # uneven sizes
+            trim = _trim_box(sc[m], 2 + j) if j == 3 else None
+            reqs.append((m, pts, trim))
+
+    trees = []
+    for m in range(n_maps):
+        t = ikd.KD_TREE(0.5, 0.6, 0.5)
+        t.Build(seeds[m])
+        trees.append(t)
+    srv = serving.Server(slots=4, hbm_budget_bytes=8 << 30)
+    ids = [srv.add_map(t) for t in trees]
+    tickets = [srv.submit(ids[m], pts, 5, 5.0, 1, trim, tag=i) for i, (m, pts, trim) in enumerate(reqs)]
+    done = srv.drain()
+    srv.close()
+    assert len(done) == len(tickets)
+    assert all(t.status == 0 for t in tickets), [t.status for t in tickets]
+    assert all(t.req.t_done_ns >= t.req.t_start_ns >= t.req.t_arrival_ns for t in tickets)
+
+    alone = []
+    for m in range(n_maps):
+        t = ikd.KD_TREE(0.5, 0.6, 0.5)
+        t.Build(seeds[m])
+        alone.append(t)
+    for i, (m, pts, trim) in enumerate(reqs):
+        x, d, idx, c = alone[m].knn(pts, 5, 5.0)
+        tk = tickets[i]
+        assert np.array_equal(tk.out_counts, c), f"request {i}: counts"
+        assert np.array_equal(tk.out_d2.view(np.uint32), d.view(np.uint32)), f"request {i}: d2"
+        assert np.array_equal(tk.out_ids, idx), f"request {i}: ids"
+        assert np.array_equal(tk.out_xyz, x), f"request {i}: points"
+        added = alone[m].Add_Points(pts, True)
+        assert tk.req.added == added, f"request {i}: Add_Points return"
+        if trim is not None:
+            assert tk.req.trimmed == alone[m].Delete_Point_Boxes(trim), f"request {i}: trim count"
+    for m in range(n_maps):
+        a, _ = trees[m].flatten()
+        b, _ = alone[m].flatten()
+        assert_same_multiset(a, b)
+
+
+def test_bad_requests_rejected(ikd, serving):
+    t = ikd.KD_TREE()
+    t.Build(np.zeros((1, 3), np.float32))
+    srv = serving.Server(slots=1, hbm_budget_bytes=1 << 30)
+    m = srv.add_map(t)
+    with pytest.raises(ikd.IkdError):
+        srv.submit(m + 1, np.zeros((4, 3), np.float32))
+    with pytest.raises(ikd.IkdError):
+        srv.submit(m, np.zeros((4, 3), np.float32), k=65)
+    srv.close()
