@@ -42,7 +42,7 @@ EXPORTS = (
     "ikd_validnum", "ikd_tree_range", "ikd_root_alpha", "ikd_flatten", "ikd_synchronize", "ikd_stream",
     "ikd_pool_stats", "ikd_set_tuning", "ikd_dump_nodes", "ikd_last_error",
     "ikd_batch_create", "ikd_batch_destroy", "ikd_batch_nearest_search_device", "ikd_batch_synchronize",
-    "ikd_batch_stream", "ikd_batch_set_tuning", "ikd_batch_last_error",
+    "ikd_batch_stream", "ikd_batch_set_tuning", "ikd_batch_last_error", "ikd_batch_radius_search_device",
     "ikd_now_ns", "ikd_server_create", "ikd_server_add_map", "ikd_server_submit", "ikd_server_wait",
     "ikd_server_pending", "ikd_server_destroy",
 )
@@ -107,6 +107,7 @@ def load_library():
         "ikd_batch_destroy": [vp],
         "ikd_batch_nearest_search_device": [vp, vp, i32, vp, vp, i64, vp, i32, vp, f64, vp, vp, vp, vp],
         "ikd_batch_synchronize": [vp],
+        "ikd_batch_radius_search_device": [vp, vp, i32, vp, vp, vp, i64, vp, vp, vp, i64, vp],
         "ikd_batch_stream": [vp, ctypes.POINTER(vp)],
         "ikd_batch_set_tuning": [vp, i32, i32],
     }
@@ -388,6 +389,18 @@ class Batch:
             None if k is None else _tensor_ptr(k), int(k_max), _tensor_ptr(max_dist) if per_q else None,
             float("inf") if per_q else float(max_dist), _tensor_ptr(out_xyz), _tensor_ptr(out_d2),
             _tensor_ptr(out_ids), _tensor_ptr(out_counts)))
+
+    def radius_search_device(self, trees, qtree, centers, radii, out_offsets, out_xyz=None, out_ids=None):
+        """Radius_Search over many trees; returns the total result count.  With
+        out_xyz/out_ids (device tensors of at least total rows) it also fills them."""
+        handles = (ctypes.c_void_p * len(trees))(*[t._h.value for t in trees])
+        total = ctypes.c_int64(0)
+        cap = 0 if out_xyz is None else int(out_xyz.shape[0])
+        self._check(self._lib.ikd_batch_radius_search_device(
+            self._h, handles, len(trees), _tensor_ptr(qtree), _tensor_ptr(centers), _tensor_ptr(radii),
+            centers.shape[0], _tensor_ptr(out_offsets), None if out_xyz is None else _tensor_ptr(out_xyz),
+            None if out_ids is None else _tensor_ptr(out_ids), cap, ctypes.byref(total)))
+        return int(total.value)
 
     def synchronize(self):
         self._check(self._lib.ikd_batch_synchronize(self._h))

@@ -247,11 +247,12 @@ long long live_count(ikd_tree_t t) {
     return (long long)t->h_root->cnt.x - t->h_root->cnt.y;
 }
 
-template <class T>
-int prim_tmp(ikd_tree_t t, size_t bytes, void** out) {
+// rocPRIM temporary storage lives in scratch slot 0 of its owner (tree or batch)
+template <class T, class Owner>
+int prim_tmp(Owner* t, size_t bytes, void** out) {
     (void)sizeof(T);
     char* p;
-    CHK(ensure(t, B_TMP, (long long)bytes, &p));
+    CHK(ensure(t, 0, (long long)bytes, &p));
     *out = p;
     return IKD_OK;
 }
@@ -293,7 +294,8 @@ int exscan_i32(ikd_tree_t t, const int* in, int* out, int n) {
     HIPCHK(prim_exscan_i32(tmp, bytes, in, out, n, t->st));
     return IKD_OK;
 }
-int exscan_i64(ikd_tree_t t, const long long* in, long long* out, int n) {
+template <class Owner>
+int exscan_i64(Owner* t, const long long* in, long long* out, int n) {
     if (n <= 0) return IKD_OK;
     size_t bytes = 0;
     HIPCHK(prim_exscan_i64(nullptr, bytes, in, out, n, t->st));
@@ -949,6 +951,7 @@ int knn_dev(ikd_tree_t t, const float* d_q, int nq, int k, double max_dist, floa
 
 int range_dev(ikd_tree_t t, int mode, const float* d_boxes, const float* d_centers, const float* d_radii, int nq,
               int64_t* h_offsets, float* h_xyz, uint32_t* h_ids, int64_t capacity) {
+    if (t->max_depth + 2 > MAX_DEPTH) CHK(compact_if_needed(t, true));
     hipStream_t st = t->st;
     long long *cnt, *off;
     CHK(ensure(t, B_CNT64, nq + 1, &cnt));
@@ -964,6 +967,7 @@ int range_dev(ikd_tree_t t, int mode, const float* d_boxes, const float* d_cente
     a.mode = mode;
     a.counts = cnt;
     a.overflow = &t->d_hdr->overflow;
+    a.stack = std::min(t->max_depth + 2, MAX_DEPTH);
     launch_range(a, 0, st);
     CHK(exscan_i64(t, cnt, off, nq + 1));
     HIPCHK(hipMemcpyAsync(h_offsets, off, (nq + 1) * sizeof(long long), hipMemcpyDeviceToHost, st));
@@ -1146,7 +1150,15 @@ int ikd_nearest_search_counters(ikd_tree_t t, const float* q, int64_t nq, int k,
     CHK(ensure(t, B_IO4, nq, &dc));
     CHK(ensure(t, B_DELBOX, nq, &dv));
     CHK(ensure(t, B_DELFL, nq, &db));
-    CHK(knn_dev(t, dq, (int)nq, k, max_dist, dx, dd, di, dc, dv, db));
+    // the counts are those of the one-lane depth-first walk (the reference's
+    // traversal), whatever launch shape the tree is tuned to
+    const int lanes = t->knn_lanes, order = t->query_order;
+    t->knn_lanes = 1;
+    t->query_order = 0;
+    int rc = knn_dev(t, dq, (int)nq, k, max_dist, dx, dd, di, dc, dv, db);
+    t->knn_lanes = lanes;
+    t->query_order = order;
+    CHK(rc);
     HIPCHK(hipMemcpyAsync(out_visits, dv, nq * sizeof(uint32_t), hipMemcpyDeviceToHost, t->st));
     HIPCHK(hipMemcpyAsync(out_boxes, db, nq * sizeof(uint32_t), hipMemcpyDeviceToHost, t->st));
     return sync_hdr(t);
@@ -1433,7 +1445,43 @@ const char* ikd_last_error(ikd_tree_t t) {
 }
 
 // ------------------------------------------------------------ multi-tree batches
-enum BatchBuf { BB_TREES = 0, BB_THR, BB_NEXT, BB_FLAG };
+enum BatchBuf { BB_TMP = 0, BB_TREES, BB_THR, BB_NEXT, BB_FLAG, BB_CNT, BB_POOLS };
+
+// shared prologue of batch launches: compact over-deep trees, make the batch
+// stream wait for the trees' pending work, report the deepest tree and the
+// largest pool
+static int batch_begin(ikd_batch_t b, const ikd_tree_t* trees, int ntrees, int* depth, long long* steps) {
+    auto* t = b;
+    *depth = 0;
+    *steps = 16;
+    for (int i = 0; i < ntrees; ++i) {
+        ikd_tree_t tr = trees[i];
+        if (!tr) {
+            b->err = "batch: null tree handle";
+            return IKD_ERR_ARG;
+        }
+        if (tr->max_depth + 2 > MAX_DEPTH) {
+            int rc = compact_if_needed(tr, true);
+            if (rc != IKD_OK) {
+                b->err = "batch: " + tr->err;
+                return rc;
+            }
+        }
+        *depth = std::max(*depth, tr->max_depth);
+        *steps = std::max(*steps, tr->pool_used + 16);
+        HIPCHK(hipEventRecord(tr->ev, tr->st));
+        HIPCHK(hipStreamWaitEvent(b->st, tr->ev, 0));
+    }
+    return IKD_OK;
+}
+
+static int batch_end(ikd_batch_t b, const ikd_tree_t* trees, int ntrees) {
+    auto* t = b;
+    HIPCHK(hipEventRecord(b->ev, b->st));
+    for (int i = 0; i < ntrees; ++i) HIPCHK(hipStreamWaitEvent(trees[i]->st, b->ev, 0));
+    return IKD_OK;
+}
+
 
 int ikd_batch_create(ikd_batch_t* out) {
     if (!out) return IKD_ERR_ARG;
@@ -1509,36 +1557,17 @@ int ikd_batch_nearest_search_device(ikd_batch_t b, const ikd_tree_t* trees, int 
         b->err = "batch: null device pointer";
         return IKD_ERR_ARG;
     }
-    std::vector<TreeRef> refs((size_t)ntrees);
-    int depth = 0;
-    long long steps = 16;
-    for (int i = 0; i < ntrees; ++i) {
-        ikd_tree_t t = trees[i];
-        if (!t) {
-            b->err = "batch: null tree handle";
-            return IKD_ERR_ARG;
-        }
-        if (t->max_depth + 2 > MAX_DEPTH) {
-            int rc = compact_if_needed(t, true);
-            if (rc != IKD_OK) {
-                b->err = "batch: " + t->err;
-                return rc;
-            }
-        }
-        refs[i] = TreeRef{t->pool.hot, t->pool.ids};
-        depth = std::max(depth, t->max_depth);
-        steps = std::max(steps, t->pool_used + 16);
-    }
     auto* t = b;  // HIPCHK reports into b->err
+    int depth;
+    long long steps;
+    CHK(batch_begin(b, trees, ntrees, &depth, &steps));
+    std::vector<TreeRef> refs((size_t)ntrees);
+    for (int i = 0; i < ntrees; ++i) refs[i] = TreeRef{trees[i]->pool.hot, trees[i]->pool.ids};
     TreeRef* d_refs;
     int *next, *flag;
     CHK(ensure(b, BB_TREES, ntrees, &d_refs));
     CHK(ensure(b, BB_NEXT, 8, &next));
     CHK(ensure(b, BB_FLAG, 1, &flag));
-    for (int i = 0; i < ntrees; ++i) {
-        HIPCHK(hipEventRecord(trees[i]->ev, trees[i]->st));
-        HIPCHK(hipStreamWaitEvent(b->st, trees[i]->ev, 0));
-    }
     HIPCHK(hipMemcpyAsync(d_refs, refs.data(), ntrees * sizeof(TreeRef), hipMemcpyHostToDevice, b->st));
     KnnArgs a{};
     a.trees = d_refs;
@@ -1569,8 +1598,67 @@ int ikd_batch_nearest_search_device(ikd_batch_t b, const ikd_tree_t* trees, int 
         return IKD_ERR_UNSUPPORTED;
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(b->ev, b->st));
-    for (int i = 0; i < ntrees; ++i) HIPCHK(hipStreamWaitEvent(trees[i]->st, b->ev, 0));
+    return batch_end(b, trees, ntrees);
+}
+
+int ikd_batch_radius_search_device(ikd_batch_t b, const ikd_tree_t* trees, int ntrees, const int32_t* d_query_tree,
+                                   const float* d_centers, const float* d_radii, int64_t nq, int64_t* d_out_offsets,
+                                   float* d_out_xyz, uint32_t* d_out_ids, int64_t capacity, int64_t* out_total) {
+    if (!b) return IKD_ERR_ARG;
+    if (!trees || ntrees <= 0 || nq < 0 || nq >= (1ll << 31) || !out_total) {
+        b->err = "batch: bad tree list, query count or total pointer";
+        return IKD_ERR_ARG;
+    }
+    *out_total = 0;
+    if (nq == 0) return IKD_OK;
+    if (!d_query_tree || !d_centers || !d_radii || !d_out_offsets) {
+        b->err = "batch: null device pointer";
+        return IKD_ERR_ARG;
+    }
+    auto* t = b;
+    int depth;
+    long long steps;
+    CHK(batch_begin(b, trees, ntrees, &depth, &steps));
+    std::vector<Pool> pools((size_t)ntrees);
+    for (int i = 0; i < ntrees; ++i) pools[i] = trees[i]->pool;
+    Pool* d_pools;
+    long long* cnt;
+    int* flag;
+    CHK(ensure(b, BB_POOLS, ntrees, &d_pools));
+    CHK(ensure(b, BB_CNT, nq + 1, &cnt));
+    CHK(ensure(b, BB_FLAG, 1, &flag));
+    HIPCHK(hipMemcpyAsync(d_pools, pools.data(), ntrees * sizeof(Pool), hipMemcpyHostToDevice, b->st));
+    launch_fill_i32(reinterpret_cast<int*>(cnt + nq), 0, 2, b->st);
+    RangeArgs a{};
+    a.pools = d_pools;
+    a.qtree = d_query_tree;
+    a.ntrees = ntrees;
+    a.centers = d_centers;
+    a.radii = d_radii;
+    a.nq = (int)nq;
+    a.mode = 1;
+    a.counts = cnt;
+    a.overflow = flag;
+    a.max_steps = steps;
+    a.stack = std::min(depth + 2, MAX_DEPTH);
+    launch_range(a, 0, b->st);
+    CHK(exscan_i64(b, cnt, reinterpret_cast<long long*>(d_out_offsets), (int)nq + 1));
+    long long total = 0;
+    HIPCHK(hipMemcpyAsync(&total, d_out_offsets + nq, sizeof(long long), hipMemcpyDeviceToHost, b->st));
+    HIPCHK(hipStreamSynchronize(b->st));
+    *out_total = total;
+    if (d_out_xyz && d_out_ids && capacity >= total && total > 0) {
+        a.offsets = reinterpret_cast<const long long*>(d_out_offsets);
+        a.out_xyz = d_out_xyz;
+        a.out_ids = d_out_ids;
+        launch_range(a, 1, b->st);
+        HIPCHK(hipGetLastError());
+    }
+    CHK(batch_end(b, trees, ntrees));
+    if (d_out_xyz && capacity < total) {
+        b->err = "batch: output capacity below the result count";
+        return IKD_ERR_CAPACITY;
+    }
     return IKD_OK;
 }
 

@@ -184,6 +184,10 @@ struct RangeArgs {
     uint32_t* out_ids;
     int* overflow;
     long long max_steps;
+    int stack;            // LDS stack entries per lane (>= tree depth + 2)
+    const Pool* pools;    // multi-tree launch: pool table with qtree[q] per query (else null)
+    const int* qtree;
+    int ntrees;           // a query naming no valid tree finds nothing
 };
 void launch_range(const RangeArgs& a, int fill, hipStream_t s);
 

@@ -676,10 +676,25 @@ __device__ __forceinline__ int classify(const RangeArgs& a, int qi, const float*
     return 2;
 }
 
-__global__ void k_range(RangeArgs a, int fill) {
-    int qi = blockIdx.x * blockDim.x + threadIdx.x;
+// One lane per query, 64 per workgroup; the depth-first stack lives in
+// dynamic LDS as [stack][64] entries (node index, top bit = whole subtree).
+// Each pop pushes at most its two children, so depth + 2 entries suffice.
+__global__ __launch_bounds__(64) void k_range(RangeArgs a, int fill) {
+    extern __shared__ int lds_dyn[];
+    int* stk = lds_dyn;
+    const int lane = threadIdx.x;
+    const int qi = blockIdx.x * 64 + lane;
     if (qi >= a.nq) return;
-    const NodeHot* __restrict__ hot = a.pool.hot;
+    Pool P = a.pool;
+    if (a.pools) {
+        const int ti = a.qtree[qi];
+        if (ti < 0 || ti >= a.ntrees) {
+            if (!fill) a.counts[qi] = 0;
+            return;
+        }
+        P = a.pools[ti];
+    }
+    const NodeHot* __restrict__ hot = P.hot;
     float qx = 0, qy = 0, qz = 0, r = 0;
     if (a.mode == 1) {
         qx = a.centers[3 * qi];
@@ -699,7 +714,7 @@ __global__ void k_range(RangeArgs a, int fill) {
             a.out_xyz[3 * pos] = f[0];
             a.out_xyz[3 * pos + 1] = f[1];
             a.out_xyz[3 * pos + 2] = f[2];
-            a.out_ids[pos] = a.pool.ids[u];
+            a.out_ids[pos] = P.ids[u];
             ++pos;
         } else {
             ++total;
@@ -707,14 +722,14 @@ __global__ void k_range(RangeArgs a, int fill) {
     };
     uint32_t lk0 = fbits(hot[SENTINEL].f[3]);
     if ((lk0 & LK_HASL) && !(lk0 & LK_TDL)) {
-        int stack[2 * MAX_DEPTH + 4];
         int sp = 0;
         int c0 = classify(a, qi, hot[SENTINEL].f + 4, qx, qy, qz, r);
         if (c0 == 1 && !fill) {
-            int4 cc = a.pool.cnt[ROOT];
+            int4 cc = P.cnt[ROOT];
             total += cc.x - cc.y;
         } else if (c0 != 0) {
-            stack[sp++] = ROOT | (c0 == 1 ? (int)0x80000000 : 0);
+            stk[lane] = ROOT | (c0 == 1 ? (int)0x80000000 : 0);
+            sp = 1;
         }
         float r2 = r * r;
         long long steps = 0;
@@ -723,7 +738,7 @@ __global__ void k_range(RangeArgs a, int fill) {
                 atomicOr(a.overflow, 8);
                 break;
             }
-            int e = stack[--sp];
+            int e = stk[--sp * 64 + lane];
             bool whole = e < 0;
             int u = e & 0x7fffffff;
             const float* f = hot[u].f;
@@ -743,11 +758,11 @@ __global__ void k_range(RangeArgs a, int fill) {
                 int cls = whole ? 1 : classify(a, qi, f + 4 + 6 * s, qx, qy, qz, r);
                 if (cls == 0) continue;
                 if (cls == 1 && !fill) {
-                    int4 cc = a.pool.cnt[c];
+                    int4 cc = P.cnt[c];
                     total += cc.x - cc.y;
                     continue;
                 }
-                if (sp < 2 * MAX_DEPTH + 4) stack[sp++] = c | (cls == 1 ? (int)0x80000000 : 0);
+                if (sp < a.stack) stk[sp++ * 64 + lane] = c | (cls == 1 ? (int)0x80000000 : 0);
                 else atomicOr(a.overflow, 8);
             }
         }
@@ -757,7 +772,8 @@ __global__ void k_range(RangeArgs a, int fill) {
 
 void launch_range(const RangeArgs& a, int fill, hipStream_t s) {
     if (a.nq <= 0) return;
-    hipLaunchKernelGGL(k_range, dim3(blocks_for(a.nq, 64)), dim3(64), 0, s, a, fill);
+    hipLaunchKernelGGL(k_range, dim3(blocks_for(a.nq, 64)), dim3(64), (size_t)a.stack * 64 * sizeof(int), s, a,
+                       fill);
 }
 
 }  // namespace ikd

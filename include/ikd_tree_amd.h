@@ -72,9 +72,11 @@ int ikd_nearest_search(ikd_tree_t t, const float* queries, int64_t nq, int k, do
 int ikd_nearest_search_device(ikd_tree_t t, const float* d_queries, int64_t nq, int k, double max_dist,
                               float* d_out_xyz, float* d_out_d2, uint32_t* d_out_ids, int32_t* d_out_counts);
 
-/* Instrumented Nearest_Search: also returns, per query, the number of nodes
- * visited and of child AABBs evaluated (no counterpart in the reference; used
- * to compute algorithmic bytes for the roofline).  Host pointers. */
+/* Instrumented Nearest_Search: returns, per query, the number of nodes visited
+ * and of child AABBs evaluated by the one-lane depth-first walk (nearer child
+ * first, the reference's traversal, ikd_Tree.cpp:868-1013), whatever launch
+ * shape the tree is tuned to (no counterpart in the reference; used to compute
+ * algorithmic bytes for the roofline).  Host pointers. */
 int ikd_nearest_search_counters(ikd_tree_t t, const float* queries, int64_t nq, int k, double max_dist,
                                 uint32_t* out_visits, uint32_t* out_boxes);
 
@@ -181,6 +183,18 @@ int ikd_batch_nearest_search_device(ikd_batch_t b, const ikd_tree_t* trees, int 
                                     const float* d_queries, int64_t nq, const int32_t* d_k, int k_max,
                                     const double* d_max_dist, double max_dist, float* d_out_xyz, float* d_out_d2,
                                     uint32_t* d_out_ids, int32_t* d_out_counts);
+
+/* Radius_Search (ikd_Tree.cpp:406-411) for nq queries spread over trees, in
+ * one count launch and one fill launch: query i searches
+ * trees[d_query_tree[i]] around d_centers[3i..] with radius d_radii[i] (an
+ * index outside the list finds nothing).  d_out_offsets (device, nq+1) gets
+ * the exclusive prefix of the counts and *out_total (host) the total; the
+ * points and ids are written to d_out_xyz / d_out_ids when both are given and
+ * capacity >= total (else IKD_ERR_CAPACITY if they were given).  Result order
+ * within a query is unspecified.  Waits for the count on the host. */
+int ikd_batch_radius_search_device(ikd_batch_t b, const ikd_tree_t* trees, int ntrees, const int32_t* d_query_tree,
+                                   const float* d_centers, const float* d_radii, int64_t nq, int64_t* d_out_offsets,
+                                   float* d_out_xyz, uint32_t* d_out_ids, int64_t capacity, int64_t* out_total);
 
 /* Wait for the batch stream; IKD_ERR_INTERNAL if a traversal overflowed. */
 int ikd_batch_synchronize(ikd_batch_t b);

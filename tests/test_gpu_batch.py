@@ -145,3 +145,43 @@ def test_batch_argument_errors(ikd):
         b.nearest_search_device([t], qt, q, 65, *_outs(4, 65, dev))
     with pytest.raises(ikd.IkdError):
         b.nearest_search_device([], qt, q, 5, *_outs(4, 5, dev))
+
+
+def test_batch_radius_matches_single_tree_calls(ikd):
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(21)
+    sizes = [50, 3000, 60000]
+    trees, clouds = [], []
+    for n in sizes:
+        c = uniform_cloud(rng, n, -20, 20)
+        t = ikd.KD_TREE()
+        t.Build(c)
+        trees.append(t)
+        clouds.append(c)
+    nq = 3000
+    qt = rng.integers(-1, 3, nq).astype(np.int32)
+    cen = uniform_cloud(rng, nq, -22, 22)
+    rad = (0.2 + 2.8 * rng.random(nq)).astype(np.float32)
+    b = ikd.Batch()
+    off = torch.empty(nq + 1, dtype=torch.int64, device=dev)
+    args = (trees, torch.from_numpy(qt).to(dev), torch.from_numpy(cen).to(dev), torch.from_numpy(rad).to(dev), off)
+    total = b.radius_search_device(*args)
+    xyz = torch.empty((max(total, 1), 3), dtype=torch.float32, device=dev)
+    ids = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+    assert b.radius_search_device(*args, xyz, ids) == total
+    b.synchronize()
+    o = off.cpu().numpy()
+    gx, gi = xyz.cpu().numpy(), ids.cpu().numpy().view(np.uint32)
+    bad = (qt < 0) | (qt > 2)
+    assert np.all(o[1:][bad] == o[:-1][bad])
+    for ti, t in enumerate(trees):
+        sel = np.nonzero(qt == ti)[0]
+        so, sx, si = t.radius_search_batch(cen[sel], rad[sel])
+        for j, qi in enumerate(sel):
+            a = np.sort(gi[o[qi]:o[qi + 1]])
+            c = np.sort(si[so[j]:so[j + 1]])
+            assert np.array_equal(a, c), f"tree {ti} query {qi}"
+    # too small an output buffer: IKD_ERR_CAPACITY, counts still valid
+    if total > 1:
+        with pytest.raises(ikd.IkdError):
+            b.radius_search_device(*args, xyz[: total - 1], ids[: total - 1])
