@@ -457,7 +457,8 @@ def serving_phase(ikd, scenes, n_robots, seconds, seed=1005):
             first.append(f)
     scan_len = rng.integers(10_000, 60_001, size=(n_robots, 4096))
     srv = serving.Server(slots=8, hbm_budget_bytes=64 << 30)
-    ids = [srv.add_map(t) for t in trees]
+    # rebuilds of subtrees above 20k points run between a map's requests
+    ids = [srv.add_map(t, defer_rebuild_points=20_000) for t in trees]
 
     def scan_fn(r, j):
         s = scs[r].scan(first[r] + j)
@@ -473,11 +474,12 @@ def serving_phase(ikd, scenes, n_robots, seconds, seed=1005):
 
     res = serving.simulate(srv, ids, scan_fn, rate_hz=10.0, duration_s=seconds, k=5, max_dist=5.0,
                            trim_fn=trim_fn, trim_every=10, seed=seed)
+    res["deferred_rebuilds_run"] = srv.maintenance_rebuilds()
     srv.close()
     for t in trees:
         t.close()
     res.update({"robots": n_robots, "map_points_total": int(sizes.sum()), "slots": 8, "hbm_budget_gb": 64,
-                "rate_hz_per_robot": 10.0})
+                "rate_hz_per_robot": 10.0, "defer_rebuild_points": 20_000})
     return res
 
 

@@ -44,12 +44,14 @@ EXPORTS = (
     "ikd_batch_create", "ikd_batch_destroy", "ikd_batch_nearest_search_device", "ikd_batch_synchronize",
     "ikd_batch_stream", "ikd_batch_set_tuning", "ikd_batch_last_error", "ikd_batch_radius_search_device",
     "ikd_now_ns", "ikd_server_create", "ikd_server_add_map", "ikd_server_submit", "ikd_server_wait",
-    "ikd_server_pending", "ikd_server_destroy",
+    "ikd_server_pending", "ikd_server_destroy", "ikd_server_set_maintenance", "ikd_server_stats",
+    "ikd_rebuild_deferred",
 )
 IKD_TUNE_QUERY_ORDER = 1
 IKD_TUNE_KNN_STACK = 2
 IKD_TUNE_LANES_PER_QUERY = 3
 IKD_TUNE_SMALL_BUILD = 4
+IKD_TUNE_DEFER_REBUILD = 5
 
 
 class IkdError(RuntimeError):
@@ -103,6 +105,7 @@ def load_library():
         "ikd_pool_stats": [vp, vp, vp, vp, vp],
         "ikd_set_tuning": [vp, i32, i32],
         "ikd_dump_nodes": [vp, vp, i64, vp],
+        "ikd_rebuild_deferred": [vp, i64, vp],
         "ikd_batch_create": [ctypes.POINTER(vp)],
         "ikd_batch_destroy": [vp],
         "ikd_batch_nearest_search_device": [vp, vp, i32, vp, vp, i64, vp, i32, vp, f64, vp, vp, vp, vp],
@@ -324,6 +327,12 @@ class KD_TREE:
 
     def set_tuning(self, key: int, value: int):
         self._check(self._lib.ikd_set_tuning(self._h, key, value))
+
+    def rebuild_deferred(self, max_points: int = 0) -> int:
+        """Run the deferred rebuilds still due (see IKD_TUNE_DEFER_REBUILD); returns how many ran."""
+        n = ctypes.c_int32(0)
+        self._check(self._lib.ikd_rebuild_deferred(self._h, int(max_points), ctypes.byref(n)))
+        return int(n.value)
 
     def dump_nodes(self):
         """Pre-order rows (oracle DUMP_DTYPE layout) of the device tree."""

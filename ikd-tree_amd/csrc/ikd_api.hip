@@ -40,6 +40,11 @@ __global__ void k_reset_counters(DevHdr* h, int which) {
     if (which & 16) h->group_count = 0;
 }
 
+__global__ void k_link_defer_flags(Pool pool, const int* __restrict__ defer, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicOr(&pool.meta[defer[i]], M_DEFER);
+}
+
 __global__ void k_link_bits(NodeHot* hot, int node, uint32_t set, uint32_t clear) {
     uint32_t* p = reinterpret_cast<uint32_t*>(&hot[node].f[3]);
     uint32_t v = *p;
@@ -183,7 +188,8 @@ enum Buf {
     B_SID2, B_SIDE, B_SEG0, B_SEG1, B_SEGD0, B_SEGD1, B_SEGAX, B_SEGMID, B_SEGCH, B_SEGFL, B_SEGSC, B_POSFL, B_POSSC,
     B_SEEDS, B_DIRTY, B_VIOL, B_TOP, B_INFO, B_FRONT0, B_FRONT1, B_EKEY0, B_EKEY1, B_RECS, B_GFLAG, B_GSCAN, B_ATTACH,
     B_PFLAG, B_PSCAN, B_ISEG, B_ISEGD, B_IO0, B_IO1, B_IO2, B_IO3, B_IO4, B_VSTART, B_DELBOX, B_DELFL, B_INSPT,
-    B_INSFL, B_CBOX, B_CPTS, B_MISC, B_MORTK, B_MORTV, B_MORTK2, B_MORTV2, B_CNT64, B_OFF64, B_ADDX, B_ADDI, B_PNEXT, B_SPAIRS, B_SPSCAN, B_COUNT
+    B_INSFL, B_CBOX, B_CPTS, B_MISC, B_MORTK, B_MORTV, B_MORTK2, B_MORTV2, B_CNT64, B_OFF64, B_ADDX, B_ADDI, B_PNEXT, B_SPAIRS, B_SPSCAN,
+    B_DEFER, B_DINFO, B_DBEFORE, B_DKEEP, B_COUNT
 };
 static_assert(B_COUNT <= 80, "scratch table");
 
@@ -543,7 +549,10 @@ int refit_finish(ikd_tree_t t) {
     CHK(ensure(t, B_DIRTY, cap, &dirty));
     CHK(ensure(t, B_VIOL, 2 * cap, &viol));
     CHK(ensure(t, B_TOP, 2 * cap, &top));
-    for (int d = t->max_depth; d >= 0; --d) launch_refit_level(t->pool, t->prm, dirty, t->d_hdr, d, viol, t->st);
+    int* defer;
+    CHK(ensure(t, B_DEFER, DEFER_CAP, &defer));
+    for (int d = t->max_depth; d >= 0; --d)
+        launch_refit_level(t->pool, t->prm, dirty, t->d_hdr, d, viol, defer, t->st);
     launch_find_top(t->pool, viol, (int)cap, t->d_hdr, top, t->st);
     launch_clear_viol(t->pool, viol, (int)cap, t->d_hdr, t->st);
     HIPCHK(hipGetLastError());
@@ -553,6 +562,53 @@ int refit(ikd_tree_t t, const int* seeds, int nseeds, int stop_at_tdel) {
     CHK(refit_begin(t));
     CHK(refit_mark(t, seeds, nullptr, nseeds, stop_at_tdel));
     return refit_finish(t);
+}
+
+// Rebuild the listed subtrees (info = node, live points, depth, parent;
+// before = nodes in the subtree) from their live points.
+int rebuild_listed(ikd_tree_t t, const std::vector<int4>& hinfo, const std::vector<int>& hbefore) {
+    hipStream_t st = t->st;
+    const int ntop = (int)hinfo.size();
+    std::vector<int> roots;
+    std::vector<int4> segs;
+    std::vector<int> segd;
+    long long total_nodes = 0, total_live = 0;
+    int maxlen = 0;
+    bool root_rebuilt = false;
+    for (int i = 0; i < ntop; ++i) {
+        int u = hinfo[i].x, live = hinfo[i].y, depth = hinfo[i].z, parent = hinfo[i].w;
+        if (u == ROOT) root_rebuilt = true;
+        if (live <= 0) {
+            // Rebuild of an all-deleted subtree leaves an empty slot (BuildTree of
+            // an empty range, cpp:574).
+            uint32_t clr = lk_has(u & 1) | lk_td(u & 1);
+            hipLaunchKernelGGL(k_link_bits, dim3(1), dim3(1), 0, st, t->pool.hot, parent, 0u, clr);
+            continue;
+        }
+        segs.push_back(make_int4((int)total_live, (int)(total_live + live - 1), u, parent));
+        segd.push_back(depth);
+        roots.push_back(u);
+        total_nodes += hbefore[i];
+        total_live += live;
+        maxlen = std::max(maxlen, live);
+    }
+    if (!roots.empty()) {
+        float4* pts;
+        int* segof;
+        CHK(enumerate_live(t, roots, total_nodes, total_live, &pts, &segof));
+        int4* dseg;
+        int* dsegd;
+        CHK(ensure(t, B_ISEG, (long long)segs.size(), &dseg));
+        CHK(ensure(t, B_ISEGD, (long long)segs.size(), &dsegd));
+        HIPCHK(hipMemcpyAsync(dseg, segs.data(), segs.size() * sizeof(int4), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dsegd, segd.data(), segd.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        if (root_rebuilt) {
+            int zero = 0;
+            HIPCHK(hipMemcpyAsync(&t->d_hdr->max_depth, &zero, sizeof(int), hipMemcpyHostToDevice, st));
+        }
+        CHK(run_build(t, pts, segof, (int)total_live, dseg, dsegd, (int)segs.size(), maxlen));
+    }
+    return IKD_OK;
 }
 
 int rebuild_tops(ikd_tree_t t) {
@@ -573,45 +629,7 @@ int rebuild_tops(ikd_tree_t t) {
         HIPCHK(hipMemcpyAsync(hinfo.data(), info, ntop * sizeof(int4), hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(hbefore.data(), before, ntop * sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        std::vector<int> roots;
-        std::vector<int4> segs;
-        std::vector<int> segd;
-        long long total_nodes = 0, total_live = 0;
-        int maxlen = 0;
-        bool root_rebuilt = false;
-        for (int i = 0; i < ntop; ++i) {
-            int u = hinfo[i].x, live = hinfo[i].y, depth = hinfo[i].z, parent = hinfo[i].w;
-            if (u == ROOT) root_rebuilt = true;
-            if (live <= 0) {
-                // Rebuild of an all-deleted subtree leaves an empty slot (BuildTree of
-                // an empty range, cpp:574).
-                uint32_t clr = lk_has(u & 1) | lk_td(u & 1);
-                hipLaunchKernelGGL(k_link_bits, dim3(1), dim3(1), 0, st, t->pool.hot, parent, 0u, clr);
-                continue;
-            }
-            segs.push_back(make_int4((int)total_live, (int)(total_live + live - 1), u, parent));
-            segd.push_back(depth);
-            roots.push_back(u);
-            total_nodes += hbefore[i];
-            total_live += live;
-            maxlen = std::max(maxlen, live);
-        }
-        if (!roots.empty()) {
-            float4* pts;
-            int* segof;
-            CHK(enumerate_live(t, roots, total_nodes, total_live, &pts, &segof));
-            int4* dseg;
-            int* dsegd;
-            CHK(ensure(t, B_ISEG, (long long)segs.size(), &dseg));
-            CHK(ensure(t, B_ISEGD, (long long)segs.size(), &dsegd));
-            HIPCHK(hipMemcpyAsync(dseg, segs.data(), segs.size() * sizeof(int4), hipMemcpyHostToDevice, st));
-            HIPCHK(hipMemcpyAsync(dsegd, segd.data(), segd.size() * sizeof(int), hipMemcpyHostToDevice, st));
-            if (root_rebuilt) {
-                int zero = 0;
-                HIPCHK(hipMemcpyAsync(&t->d_hdr->max_depth, &zero, sizeof(int), hipMemcpyHostToDevice, st));
-            }
-            CHK(run_build(t, pts, segof, (int)total_live, dseg, dsegd, (int)segs.size(), maxlen));
-        }
+        CHK(rebuild_listed(t, hinfo, hbefore));
         t->rebuilds += ntop;
         // The refit already handed every ancestor the post-rebuild counts, so
         // the rebuilt subtrees leave nothing to re-check.
@@ -1407,9 +1425,76 @@ int ikd_stream(ikd_tree_t t, void** out) {
     return IKD_OK;
 }
 
+int ikd_rebuild_deferred(ikd_tree_t t, int64_t max_points, int32_t* out_rebuilt) {
+    CHECK_T();
+    if (out_rebuilt) *out_rebuilt = 0;
+    CHK(sync_hdr(t));
+    int n = std::min(t->h_hdr->defer_count, DEFER_CAP);
+    if (n <= 0) return IKD_OK;
+    hipStream_t st = t->st;
+    int *defer, *before, *keep;
+    int4* info;
+    CHK(ensure(t, B_DEFER, DEFER_CAP, &defer));
+    CHK(ensure(t, B_DINFO, n, &info));
+    CHK(ensure(t, B_DBEFORE, n, &before));
+    CHK(ensure(t, B_DKEEP, n, &keep));
+    launch_defer_check(t->pool, t->prm, defer, n, info, before, keep, st);
+    std::vector<int> hdef(n), hkeep(n), hbef(n);
+    std::vector<int4> hinf(n);
+    HIPCHK(hipMemcpyAsync(hdef.data(), defer, n * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hkeep.data(), keep, n * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hinf.data(), info, n * sizeof(int4), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hbef.data(), before, n * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    // take due entries in list order while the point budget lasts (at least one)
+    std::vector<int4> take;
+    std::vector<int> take_before, rest;
+    long long budget = max_points > 0 ? max_points : LLONG_MAX, used = 0;
+    for (int i = 0; i < n; ++i) {
+        if (!hkeep[i]) continue;
+        if (!take.empty() && used + hinf[i].y > budget) {
+            rest.push_back(hdef[i]);
+            continue;
+        }
+        take.push_back(hinf[i]);
+        take_before.push_back(hbef[i]);
+        used += hinf[i].y;
+    }
+    // every processed entry leaves the list; entries over the budget stay listed
+    launch_defer_clear(t->pool, defer, n, st);
+    int cnt = (int)rest.size();
+    if (cnt > 0) {
+        HIPCHK(hipMemcpyAsync(defer, rest.data(), cnt * sizeof(int), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_link_defer_flags, dim3(nblk(cnt)), dim3(BS), 0, st, t->pool, defer, cnt);
+    }
+    HIPCHK(hipMemcpyAsync(&t->d_hdr->defer_count, &cnt, sizeof(int), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (take.empty()) return IKD_OK;
+    CHK(rebuild_listed(t, take, take_before));
+    t->rebuilds += (long long)take.size();
+    // the ancestors still hold the pre-rebuild counts: refit the paths above
+    std::vector<int> seeds;
+    for (const int4& e : take) seeds.push_back(e.y > 0 ? e.x : e.w);
+    int* dseeds;
+    CHK(ensure(t, B_DKEEP, (long long)seeds.size(), &dseeds));
+    HIPCHK(hipMemcpyAsync(dseeds, seeds.data(), seeds.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    CHK(sync_hdr(t));
+    CHK(refit_begin(t));
+    CHK(refit_mark(t, dseeds, nullptr, (int)seeds.size(), 0));
+    CHK(refit_finish(t));
+    CHK(rebuild_tops(t));
+    CHK(read_root(t));
+    if (out_rebuilt) *out_rebuilt = (int32_t)take.size();
+    return IKD_OK;
+}
+
 int ikd_set_tuning(ikd_tree_t t, int key, int value) {
     CHECK_T();
     switch (key) {
+        case IKD_TUNE_DEFER_REBUILD:
+            if (value < 0) return IKD_ERR_ARG;
+            t->prm.defer_points = value;
+            return IKD_OK;
         case IKD_TUNE_QUERY_ORDER:
             t->query_order = value ? 1 : 0;
             return IKD_OK;

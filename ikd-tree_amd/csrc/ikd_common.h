@@ -54,6 +54,7 @@ constexpr uint32_t M_DIRTY = 1u << 8;  // on a modified path (refit pending)
 constexpr uint32_t M_VIOL = 1u << 9;   // violates Criterion_Check after refit
 constexpr uint32_t M_TOP = 1u << 10;   // topmost violator: subtree is rebuilt
 constexpr uint32_t M_CLAIM = 1u << 11; // claimed by a Delete_Points lane
+constexpr uint32_t M_DEFER = 1u << 12; // violator whose rebuild was deferred (listed once)
 
 constexpr int SENTINEL = 0;
 constexpr int ROOT = 2;

@@ -27,8 +27,10 @@ struct DevHdr {
     int pair_pending;   // build: pairs taken by the level just written
     int front3[3];      // enumerate: rotating frontier counters
     int added;          // downsample: Add_Points return value of the batch
-    int pad[2];
+    int defer_count;    // deferred rebuilds listed (kept across calls)
+    int pad;
 };
+constexpr int DEFER_CAP = 4096;  // deferred-rebuild list entries
 
 struct Pool {
     NodeHot* hot;
@@ -42,6 +44,7 @@ struct Params {
     float delete_param;
     float balance_param;
     float box_length;
+    int defer_points;  // violators with more live points are listed for a later rebuild (0: never)
 };
 
 // ---------------------------------------------------------------- build
@@ -84,7 +87,10 @@ void launch_small_commit(const int* pairs, const int* pair_scan, int S, DevHdr* 
 // ---------------------------------------------------------------- refit
 void launch_mark_dirty(Pool pool, const int* seeds, const int* nseeds_dev, int bound, int stop_at_tdel,
                        int* dirty, DevHdr* hdr, hipStream_t s);
-void launch_refit_level(Pool pool, Params prm, const int* dirty, DevHdr* hdr, int depth, int* viol,
+void launch_defer_check(Pool pool, Params prm, const int* defer, int n, int4* info, int* before, int* keep,
+                        hipStream_t s);
+void launch_defer_clear(Pool pool, const int* defer, int n, hipStream_t s);
+void launch_refit_level(Pool pool, Params prm, const int* dirty, DevHdr* hdr, int depth, int* viol, int* defer,
                         hipStream_t s);
 void launch_find_top(Pool pool, const int* viol, int bound, DevHdr* hdr, int* top, hipStream_t s);
 void launch_clear_viol(Pool pool, const int* viol, int bound, DevHdr* hdr, hipStream_t s);

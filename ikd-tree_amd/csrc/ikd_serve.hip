@@ -66,6 +66,8 @@ struct ikd_server_s {
     int64_t budget = 0;
     int64_t in_flight_bytes = 0;
     int64_t in_flight = 0;
+    int64_t maintain_points = -1;  // deferred-rebuild budget per idle map (-1: no limit, 0: never)
+    int64_t maintenance_runs = 0;
     bool stop = false;
     std::string err;
 
@@ -175,10 +177,23 @@ struct ikd_server_s {
             r->t_done_ns = now_ns();
             if (rc != IKD_OK && r->status == IKD_OK) r->status = rc;
             lk.lock();
+            // hand the request back before any maintenance of its map
+            done.push_back(r);
+            cv_done.notify_all();
+            bool waiting = false;
+            for (ikd_request_t* c : pending) waiting = waiting || c->map == r->map;
+            if (!waiting && maintain_points != 0) {
+                // the map is idle: run its deferred rebuilds now, off the
+                // requests' latency path (a request arriving meanwhile waits)
+                lk.unlock();
+                int32_t nreb = 0;
+                ikd_rebuild_deferred(maps[r->map], maintain_points, &nreb);
+                lk.lock();
+                maintenance_runs += nreb;
+            }
             busy[r->map] = 0;
             in_flight_bytes -= need;
             --in_flight;
-            done.push_back(r);
             cv_done.notify_all();
             cv_work.notify_all();
         }
@@ -238,6 +253,20 @@ int ikd_server_wait(ikd_server_t s, ikd_request_t** out, int32_t max_out, int32_
         s->done.pop_front();
     }
     *out_n = n;
+    return IKD_OK;
+}
+
+int ikd_server_set_maintenance(ikd_server_t s, int64_t max_points) {
+    if (!s) return IKD_ERR_ARG;
+    std::lock_guard<std::mutex> g(s->mu);
+    s->maintain_points = max_points;
+    return IKD_OK;
+}
+
+int ikd_server_stats(ikd_server_t s, int64_t* out_rebuilds) {
+    if (!s) return IKD_ERR_ARG;
+    std::lock_guard<std::mutex> g(s->mu);
+    if (out_rebuilds) *out_rebuilds = s->maintenance_runs;
     return IKD_OK;
 }
 

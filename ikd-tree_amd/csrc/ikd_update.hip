@@ -76,7 +76,7 @@ __global__ void k_mark_dirty(Pool pool, const int* __restrict__ seeds, const int
 
 // Update (cpp:1183-1323) of one node from its children, then
 // Criterion_Check (cpp:1089-1107).
-__device__ void refit_node(Pool pool, Params prm, int u, int* viol, DevHdr* hdr) {
+__device__ void refit_node(Pool pool, Params prm, int u, int* viol, int* defer, DevHdr* hdr) {
     const NodeHot& h = pool.hot[u];
     uint32_t lk = fbits(h.f[3]);
     uint32_t m = pool.meta[u];
@@ -127,7 +127,22 @@ __device__ void refit_node(Pool pool, Params prm, int u, int* viol, DevHdr* hdr)
         int son = has[0] ? csize[0] : csize[1];
         float del_eval = (float)inv / (float)size;
         float bal_eval = (float)son / (float)(size - 1);
-        if (del_eval > prm.delete_param || bal_eval > prm.balance_param || bal_eval < 1.0f - prm.balance_param) {
+        bool violates = del_eval > prm.delete_param || bal_eval > prm.balance_param ||
+                        bal_eval < 1.0f - prm.balance_param;
+        if (violates && prm.defer_points > 0 && size - inv > prm.defer_points && defer) {
+            // Large subtree: keep its true counts and list it (once) for a
+            // later rebuild, as the reference hands subtrees of >= 1500 nodes
+            // to its background rebuild thread (cpp:627-633).
+            violates = false;
+            if (!(m & M_DEFER)) {
+                int pos = atomicAdd(&hdr->defer_count, 1);
+                if (pos < DEFER_CAP) {
+                    defer[pos] = u;
+                    nm |= M_DEFER;
+                }
+            }
+        }
+        if (violates) {
             nm |= M_VIOL;
             int pos = atomicAdd(&hdr->viol_count, 1);
             viol[2 * pos] = u;
@@ -150,13 +165,70 @@ __device__ void refit_node(Pool pool, Params prm, int u, int* viol, DevHdr* hdr)
 }
 
 __global__ void k_refit_level(Pool pool, Params prm, const int* __restrict__ dirty, DevHdr* hdr, int depth,
-                              int* __restrict__ viol) {
+                              int* __restrict__ viol, int* __restrict__ defer) {
     int n = hdr->dirty_count;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         int u = dirty[i];
         if (pool.cnt[u].w != depth) continue;
-        refit_node(pool, prm, u, viol, hdr);
+        refit_node(pool, prm, u, viol, defer, hdr);
     }
+}
+
+// Deferred rebuilds: an entry is still due if its node is linked into the
+// tree (every parent link on the way to the root points back at it), no
+// subtree on the way is deleted, no ancestor is itself listed (the ancestor's
+// rebuild covers it) and it still fails Criterion_Check on its current counts.
+// keep[i] = 1 for due entries, with info (node, live, depth, parent) and the
+// node count before the rebuild.
+__global__ void k_defer_check(Pool pool, Params prm, const int* __restrict__ defer, int n, int4* __restrict__ info,
+                              int* __restrict__ before, int* __restrict__ keep) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keep[i] = 0;
+    int u = defer[i];
+    uint32_t mu = pool.meta[u];
+    if (!(mu & M_DEFER) || (mu & M_TDEL)) return;
+    int v = u;
+    for (int step = 0; step <= WALK_LIMIT; ++step) {
+        int p = pool.par[v];
+        if (p < 0) return;
+        uint32_t lk = fbits(pool.hot[p].f[3]);
+        int side = v & 1;
+        if (!(lk & lk_has(side)) || (lk & lk_td(side)) || (int)(2 * lk_pair(lk)) + side != v) return;
+        if (p == SENTINEL) break;
+        uint32_t mp = pool.meta[p];
+        if (mp & (M_TDEL | M_DEFER)) return;
+        v = p;
+    }
+    if (v != ROOT) return;
+    int4 c = pool.cnt[u];
+    int size = c.x, inv = c.y;
+    if (size <= MIN_UNBALANCED_SIZE) return;
+    uint32_t lk = fbits(pool.hot[u].f[3]);
+    int pair = (int)lk_pair(lk);
+    int son = (lk & LK_HASL) ? pool.cnt[2 * pair].x : ((lk & LK_HASR) ? pool.cnt[2 * pair + 1].x : 0);
+    float del_eval = (float)inv / (float)size;
+    float bal_eval = (float)son / (float)(size - 1);
+    if (!(del_eval > prm.delete_param || bal_eval > prm.balance_param || bal_eval < 1.0f - prm.balance_param)) return;
+    keep[i] = 1;
+    info[i] = make_int4(u, size - inv, c.w, pool.par[u]);
+    before[i] = size;
+}
+
+__global__ void k_defer_clear(Pool pool, const int* __restrict__ defer, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAnd(&pool.meta[defer[i]], ~M_DEFER);
+}
+
+void launch_defer_check(Pool pool, Params prm, const int* defer, int n, int4* info, int* before, int* keep,
+                        hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_defer_check, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, prm, defer, n, info, before,
+                       keep);
+}
+void launch_defer_clear(Pool pool, const int* defer, int n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_defer_clear, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, defer, n);
 }
 
 // A violator is rebuilt only if no ancestor violates too: the reference
@@ -193,9 +265,9 @@ void launch_mark_dirty(Pool pool, const int* seeds, const int* nseeds_dev, int b
     hipLaunchKernelGGL(k_mark_dirty, dim3(blocks_for(bound, 256)), dim3(256), 0, s, pool, seeds, nseeds_dev, bound,
                        stop_at_tdel, dirty, hdr);
 }
-void launch_refit_level(Pool pool, Params prm, const int* dirty, DevHdr* hdr, int depth, int* viol,
+void launch_refit_level(Pool pool, Params prm, const int* dirty, DevHdr* hdr, int depth, int* viol, int* defer,
                         hipStream_t s) {
-    hipLaunchKernelGGL(k_refit_level, dim3(512), dim3(256), 0, s, pool, prm, dirty, hdr, depth, viol);
+    hipLaunchKernelGGL(k_refit_level, dim3(512), dim3(256), 0, s, pool, prm, dirty, hdr, depth, viol, defer);
 }
 void launch_find_top(Pool pool, const int* viol, int bound, DevHdr* hdr, int* top, hipStream_t s) {
     if (bound <= 0) return;

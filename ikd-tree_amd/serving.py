@@ -48,6 +48,8 @@ def _lib():
             "ikd_server_wait": [vp, ctypes.POINTER(ctypes.POINTER(Request)), i32, i32, ctypes.POINTER(i32)],
             "ikd_server_pending": [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)],
             "ikd_server_destroy": [vp],
+            "ikd_server_set_maintenance": [vp, i64],
+            "ikd_server_stats": [vp, ctypes.POINTER(i64)],
         }.items():
             fn = getattr(lib, name)
             fn.argtypes = args
@@ -110,7 +112,22 @@ class Server:
         self._tickets = {}
         self._maps = []
 
-    def add_map(self, tree) -> int:
+    def set_maintenance(self, max_points: int):
+        """Deferred-rebuild budget the workers spend on an idle map (-1: no limit, 0: never)."""
+        rc = self._lib.ikd_server_set_maintenance(self._h, int(max_points))
+        if rc != 0:
+            raise _pkg.IkdError(rc, "set_maintenance failed")
+
+    def maintenance_rebuilds(self) -> int:
+        v = ctypes.c_int64(0)
+        self._lib.ikd_server_stats(self._h, ctypes.byref(v))
+        return int(v.value)
+
+    def add_map(self, tree, defer_rebuild_points: int | None = None) -> int:
+        """Register a map; with defer_rebuild_points its large rebuilds are
+        deferred to the idle time between its requests."""
+        if defer_rebuild_points is not None:
+            tree.set_tuning(_pkg.IKD_TUNE_DEFER_REBUILD, int(defer_rebuild_points))
         m = ctypes.c_int32(0)
         rc = self._lib.ikd_server_add_map(self._h, tree._h, ctypes.byref(m))
         if rc != 0:

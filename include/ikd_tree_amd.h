@@ -151,7 +151,20 @@ int ikd_pool_stats(ikd_tree_t t, int64_t* used_nodes, int64_t* capacity_nodes, i
  *                       with one workgroup in one launch; 0 always uses the
  *                       level-synchronous build (identical trees). */
 #define IKD_TUNE_SMALL_BUILD 4
+/*  IKD_TUNE_DEFER_REBUILD N > 0: a subtree with more than N live points that
+ *                       fails Criterion_Check is not rebuilt inside the update
+ *                       call but listed for ikd_rebuild_deferred (the reference
+ *                       hands subtrees of >= 1500 nodes to a background thread,
+ *                       ikd_Tree.cpp:627-633); 0 (default) rebuilds at once.
+ *                       Query results and live sets never depend on it. */
+#define IKD_TUNE_DEFER_REBUILD 5
 int ikd_set_tuning(ikd_tree_t t, int key, int value);
+
+/* Run the deferred rebuilds that are still due, in the order they were
+ * listed, until their live points exceed max_points (at least one; <= 0: no
+ * limit); *out_rebuilt gets the number run.  Rebuild_Thread's work
+ * (ikd_Tree.cpp:200-350), run when the caller has time, e.g. between scans. */
+int ikd_rebuild_deferred(ikd_tree_t t, int64_t max_points, int32_t* out_rebuilt);
 
 /* Pre-order dump of the tree (diagnostic, no counterpart in the reference):
  * per node x, y, z, id (4 x 4 bytes), axis, TreeSize, invalid_point_num, flags
@@ -247,6 +260,13 @@ int ikd_server_submit(ikd_server_t s, ikd_request_t* req);
  * until nothing is pending or running; 0: do not wait). */
 int ikd_server_wait(ikd_server_t s, ikd_request_t** out, int32_t max_out, int32_t timeout_ms, int32_t* out_n);
 int ikd_server_pending(ikd_server_t s, int64_t* out_pending, int64_t* out_in_flight);
+/* After a request, when no other request for the same map is pending, the
+ * worker runs ikd_rebuild_deferred(map, max_points) before taking new work
+ * (-1, the default: no limit; 0: never).  Deferral itself is per tree
+ * (IKD_TUNE_DEFER_REBUILD). */
+int ikd_server_set_maintenance(ikd_server_t s, int64_t max_points);
+/* Deferred rebuilds the workers have run so far. */
+int ikd_server_stats(ikd_server_t s, int64_t* out_rebuilds);
 /* Finish every submitted request, then stop the workers. */
 int ikd_server_destroy(ikd_server_t s);
 

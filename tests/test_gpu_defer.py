@@ -1,0 +1,103 @@
+"""GPU: deferred rebuilds (IKD_TUNE_DEFER_REBUILD + ikd_rebuild_deferred).
+
+Large subtrees that fail Criterion_Check are listed instead of rebuilt inside
+the update call, then rebuilt by ikd_rebuild_deferred, the device analogue of
+the reference's background rebuild thread (ikd_Tree.cpp:627-633).  Bar: live
+sets, kNN results, box results and Add_Points / Delete_Point_Boxes returns are
+identical with and without deferral and to the oracle, before and after the
+deferred rebuilds run; the deferred rebuilds actually run and make the tree
+shallower.
+"""
+import importlib
+
+import numpy as np
+import pytest
+
+from common import assert_same_multiset, random_boxes, uniform_cloud
+from oracle import OracleTree
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ikd():
+    return importlib.import_module("ikd-tree_amd")
+
+
+def _same_queries(a, b, rng):
+    q = uniform_cloud(rng, 3000, -60, 60)
+    _, da, ia, ca = a.knn(q, 5, 8.0)
+    _, db, ib, cb = b.knn(q, 5, 8.0)
+    assert np.array_equal(ca, cb)
+    assert np.array_equal(da.view(np.uint32), db.view(np.uint32))
+    assert np.array_equal(ia, ib)
+
+
+def test_deferred_rebuilds_keep_results(ikd):
+    rng = np.random.default_rng(44)
+    base = uniform_cloud(rng, 100_000, -50, 50)
+    plain = ikd.KD_TREE(0.5, 0.6, 0.5)
+    lazy = ikd.KD_TREE(0.5, 0.6, 0.5)
+    lazy.set_tuning(ikd.IKD_TUNE_DEFER_REBUILD, 2000)
+    ora = OracleTree(0.5, 0.6, 0.5)
+    for t in (plain, lazy, ora):
+        t.Build(base)
+    for step in range(8):
+        # clustered inserts on one side unbalance the upper levels
+        c = np.array([30.0 + 2 * step, -40.0 + 5 * step, 10.0], np.float32)
+        pts = (c + rng.normal(0, 3.0, (20_000, 3))).astype(np.float32)
+        ids = np.arange(1_000_000 + step * 20_000, 1_000_000 + (step + 1) * 20_000, dtype=np.uint32)
+        ra = plain.Add_Points(pts, True, ids)
+        rb = lazy.Add_Points(pts, True, ids)
+        rc = ora.Add_Points(pts, True, ids)
+        assert ra == rb == rc, f"Add_Points returns step {step}"
+        boxes = random_boxes(rng, 3, -50, 50, 5, 30)
+        da = plain.Delete_Point_Boxes(boxes)
+        db = lazy.Delete_Point_Boxes(boxes)
+        dc = ora.Delete_Point_Boxes(boxes)
+        assert da == db == dc, f"Delete_Point_Boxes returns step {step}"
+        assert plain.validnum() == lazy.validnum() == ora.validnum()
+        _same_queries(plain, lazy, rng)
+    before = lazy.pool_stats()
+    n = lazy.rebuild_deferred(0)
+    after = lazy.pool_stats()
+    assert n > 0 and after["rebuilds"] == before["rebuilds"] + n
+    assert lazy.rebuild_deferred(0) == 0  # nothing left that is due
+    a, ia = plain.flatten()
+    b, ib = lazy.flatten()
+    o, io = ora.flatten()
+    assert_same_multiset(a, b)
+    assert_same_multiset(a, o)
+    assert np.array_equal(np.sort(ia), np.sort(ib))
+    _same_queries(plain, lazy, rng)
+    boxes = random_boxes(rng, 20, -50, 50, 5, 30)
+    go, _, gi = lazy.box_search_batch(boxes)
+    po, _, pi = plain.box_search_batch(boxes)
+    assert np.array_equal(go, po)
+    for i in range(20):
+        assert np.array_equal(np.sort(gi[go[i]:go[i + 1]]), np.sort(pi[po[i]:po[i + 1]]))
+    assert after["max_depth"] <= before["max_depth"]
+
+
+def test_budgeted_deferred_rebuilds(ikd):
+    rng = np.random.default_rng(45)
+    t = ikd.KD_TREE(0.5, 0.6, 0.5)
+    t.set_tuning(ikd.IKD_TUNE_DEFER_REBUILD, 500)
+    t.Build(uniform_cloud(rng, 50_000, -50, 50))
+    ref = ikd.KD_TREE(0.5, 0.6, 0.5)
+    ref.Build(t.flatten()[0], t.flatten()[1])
+    for step in range(6):
+        pts = (np.array([20.0, 20.0, 0.0], np.float32) + rng.normal(0, 2.0, (10_000, 3))).astype(np.float32)
+        ids = np.arange(100_000 + step * 10_000, 100_000 + (step + 1) * 10_000, dtype=np.uint32)
+        t.Add_Points(pts, False, ids)
+        ref.Add_Points(pts, False, ids)
+    total = 0
+    while True:
+        n = t.rebuild_deferred(1)  # one subtree per call
+        assert n in (0, 1)
+        if n == 0:
+            break
+        total += n
+        _same_queries(t, ref, rng)
+        assert total < 10_000
+    assert total > 0

@@ -44,7 +44,8 @@ def _trim_box(scene, frame, half=60.0):
     ], np.float32)
 
 
-def test_mixed_equals_alone(ikd, serving, scenes):
+@pytest.mark.parametrize("defer", [None, 5000])
+def test_mixed_equals_alone(ikd, serving, scenes, defer):
     n_maps, per_map = 4, 5
     sc = [scenes.MappingScene(1005 + m) for m in range(n_maps)]
     seeds = [np.concatenate([s.scan(f) for f in range(2)]) for s in sc]
@@ -61,7 +62,7 @@ def test_mixed_equals_alone(ikd, serving, scenes):
         t.Build(seeds[m])
         trees.append(t)
     srv = serving.Server(slots=4, hbm_budget_bytes=8 << 30)
-    ids = [srv.add_map(t) for t in trees]
+    ids = [srv.add_map(t, defer_rebuild_points=defer) for t in trees]
     tickets = [srv.submit(ids[m], pts, 5, 5.0, 1, trim, tag=i) for i, (m, pts, trim) in enumerate(reqs)]
     done = srv.drain()
     srv.close()
