@@ -189,7 +189,7 @@ enum Buf {
     B_SEEDS, B_DIRTY, B_VIOL, B_TOP, B_INFO, B_FRONT0, B_FRONT1, B_EKEY0, B_EKEY1, B_RECS, B_GFLAG, B_GSCAN, B_ATTACH,
     B_PFLAG, B_PSCAN, B_ISEG, B_ISEGD, B_IO0, B_IO1, B_IO2, B_IO3, B_IO4, B_VSTART, B_DELBOX, B_DELFL, B_INSPT,
     B_INSFL, B_CBOX, B_CPTS, B_MISC, B_MORTK, B_MORTV, B_MORTK2, B_MORTV2, B_CNT64, B_OFF64, B_ADDX, B_ADDI, B_PNEXT, B_SPAIRS, B_SPSCAN,
-    B_DEFER, B_DINFO, B_DBEFORE, B_DKEEP, B_COUNT
+    B_DEFER, B_DINFO, B_DBEFORE, B_DKEEP, B_PTS3, B_COUNT
 };
 static_assert(B_COUNT <= 80, "scratch table");
 
@@ -384,9 +384,9 @@ int run_build(ikd_tree_t t, const float4* pts, const int* seg_of, int m, const i
         int *pairs, *pscan;
         CHK(ensure(t, B_SPAIRS, S, &pairs));
         CHK(ensure(t, B_SPSCAN, S, &pscan));
-        launch_small_pairs(d_seg, S, pairs, st);
+        launch_small_pairs(d_seg, S, nullptr, pairs, st);
         CHK(exscan_i32(t, pairs, pscan, S));
-        launch_build_small(pts, d_seg, d_seg_depth, pscan, S, t->pool, t->d_hdr, st);
+        launch_build_small(pts, d_seg, d_seg_depth, pscan, S, nullptr, nullptr, t->pool, t->d_hdr, st);
         launch_small_commit(pairs, pscan, S, t->d_hdr, st);
         HIPCHK(hipGetLastError());
         return IKD_OK;
@@ -437,7 +437,14 @@ int run_build(ikd_tree_t t, const float4* pts, const int* seg_of, int m, const i
     HIPCHK(hipStreamSynchronize(st));
 
     int levels = bits_for(maxlen);  // floor(log2(maxlen)) + 1
-    for (int L = 0; L < levels; ++L) {
+    // A segment of x points leaves children of at most x/2: from level Ls on
+    // every segment fits one workgroup, and one launch builds all of them.
+    int Ls = levels;
+    if (t->small_build) {
+        Ls = 0;
+        while ((maxlen >> Ls) > SMALL_MAX) ++Ls;
+    }
+    for (int L = 0; L < std::min(levels, Ls); ++L) {
         int cur = L & 1;
         long long bound = std::min<long long>((long long)S << std::min(L, 30), m);
         BuildLevel BL;
@@ -471,6 +478,22 @@ int run_build(ikd_tree_t t, const float4* pts, const int* seg_of, int m, const i
             CHK(exscan_u64(t, posfl, possc, 3 * m));
             launch_build_scatter(BL, st);
         }
+    }
+    if (Ls < levels) {
+        // level Ls: its segments (count on the device) and its x-ordered list
+        const int cur = Ls & 1;
+        const int bound = (int)std::min<long long>((long long)S << std::min(Ls, 30), m);
+        float4* gpts;
+        int *pairs, *pscan;
+        CHK(ensure(t, B_PTS3, m, &gpts));
+        CHK(ensure(t, B_SPAIRS, bound, &pairs));
+        CHK(ensure(t, B_SPSCAN, bound, &pscan));
+        launch_gather_list(pts, perm[cur][0], m, gpts, st);
+        const int* count_dev = &t->d_hdr->seg_count[cur];
+        launch_small_pairs(seg[cur], bound, count_dev, pairs, st);
+        CHK(exscan_i32(t, pairs, pscan, bound));
+        launch_build_small(gpts, seg[cur], segd[cur], pscan, bound, count_dev, perm[cur][0], t->pool, t->d_hdr, st);
+        launch_small_commit(pairs, pscan, bound, t->d_hdr, st);
     }
     HIPCHK(hipGetLastError());
     return IKD_OK;
@@ -745,14 +768,32 @@ int delete_boxes_dev(ikd_tree_t t, const float* d_boxes, int nb, int ds) {
     hipStream_t st = t->st;
     unsigned int* recs;
     long long cap = std::max<long long>(t->pool_used, 16) + 4ll * nb;
+    long long fcap = std::max<long long>(std::max<long long>(t->pool_used, 1 << 16), nb);
+    int2 *f0, *f1;
     CHK(ensure(t, B_RECS, cap, &recs));
+    CHK(ensure(t, B_FRONT0, fcap, &f0));
+    CHK(ensure(t, B_FRONT1, fcap, &f1));
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, t->d_hdr, 2);
-    launch_box_delete_traverse(t->pool, d_boxes, nb, recs, t->d_hdr, (int)cap, t->pool_used + 16, st);
-    CHK(sync_hdr(t));
+    launch_fill_i32(t->d_hdr->front3, 0, 3, st);
+    launch_fill_i32(&t->d_hdr->front_over, 0, 1, st);
+    // levels until the frontier is empty (the host's depth may lag by a call)
+    int L0 = 0, L1 = t->max_depth + 2;
+    while (true) {
+        launch_box_delete_bfs(t->pool, d_boxes, nb, f0, f1, (int)fcap, L0, L1, recs, t->d_hdr, (int)cap, st);
+        CHK(sync_hdr(t));
+        if (t->h_hdr->front_over || t->h_hdr->front3[L1 % 3] == 0) break;
+        if (L1 > 4 * MAX_DEPTH) {
+            t->err = "box delete: frontier did not drain";
+            return IKD_ERR_INTERNAL;
+        }
+        L0 = L1;
+        L1 += 8;
+    }
     int nrec = t->h_hdr->rec_count;
-    if (nrec > cap) {
-        // overlapping boxes recorded more than one entry per node: the union
-        // is order-free, so apply the boxes in halves
+    if (nrec > cap || t->h_hdr->front_over) {
+        // overlapping boxes recorded more than one entry per node, or their
+        // frontiers together outgrew the buffer: the union is order-free, so
+        // apply the boxes in halves
         if (nb == 1) {
             t->err = "box delete: record buffer overflow";
             return IKD_ERR_INTERNAL;
@@ -809,7 +850,8 @@ int add_points_ds_chunk(ikd_tree_t t, const float* d_xyz, const uint32_t* d_ids,
         const bool deleting = apply && pass == passes - 1;
         hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, t->d_hdr, 2 | 8);
         launch_voxel_fold2(t->pool, pts, vb, kb, vstart, n, n, t->prm.box_length, check, delbox, delfl, inspt, insfl,
-                           t->d_hdr, t->pool_used + 16, deleting ? dseeds : nullptr, (int)dcap, st);
+                           t->d_hdr, t->pool_used + 16, deleting ? dseeds : nullptr, (int)dcap,
+                           std::min(t->max_depth + 2, MAX_DEPTH), st);
         int hbad = 0;
         HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
         CHK(sync_hdr(t));

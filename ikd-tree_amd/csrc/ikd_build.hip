@@ -197,6 +197,18 @@ __global__ void k_build_scatter(BuildLevel L) {
 
 static inline int blocks_for(long long n, int bs) { return (int)((n + bs - 1) / bs); }
 
+// out[j] = pts[perm[j]]: a level's points in list order, so every segment's
+// points are contiguous for the one-workgroup build of the levels below
+__global__ void k_gather_list(const float4* __restrict__ pts, const uint32_t* __restrict__ perm, int m,
+                              float4* __restrict__ out) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < m) out[j] = pts[perm[j]];
+}
+void launch_gather_list(const float4* pts, const uint32_t* perm, int m, float4* out, hipStream_t s) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_gather_list, dim3(blocks_for(m, 256)), dim3(256), 0, s, pts, perm, m, out);
+}
+
 void launch_build_keys(const float4* pts, const int* seg_of, int m, int axis, unsigned long long* keys,
                        uint32_t* vals, hipStream_t s) {
     if (m <= 0) return;

@@ -16,8 +16,9 @@
 
 namespace ikd {
 
-constexpr int SB = 256;  // threads per workgroup
-constexpr int SMAX_SEG = SMALL_MAX / 2 + 1;
+// Two shapes: 64 threads for segments of at most 128 points (most insert
+// groups hold a handful), 256 threads for the rest up to SMALL_MAX.
+constexpr int TINY_MAX = 128;
 
 // Nodes with at least one child in the median-split tree of m points.  A node
 // of size x >= 2 leaves children of sizes (x-1)>>1 and x-1-((x-1)>>1), so a
@@ -49,10 +50,14 @@ __device__ int median_tree_pairs(int m) {
     return (int)internal;
 }
 
-__global__ void k_small_pairs(const int4* __restrict__ seg, int S, int* __restrict__ pairs) {
+// count_dev (optional) holds the number of live segments among the S slots;
+// slots past it take no pairs.
+__global__ void k_small_pairs(const int4* __restrict__ seg, int S, const int* __restrict__ count_dev,
+                              int* __restrict__ pairs) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= S) return;
-    pairs[g] = median_tree_pairs(seg[g].y - seg[g].x + 1);
+    const int count = count_dev ? *count_dev : S;
+    pairs[g] = g < count ? median_tree_pairs(seg[g].y - seg[g].x + 1) : 0;
 }
 
 __global__ void k_small_commit(const int* __restrict__ pairs, const int* __restrict__ scan, int S, DevHdr* hdr) {
@@ -71,6 +76,7 @@ __device__ __forceinline__ void write_child_box_s(NodeHot* hot, int parent, int 
 }
 
 // Exclusive scan of one int per thread across the workgroup; *total gets the sum.
+template <int SB>
 __device__ int block_exscan(int v, int* tmp, int* total) {
     const int t = threadIdx.x;
     tmp[t] = v;
@@ -87,26 +93,58 @@ __device__ int block_exscan(int v, int* tmp, int* total) {
     return ex;
 }
 
+// Ascending bitonic sort of skey[0, P) (P a power of two) by the workgroup.
+template <int SB>
+__device__ void block_bitonic(unsigned long long* skey, int P) {
+    const int t = threadIdx.x;
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = t; i < P; i += SB) {
+                int ixj = i ^ j;
+                if (ixj > i) {
+                    unsigned long long x = skey[i], y = skey[ixj];
+                    bool up = (i & k) == 0;
+                    if ((x > y) == up) {
+                        skey[i] = y;
+                        skey[ixj] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// orig (optional) gives each point's index in the caller's original input:
+// equal coordinates are then ordered by it, as the global build's stable
+// presorted lists order them; without it, by position in the segment.
+template <int SB, int SMAX>
 __global__ __launch_bounds__(SB) void k_build_small(const float4* __restrict__ pts, const int4* __restrict__ seg,
                                                      const int* __restrict__ seg_depth,
-                                                     const int* __restrict__ pair_scan, Pool pool, DevHdr* hdr) {
-    __shared__ float px[SMALL_MAX], py[SMALL_MAX], pz[SMALL_MAX];
-    __shared__ unsigned long long skey[SMALL_MAX];  // sort keys, then running counts
-    __shared__ unsigned short lst[2][3][SMALL_MAX];
-    __shared__ short sid[2][SMALL_MAX];
-    __shared__ unsigned char side[SMALL_MAX];
+                                                     const int* __restrict__ pair_scan,
+                                                     const int* __restrict__ count_dev,
+                                                     const uint32_t* __restrict__ orig, Pool pool, DevHdr* hdr) {
+    constexpr int SMAX_SEG = SMAX / 2 + 1;
+    __shared__ float px[SMAX], py[SMAX], pz[SMAX];
+    __shared__ unsigned long long skey[SMAX];  // sort keys, then running counts
+    __shared__ unsigned short lst[2][3][SMAX];
+    __shared__ short sid[2][SMAX];
+    __shared__ unsigned char side[SMAX];
     __shared__ short seg_l[2][SMAX_SEG], seg_r[2][SMAX_SEG];
     __shared__ int seg_node[2][SMAX_SEG], seg_par[2][SMAX_SEG];
     __shared__ unsigned char seg_axis[SMAX_SEG];
     __shared__ short seg_mid[SMAX_SEG], seg_cl[SMAX_SEG], seg_cr[SMAX_SEG];
     __shared__ int scan_tmp[SB];
+    __shared__ unsigned short rnk[SMAX];
 
     const int g = blockIdx.x;
     const int t = threadIdx.x;
+    if (count_dev && g >= *count_dev) return;
     const int4 sg = seg[g];
     const int l0 = sg.x;
     const int m = sg.y - sg.x + 1;
-    if (m <= 0 || m > SMALL_MAX) return;  // the host routes larger segments elsewhere
+    // each shape takes its own size class; the host guarantees m <= SMALL_MAX
+    if (m <= 0 || m > SMAX || (SMAX > TINY_MAX && m <= TINY_MAX)) return;
     const int depth0 = seg_depth[g];
     int next_pair = hdr->pair_used + pair_scan[g];
 
@@ -118,29 +156,25 @@ __global__ __launch_bounds__(SB) void k_build_small(const float4* __restrict__ p
     }
     int P = 1;
     while (P < m) P <<= 1;
+    // tie-break rank of every point: its order by original index, or its position
+    if (orig) {
+        for (int i = t; i < P; i += SB)
+            skey[i] = i < m ? (((unsigned long long)orig[l0 + i] << 16) | (unsigned)i) : ~0ull;
+        __syncthreads();
+        block_bitonic<SB>(skey, P);
+        for (int i = t; i < m; i += SB) rnk[skey[i] & 0xffffu] = (unsigned short)i;
+    } else {
+        for (int i = t; i < m; i += SB) rnk[i] = (unsigned short)i;
+    }
     __syncthreads();
-    // three stable sorted index lists: key (coordinate, input position)
+    // three stable sorted index lists: key (coordinate, rank, position)
     for (int a = 0; a < 3; ++a) {
         const float* c = a == 0 ? px : (a == 1 ? py : pz);
         for (int i = t; i < P; i += SB)
-            skey[i] = i < m ? (((unsigned long long)float_key(c[i]) << 32) | (unsigned)i) : ~0ull;
+            skey[i] = i < m ? (((unsigned long long)float_key(c[i]) << 32) | ((unsigned)rnk[i] << 16) | (unsigned)i)
+                            : ~0ull;
         __syncthreads();
-        for (int k = 2; k <= P; k <<= 1) {
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                for (int i = t; i < P; i += SB) {
-                    int ixj = i ^ j;
-                    if (ixj > i) {
-                        unsigned long long x = skey[i], y = skey[ixj];
-                        bool up = (i & k) == 0;
-                        if ((x > y) == up) {
-                            skey[i] = y;
-                            skey[ixj] = x;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-        }
+        block_bitonic<SB>(skey, P);
         for (int i = t; i < m; i += SB) lst[0][a][i] = (unsigned short)(skey[i] & 0xffffu);
         __syncthreads();
     }
@@ -179,8 +213,8 @@ __global__ __launch_bounds__(SB) void k_build_small(const float4* __restrict__ p
             my_children += hl + hr;
         }
         int tot_pairs, tot_children;
-        int pair_off = block_exscan(my_pairs, scan_tmp, &tot_pairs);
-        int child_off = block_exscan(my_children, scan_tmp, &tot_children);
+        int pair_off = block_exscan<SB>(my_pairs, scan_tmp, &tot_pairs);
+        int child_off = block_exscan<SB>(my_children, scan_tmp, &tot_children);
         for (int q = 0; q < per; ++q) {
             const int s = t * per + q;
             if (s >= ns) break;
@@ -242,8 +276,8 @@ __global__ __launch_bounds__(SB) void k_build_small(const float4* __restrict__ p
                 cr += sd == 1;
             }
             int tl_, tr_;
-            int a0 = block_exscan(cl, scan_tmp, &tl_);
-            int a1 = block_exscan(cr, scan_tmp, &tr_);
+            int a0 = block_exscan<SB>(cl, scan_tmp, &tl_);
+            int a1 = block_exscan<SB>(cr, scan_tmp, &tr_);
             for (int q = 0; q < chunk; ++q) {
                 const int j = t * chunk + q;
                 if (j >= m) break;
@@ -284,14 +318,17 @@ __global__ __launch_bounds__(SB) void k_build_small(const float4* __restrict__ p
 
 static inline int blocks_for(long long n, int bs) { return (int)((n + bs - 1) / bs); }
 
-void launch_small_pairs(const int4* seg, int S, int* pairs, hipStream_t s) {
+void launch_small_pairs(const int4* seg, int S, const int* count_dev, int* pairs, hipStream_t s) {
     if (S <= 0) return;
-    hipLaunchKernelGGL(k_small_pairs, dim3(blocks_for(S, 256)), dim3(256), 0, s, seg, S, pairs);
+    hipLaunchKernelGGL(k_small_pairs, dim3(blocks_for(S, 256)), dim3(256), 0, s, seg, S, count_dev, pairs);
 }
-void launch_build_small(const float4* pts, const int4* seg, const int* seg_depth, const int* pair_scan, int S, Pool pool,
-                        DevHdr* hdr, hipStream_t s) {
+void launch_build_small(const float4* pts, const int4* seg, const int* seg_depth, const int* pair_scan, int S,
+                        const int* count_dev, const uint32_t* orig, Pool pool, DevHdr* hdr, hipStream_t s) {
     if (S <= 0) return;
-    hipLaunchKernelGGL(k_build_small, dim3(S), dim3(SB), 0, s, pts, seg, seg_depth, pair_scan, pool, hdr);
+    hipLaunchKernelGGL((k_build_small<64, TINY_MAX>), dim3(S), dim3(64), 0, s, pts, seg, seg_depth, pair_scan,
+                       count_dev, orig, pool, hdr);
+    hipLaunchKernelGGL((k_build_small<256, SMALL_MAX>), dim3(S), dim3(256), 0, s, pts, seg, seg_depth, pair_scan,
+                       count_dev, orig, pool, hdr);
 }
 void launch_small_commit(const int* pairs, const int* pair_scan, int S, DevHdr* hdr, hipStream_t s) {
     if (S <= 0) return;

@@ -28,7 +28,7 @@ struct DevHdr {
     int front3[3];      // enumerate: rotating frontier counters
     int added;          // downsample: Add_Points return value of the batch
     int defer_count;    // deferred rebuilds listed (kept across calls)
-    int pad;
+    int front_over;     // a box-deletion frontier outgrew its buffer
 };
 constexpr int DEFER_CAP = 4096;  // deferred-rebuild list entries
 
@@ -79,9 +79,10 @@ void launch_build_side(const BuildLevel& L, DevHdr* hdr, hipStream_t s);
 void launch_build_flags(const BuildLevel& L, hipStream_t s);
 void launch_build_scatter(const BuildLevel& L, hipStream_t s);
 
-void launch_small_pairs(const int4* seg, int S, int* pairs, hipStream_t s);
-void launch_build_small(const float4* pts, const int4* seg, const int* seg_depth, const int* pair_scan, int S, Pool pool,
-                        DevHdr* hdr, hipStream_t s);
+void launch_small_pairs(const int4* seg, int S, const int* count_dev, int* pairs, hipStream_t s);
+void launch_build_small(const float4* pts, const int4* seg, const int* seg_depth, const int* pair_scan, int S,
+                        const int* count_dev, const uint32_t* orig, Pool pool, DevHdr* hdr, hipStream_t s);
+void launch_gather_list(const float4* pts, const uint32_t* perm, int m, float4* out, hipStream_t s);
 void launch_small_commit(const int* pairs, const int* pair_scan, int S, DevHdr* hdr, hipStream_t s);
 
 // ---------------------------------------------------------------- refit
@@ -112,8 +113,9 @@ void launch_insert_pairs(Pool pool, const int* attach, const int* pair_flag, con
 void launch_insert_link(Pool pool, const int* attach, const int4* seg, DevHdr* hdr, int bound, hipStream_t s);
 
 // ---------------------------------------------------------------- deletes
-void launch_box_delete_traverse(Pool pool, const float* boxes, int nb, unsigned int* recs, DevHdr* hdr, int cap,
-                                long long max_steps, hipStream_t s);
+// levels [L0, L1) of the breadth-first box deletion (L0 == 0 seeds the frontier)
+void launch_box_delete_bfs(Pool pool, const float* boxes, int nb, int2* f0, int2* f1, int fcap, int L0, int L1,
+                           unsigned int* recs, DevHdr* hdr, int cap, hipStream_t s);
 void launch_box_delete_apply(Pool pool, const unsigned int* recs, DevHdr* hdr, int bound, int ds, int* seeds,
                              hipStream_t s);
 void launch_delete_points_find(Pool pool, const float* xyz, int n, int* cand, DevHdr* hdr, hipStream_t s);
@@ -129,7 +131,7 @@ void launch_voxel_starts(const int* flag, const int* scan, int n, int* vstart, D
 void launch_voxel_fold2(Pool pool, const float4* pts, const uint32_t* order, const unsigned long long* keys,
                         const int* vstart, int n, int bound, float ds, int check, float* del_boxes, int* del_flag,
                         float4* ins_pts, int* ins_flag, DevHdr* hdr, long long max_steps, int* del_out, int del_cap,
-                        hipStream_t s);
+                        int stack_cap, hipStream_t s);
 void launch_compact_boxes(const int* flag, const int* scan, const float* in, int bound, DevHdr* hdr, float* out,
                           int* out_count, hipStream_t s);
 void launch_compact_pts(const int* flag, const int* scan, const float4* in, int bound, DevHdr* hdr, float4* out,

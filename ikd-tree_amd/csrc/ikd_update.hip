@@ -480,59 +480,80 @@ void launch_insert_link(Pool pool, const int* attach, const int4* seg, DevHdr* h
 }
 
 // ============================================================ box deletion
-// Delete_by_range (cpp:647-710) for many boxes at once, one lane per box.
+// Delete_by_range (cpp:647-710) for many boxes at once.
 // Pass 1 records what each box deletes: COVER for a subtree whose AABB lies
 // inside the half-open box (the subtree root takes the lazy label), POINT
 // for a single live node point inside it.
 constexpr uint32_t REC_COVER = 0x80000000u;
 
-__global__ void k_box_delete_traverse(Pool pool, const float* __restrict__ boxes, int nb,
-                                      unsigned int* __restrict__ recs, DevHdr* hdr, int cap, long long max_steps) {
+// Pass 1 runs breadth-first over all boxes at once: a frontier of (node, box)
+// pairs per tree level, one lane per pair, so a box that cuts through much of
+// the tree (a sliding-window trim) is walked by many lanes instead of one.
+// front3[lvl % 3] holds the level's frontier size.
+__global__ void k_box_delete_init(Pool pool, const float* __restrict__ boxes, int nb, int2* __restrict__ front,
+                                  unsigned int* __restrict__ recs, DevHdr* hdr, int cap) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
-    float box[6];
-#pragma unroll
-    for (int a = 0; a < 6; ++a) box[a] = boxes[6 * b + a];
     uint32_t lk0 = load_link(pool.hot, SENTINEL);
     if (!(lk0 & LK_HASL) || (lk0 & LK_TDL)) return;
-    auto rec = [&](uint32_t v) {
-        int pos = atomicAdd(&hdr->rec_count, 1);
-        if (pos < cap) recs[pos] = v;
-    };
+    const float* box = boxes + 6 * b;
     const float* rb = pool.hot[SENTINEL].f + 4;
     if (box_disjoint(box, rb)) return;
     if (box_covers(box, rb)) {
-        rec(REC_COVER | (uint32_t)ROOT);
+        int pos = atomicAdd(&hdr->rec_count, 1);
+        if (pos < cap) recs[pos] = REC_COVER | (uint32_t)ROOT;
         return;
     }
-    int stack[2 * MAX_DEPTH + 4];
-    int sp = 0;
-    stack[sp++] = ROOT;
-    long long steps = 0;
-    while (sp > 0) {
-        if (++steps > max_steps) {
-            atomicOr(&hdr->overflow, 2);
-            break;
-        }
-        int u = stack[--sp];
+    int pos = atomicAdd(&hdr->front3[0], 1);
+    front[pos] = make_int2(ROOT, b);  // nb <= the frontier capacity
+}
+
+__global__ void k_box_delete_level(Pool pool, const float* __restrict__ boxes, const int2* __restrict__ front,
+                                   int lvl, int2* __restrict__ front_next, unsigned int* __restrict__ recs,
+                                   DevHdr* hdr, int fcap, int cap) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid == 0) hdr->front3[(lvl + 2) % 3] = 0;
+    const int count = min(hdr->front3[lvl % 3], fcap);
+    for (int i = tid; i < count; i += gridDim.x * blockDim.x) {
+        const int2 e = front[i];
+        const int u = e.x;
+        const float* box = boxes + 6 * e.y;
         const float* f = pool.hot[u].f;
-        uint32_t lk = fbits(f[3]);
-        if (!(lk & LK_PDEL) && box_contains(box, f[0], f[1], f[2])) rec((uint32_t)u);
-        int pair = (int)lk_pair(lk);
+        const uint32_t lk = fbits(f[3]);
+        if (!(lk & LK_PDEL) && box_contains(box, f[0], f[1], f[2])) {
+            int pos = atomicAdd(&hdr->rec_count, 1);
+            if (pos < cap) recs[pos] = (uint32_t)u;
+        }
+        const int pair = (int)lk_pair(lk);
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             if (!(lk & lk_has(s)) || (lk & lk_td(s))) continue;
             const float* cb = f + 4 + 6 * s;
             if (box_disjoint(box, cb)) continue;
-            int c = 2 * pair + s;
+            const int c = 2 * pair + s;
             if (box_covers(box, cb)) {
-                rec(REC_COVER | (uint32_t)c);
-            } else if (sp < 2 * MAX_DEPTH + 4) {
-                stack[sp++] = c;
+                int pos = atomicAdd(&hdr->rec_count, 1);
+                if (pos < cap) recs[pos] = REC_COVER | (uint32_t)c;
             } else {
-                atomicOr(&hdr->overflow, 2);
+                int pos = atomicAdd(&hdr->front3[(lvl + 1) % 3], 1);
+                if (pos < fcap) front_next[pos] = make_int2(c, e.y);
+                else hdr->front_over = 1;
             }
         }
+    }
+}
+
+void launch_box_delete_bfs(Pool pool, const float* boxes, int nb, int2* f0, int2* f1, int fcap, int L0, int L1,
+                           unsigned int* recs, DevHdr* hdr, int cap, hipStream_t s) {
+    if (nb <= 0) return;
+    if (L0 == 0)
+        hipLaunchKernelGGL(k_box_delete_init, dim3(blocks_for(nb, 64)), dim3(64), 0, s, pool, boxes, nb, f0, recs,
+                           hdr, cap);
+    for (int L = L0; L < L1; ++L) {
+        const int2* fin = (L & 1) ? f1 : f0;
+        int2* fout = (L & 1) ? f0 : f1;
+        hipLaunchKernelGGL(k_box_delete_level, dim3(512), dim3(256), 0, s, pool, boxes, fin, L, fout, recs, hdr, fcap,
+                           cap);
     }
 }
 
@@ -562,12 +583,6 @@ __global__ void k_box_delete_apply(Pool pool, const unsigned int* __restrict__ r
     }
 }
 
-void launch_box_delete_traverse(Pool pool, const float* boxes, int nb, unsigned int* recs, DevHdr* hdr, int cap,
-                                long long max_steps, hipStream_t s) {
-    if (nb <= 0) return;
-    hipLaunchKernelGGL(k_box_delete_traverse, dim3(blocks_for(nb, 64)), dim3(64), 0, s, pool, boxes, nb, recs, hdr,
-                       cap, max_steps);
-}
 void launch_box_delete_apply(Pool pool, const unsigned int* recs, DevHdr* hdr, int bound, int ds, int* seeds,
                              hipStream_t s) {
     if (bound <= 0) return;

@@ -123,7 +123,10 @@ __global__ void k_voxel_fold(Pool pool, const float4* __restrict__ pts, const ui
                              const unsigned long long* __restrict__ keys, const int* __restrict__ vstart, int n,
                              int bound, float ds, int check, float* __restrict__ del_boxes, int* __restrict__ del_flag,
                              float4* __restrict__ ins_pts, int* __restrict__ ins_flag, DevHdr* hdr,
-                             long long max_steps, int* __restrict__ del_out, int del_cap) {
+                             long long max_steps, int* __restrict__ del_out, int del_cap, int stack_cap) {
+    // per-lane depth-first stacks in LDS, [stack_cap][64]
+    extern __shared__ int lds_dyn[];
+    const int lane = threadIdx.x;
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= bound || v >= hdr->group_count) return;
     int j0 = vstart[v], j1 = vstart[v + 1];
@@ -150,16 +153,16 @@ __global__ void k_voxel_fold(Pool pool, const float4* __restrict__ pts, const ui
     float best_sd = INFINITY;
     uint32_t lk0 = fbits(pool.hot[SENTINEL].f[3]);
     if ((lk0 & LK_HASL) && !(lk0 & LK_TDL) && !box_disjoint(box, pool.hot[SENTINEL].f + 4)) {
-        int stack[2 * MAX_DEPTH + 4];
         int sp = 0;
-        stack[sp++] = ROOT;
+        lds_dyn[lane] = ROOT;
+        sp = 1;
         long long steps = 0;
         while (sp > 0) {
             if (++steps > max_steps) {
                 atomicOr(&hdr->overflow, 4);
                 break;
             }
-            int u = stack[--sp];
+            int u = lds_dyn[--sp * 64 + lane];
             const float* f = pool.hot[u].f;
             uint32_t lk = fbits(f[3]);
             if (!(lk & LK_PDEL) && box_contains(box, f[0], f[1], f[2])) {
@@ -180,7 +183,7 @@ __global__ void k_voxel_fold(Pool pool, const float4* __restrict__ pts, const ui
             for (int s = 1; s >= 0; --s) {
                 if (!(lk & lk_has(s)) || (lk & lk_td(s))) continue;
                 if (box_disjoint(box, f + 4 + 6 * s)) continue;
-                if (sp < 2 * MAX_DEPTH + 4) stack[sp++] = 2 * pair + s;
+                if (sp < stack_cap) lds_dyn[sp++ * 64 + lane] = 2 * pair + s;
                 else atomicOr(&hdr->overflow, 4);
             }
         }
@@ -241,16 +244,16 @@ __global__ void k_voxel_fold(Pool pool, const float4* __restrict__ pts, const ui
     // new point replaces them: Delete_by_range(voxel, downsample) of cpp:446.
     // Voxel boxes are disjoint here, so each S0 node is written by one lane.
     if (del_out && have_new && n0 > 0 && split == 0x7fffffff) {
-        int stack[2 * MAX_DEPTH + 4];
         int sp = 0;
-        stack[sp++] = ROOT;
+        lds_dyn[lane] = ROOT;
+        sp = 1;
         long long steps = 0;
         while (sp > 0) {
             if (++steps > max_steps) {
                 atomicOr(&hdr->overflow, 4);
                 break;
             }
-            int u = stack[--sp];
+            int u = lds_dyn[--sp * 64 + lane];
             float* f = pool.hot[u].f;
             uint32_t lk = fbits(f[3]);
             if (!(lk & LK_PDEL) && box_contains(box, f[0], f[1], f[2])) {
@@ -265,7 +268,7 @@ __global__ void k_voxel_fold(Pool pool, const float4* __restrict__ pts, const ui
             for (int s2 = 1; s2 >= 0; --s2) {
                 if (!(lk & lk_has(s2)) || (lk & lk_td(s2))) continue;
                 if (box_disjoint(box, f + 4 + 6 * s2)) continue;
-                if (sp < 2 * MAX_DEPTH + 4) stack[sp++] = 2 * pair + s2;
+                if (sp < stack_cap) lds_dyn[sp++ * 64 + lane] = 2 * pair + s2;
                 else atomicOr(&hdr->overflow, 4);
             }
         }
@@ -312,10 +315,11 @@ void launch_voxel_starts(const int* flag, const int* scan, int n, int* vstart, D
 void launch_voxel_fold2(Pool pool, const float4* pts, const uint32_t* order, const unsigned long long* keys,
                         const int* vstart, int n, int bound, float ds, int check, float* del_boxes, int* del_flag,
                         float4* ins_pts, int* ins_flag, DevHdr* hdr, long long max_steps, int* del_out, int del_cap,
-                        hipStream_t s) {
+                        int stack_cap, hipStream_t s) {
     if (bound <= 0) return;
-    hipLaunchKernelGGL(k_voxel_fold, dim3(blocks_for(bound, 64)), dim3(64), 0, s, pool, pts, order, keys, vstart, n,
-                       bound, ds, check, del_boxes, del_flag, ins_pts, ins_flag, hdr, max_steps, del_out, del_cap);
+    hipLaunchKernelGGL(k_voxel_fold, dim3(blocks_for(bound, 64)), dim3(64), (size_t)stack_cap * 64 * sizeof(int), s,
+                       pool, pts, order, keys, vstart, n, bound, ds, check, del_boxes, del_flag, ins_pts, ins_flag, hdr,
+                       max_steps, del_out, del_cap, stack_cap);
 }
 void launch_compact_boxes(const int* flag, const int* scan, const float* in, int bound, DevHdr* hdr, float* out,
                           int* out_count, hipStream_t s) {

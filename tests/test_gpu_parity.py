@@ -91,6 +91,22 @@ def test_build_structure_matches_oracle(ikd, n, small):
     _same_structure(g.dump_nodes(), o.dump())
 
 
+@pytest.mark.parametrize("n", [700, 1500, 5000, 40000])
+def test_build_paths_identical_with_ties(ikd, n):
+    """Grid points (many equal keys on every axis): the level-synchronous build
+    and the hybrid build (levels, then one workgroup per small segment) place
+    equal keys the same way, so the two trees are node for node the same."""
+    rng = np.random.default_rng(70 + n)
+    pts = grid_cloud(rng, n, step=0.5, extent=12)
+    dumps = []
+    for small in (0, 1):
+        g = ikd.KD_TREE()
+        g.set_tuning(ikd.IKD_TUNE_SMALL_BUILD, small)
+        g.Build(pts)
+        dumps.append(g.dump_nodes())
+    _same_structure(dumps[0], dumps[1])
+
+
 def test_morton_order_does_not_change_results(ikd):
     rng = np.random.default_rng(7)
     pts = uniform_cloud(rng, 5000)
