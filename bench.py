@@ -270,7 +270,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": b_gbs / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(),
             "kernel": knn_kernel_name(k, b_cfg[0]),
             "algorithmic_bytes_per_launch": b_bpl,
             "avg_launch_ms": b_avg * 1e3,
@@ -504,6 +504,19 @@ def copy_calibration(torch, mib):
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 10}))
+
+
+def measured_traffic():
+    """HBM bytes per batched launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (profiles/knn_2M_map_hbm_traffic.md),
+    or None when that summary is absent."""
+    p = os.path.join(ROOT, "profiles", "knn_2M_map_hbm_traffic.md")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def knn_kernel_name(k, lanes):

@@ -66,17 +66,18 @@ def build_library(jobs: int = 8) -> str:
     return LIB_PATH
 
 
-_lib = None
+_libs = {}
 
 
-def load_library():
-    """Load the HIP library; raises ImportError if it has not been built."""
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
-        raise ImportError(f"{LIB_PATH} is missing: build it with build_library() or `make -C {CSRC}`")
-    lib = ctypes.CDLL(LIB_PATH)
+def load_library(path: str | None = None):
+    """Load the HIP library (or, for debugging, another build of the same
+    sources at `path`); raises ImportError if it has not been built."""
+    path = path or LIB_PATH
+    if path in _libs:
+        return _libs[path]
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with build_library() or `make -C {CSRC}`")
+    lib = ctypes.CDLL(path)
     vp, i64, i32, f32, f64 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_float, ctypes.c_double
     sig = {
         "ikd_create": [f32, f32, f32, ctypes.POINTER(vp)],
@@ -122,7 +123,7 @@ def load_library():
     lib.ikd_last_error.restype = ctypes.c_char_p
     lib.ikd_batch_last_error.argtypes = [vp]
     lib.ikd_batch_last_error.restype = ctypes.c_char_p
-    _lib = lib
+    _libs[path] = lib
     return lib
 
 
@@ -148,8 +149,8 @@ def _tensor_ptr(t):
 class KD_TREE:
     """``KD_TREE<PointType>`` (ikd-Tree/ikd_Tree.h:59-249) on one MI355X."""
 
-    def __init__(self, delete_param: float = 0.5, balance_param: float = 0.6, box_length: float = 0.2):
-        self._lib = load_library()
+    def __init__(self, delete_param: float = 0.5, balance_param: float = 0.6, box_length: float = 0.2, lib=None):
+        self._lib = lib if lib is not None else load_library()
         h = ctypes.c_void_p()
         self._check(self._lib.ikd_create(delete_param, balance_param, box_length, ctypes.byref(h)), handle=None)
         self._h = h
@@ -362,8 +363,8 @@ class Batch:
     tensor, or None for k_max) and max_dist (a float64 tensor, or one float).
     Results are identical to one ``KD_TREE.knn_device`` call per tree."""
 
-    def __init__(self):
-        self._lib = load_library()
+    def __init__(self, lib=None):
+        self._lib = lib if lib is not None else load_library()
         h = ctypes.c_void_p()
         rc = self._lib.ikd_batch_create(ctypes.byref(h))
         if rc != IKD_OK:

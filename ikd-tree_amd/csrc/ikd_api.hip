@@ -488,7 +488,7 @@ int run_build(ikd_tree_t t, const float4* pts, const int* seg_of, int m, const i
         CHK(ensure(t, B_PTS3, m, &gpts));
         CHK(ensure(t, B_SPAIRS, bound, &pairs));
         CHK(ensure(t, B_SPSCAN, bound, &pscan));
-        launch_gather_list(pts, perm[cur][0], m, gpts, st);
+        launch_gather_list(pts, perm[cur][0], sid[cur], m, gpts, st);
         const int* count_dev = &t->d_hdr->seg_count[cur];
         launch_small_pairs(seg[cur], bound, count_dev, pairs, st);
         CHK(exscan_i32(t, pairs, pscan, bound));

@@ -198,15 +198,19 @@ __global__ void k_build_scatter(BuildLevel L) {
 static inline int blocks_for(long long n, int bs) { return (int)((n + bs - 1) / bs); }
 
 // out[j] = pts[perm[j]]: a level's points in list order, so every segment's
-// points are contiguous for the one-workgroup build of the levels below
-__global__ void k_gather_list(const float4* __restrict__ pts, const uint32_t* __restrict__ perm, int m,
-                              float4* __restrict__ out) {
+// points are contiguous for the one-workgroup build of the levels below.
+// Positions whose point is already a node of an upper level (sid < 0) are
+// never written by the partitions and hold no valid index: skip them.
+__global__ void k_gather_list(const float4* __restrict__ pts, const uint32_t* __restrict__ perm,
+                              const int* __restrict__ sid, int m, float4* __restrict__ out) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < m) out[j] = pts[perm[j]];
+    if (j >= m) return;
+    out[j] = sid[j] >= 0 ? pts[perm[j]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
-void launch_gather_list(const float4* pts, const uint32_t* perm, int m, float4* out, hipStream_t s) {
+void launch_gather_list(const float4* pts, const uint32_t* perm, const int* sid, int m, float4* out,
+                        hipStream_t s) {
     if (m <= 0) return;
-    hipLaunchKernelGGL(k_gather_list, dim3(blocks_for(m, 256)), dim3(256), 0, s, pts, perm, m, out);
+    hipLaunchKernelGGL(k_gather_list, dim3(blocks_for(m, 256)), dim3(256), 0, s, pts, perm, sid, m, out);
 }
 
 void launch_build_keys(const float4* pts, const int* seg_of, int m, int axis, unsigned long long* keys,

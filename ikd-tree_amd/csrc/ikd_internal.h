@@ -82,7 +82,7 @@ void launch_build_scatter(const BuildLevel& L, hipStream_t s);
 void launch_small_pairs(const int4* seg, int S, const int* count_dev, int* pairs, hipStream_t s);
 void launch_build_small(const float4* pts, const int4* seg, const int* seg_depth, const int* pair_scan, int S,
                         const int* count_dev, const uint32_t* orig, Pool pool, DevHdr* hdr, hipStream_t s);
-void launch_gather_list(const float4* pts, const uint32_t* perm, int m, float4* out, hipStream_t s);
+void launch_gather_list(const float4* pts, const uint32_t* perm, const int* sid, int m, float4* out, hipStream_t s);
 void launch_small_commit(const int* pairs, const int* pair_scan, int S, DevHdr* hdr, hipStream_t s);
 
 // ---------------------------------------------------------------- refit

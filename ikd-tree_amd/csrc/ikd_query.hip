@@ -344,9 +344,13 @@ __global__ __launch_bounds__(64) void k_knn_team(KnnArgs a) {
         // lanes still idle receive the bottom entry of a busy teammate
         const unsigned long long imask = __ballot(valid && n < 0) & team_mask;
         const unsigned long long dmask = __ballot(tail != head) & team_mask;
-        if (imask != 0 && dmask != 0) {
+        // teams with idle and busy lanes hand work over; the fences run on
+        // every lane of the wavefront whenever any team does, so the wave
+        // barriers are never inside team-divergent control flow
+        const bool donate = imask != 0 && dmask != 0;
+        if (__ballot(donate) != 0) {
             const int ni = __popcll(imask), nd = __popcll(dmask);
-            if (tail != head) {
+            if (donate && tail != head) {
                 const int r = __popcll(dmask & below);
                 if (r < ni) {
                     const int s = head & smask;
@@ -356,7 +360,7 @@ __global__ __launch_bounds__(64) void k_knn_team(KnnArgs a) {
                 }
             }
             wave_lds_fence();
-            if (valid && n < 0) {
+            if (donate && valid && n < 0) {
                 const int r = __popcll(imask & below);
                 if (r < nd) {
                     n = dn_n[team][r];

@@ -12,3 +12,14 @@ if HERE not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the built HIP library")
+
+
+def pytest_sessionstart(session):
+    """IKD_TEST_LIB=<path> points the package at another build of the same
+    sources (the CPU emulation build in tests/cpu_emu, for ASan runs of the
+    gpu-marked tests on a machine without a GPU)."""
+    path = os.environ.get("IKD_TEST_LIB")
+    if path:
+        import importlib
+        pkg = importlib.import_module("ikd-tree_amd")
+        pkg.LIB_PATH = os.path.abspath(path)

@@ -6,6 +6,7 @@ entry point, and equal to brute force under the (d2, x, y, z, id) order; uneven
 tree sizes, uneven query counts, mixed k and mixed max_dist in one launch.
 """
 import importlib
+import os
 
 import numpy as np
 import pytest
@@ -15,6 +16,8 @@ from common import INF, brute_knn, grid_cloud, uniform_cloud
 pytestmark = pytest.mark.gpu
 
 torch = pytest.importorskip("torch")
+# the device the tensors live on ("cpu" for the CPU emulation build of the library)
+DEV = os.environ.get("IKD_TEST_DEVICE", "cuda:0")
 
 
 @pytest.fixture(scope="module")
@@ -47,7 +50,7 @@ def _mixed_batch(seed, sizes, nqs, ks=(1, 5, 10, 20), dists=(1.0, 5.0, INF), gri
 
 @pytest.mark.parametrize("lanes", [-1, 1, 2, 0])
 def test_batch_matches_single_tree_calls(ikd, lanes):
-    dev = torch.device("cuda:0")
+    dev = torch.device(DEV)
     sizes = [1, 17, 900, 5000, 40000, 120000]
     nqs = [30, 200, 1000, 3000, 700, 9000]
     clouds, qt, q, qk, qd = _mixed_batch(11, sizes, nqs, grid_tree=3)
@@ -89,7 +92,7 @@ def test_batch_matches_single_tree_calls(ikd, lanes):
 
 
 def test_batch_uniform_k_and_bad_tree_index(ikd):
-    dev = torch.device("cuda:0")
+    dev = torch.device(DEV)
     rng = np.random.default_rng(5)
     a, c = ikd.KD_TREE(), ikd.KD_TREE()
     pa, pc = uniform_cloud(rng, 3000), uniform_cloud(rng, 500)
@@ -115,7 +118,7 @@ def test_batch_uniform_k_and_bad_tree_index(ikd):
 
 def test_batch_orders_after_tree_updates(ikd):
     """A launch sees every update queued on the trees before it, without a host sync."""
-    dev = torch.device("cuda:0")
+    dev = torch.device(DEV)
     rng = np.random.default_rng(9)
     t = ikd.KD_TREE(0.5, 0.6, 0.5)
     t.Build(uniform_cloud(rng, 20000))
@@ -133,7 +136,7 @@ def test_batch_orders_after_tree_updates(ikd):
 
 
 def test_batch_argument_errors(ikd):
-    dev = torch.device("cuda:0")
+    dev = torch.device(DEV)
     t = ikd.KD_TREE()
     t.Build(uniform_cloud(np.random.default_rng(1), 100))
     b = ikd.Batch()
@@ -148,7 +151,7 @@ def test_batch_argument_errors(ikd):
 
 
 def test_batch_radius_matches_single_tree_calls(ikd):
-    dev = torch.device("cuda:0")
+    dev = torch.device(DEV)
     rng = np.random.default_rng(21)
     sizes = [50, 3000, 60000]
     trees, clouds = [], []
