@@ -57,9 +57,10 @@ def test_product_path_does_not_import_the_oracle():
 def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     pkg = _pkg()
     monkeypatch.setattr(pkg, "LIB_PATH", str(tmp_path / "absent.so"))
-    monkeypatch.setattr(pkg, "_lib", None)
     with pytest.raises(ImportError):
         pkg.load_library()
+    with pytest.raises(ImportError):
+        pkg.KD_TREE()
 
 
 def _compile_facade(out_path):
