@@ -189,7 +189,7 @@ enum Buf {
     B_SEEDS, B_DIRTY, B_VIOL, B_TOP, B_INFO, B_FRONT0, B_FRONT1, B_EKEY0, B_EKEY1, B_RECS, B_GFLAG, B_GSCAN, B_ATTACH,
     B_PFLAG, B_PSCAN, B_ISEG, B_ISEGD, B_IO0, B_IO1, B_IO2, B_IO3, B_IO4, B_VSTART, B_DELBOX, B_DELFL, B_INSPT,
     B_INSFL, B_CBOX, B_CPTS, B_MISC, B_MORTK, B_MORTV, B_MORTK2, B_MORTV2, B_CNT64, B_OFF64, B_ADDX, B_ADDI, B_PNEXT, B_SPAIRS, B_SPSCAN,
-    B_DEFER, B_DINFO, B_DBEFORE, B_DKEEP, B_PTS3, B_COUNT
+    B_DEFER, B_DINFO, B_DBEFORE, B_DKEEP, B_PTS3, B_PEND, B_START, B_COUNT
 };
 static_assert(B_COUNT <= 80, "scratch table");
 
@@ -561,9 +561,10 @@ int refit_begin(ikd_tree_t t) {
     return IKD_OK;
 }
 int refit_mark(ikd_tree_t t, const int* seeds, const int* nseeds_dev, int bound, int stop_at_tdel) {
-    int* dirty;
+    int *dirty, *pend;
     CHK(ensure(t, B_DIRTY, std::max<long long>(t->pool_used, 16), &dirty));
-    launch_mark_dirty(t->pool, seeds, nseeds_dev, bound, stop_at_tdel, dirty, t->d_hdr, t->st);
+    CHK(ensure(t, B_PEND, std::max<long long>(t->cap, 16), &pend));  // indexed by node
+    launch_mark_dirty(t->pool, seeds, nseeds_dev, bound, stop_at_tdel, dirty, pend, t->d_hdr, t->st);
     return IKD_OK;
 }
 int refit_finish(ikd_tree_t t) {
@@ -572,10 +573,11 @@ int refit_finish(ikd_tree_t t) {
     CHK(ensure(t, B_DIRTY, cap, &dirty));
     CHK(ensure(t, B_VIOL, 2 * cap, &viol));
     CHK(ensure(t, B_TOP, 2 * cap, &top));
-    int* defer;
+    int *defer, *pend, *start;
     CHK(ensure(t, B_DEFER, DEFER_CAP, &defer));
-    for (int d = t->max_depth; d >= 0; --d)
-        launch_refit_level(t->pool, t->prm, dirty, t->d_hdr, d, viol, defer, t->st);
+    CHK(ensure(t, B_PEND, std::max<long long>(t->cap, 16), &pend));
+    CHK(ensure(t, B_START, cap, &start));
+    launch_refit_up(t->pool, t->prm, dirty, pend, start, t->d_hdr, viol, defer, t->st);
     launch_find_top(t->pool, viol, (int)cap, t->d_hdr, top, t->st);
     launch_clear_viol(t->pool, viol, (int)cap, t->d_hdr, t->st);
     HIPCHK(hipGetLastError());

@@ -75,22 +75,34 @@ __device__ __forceinline__ void write_child_box_s(NodeHot* hot, int parent, int 
     }
 }
 
-// Exclusive scan of one int per thread across the workgroup; *total gets the sum.
+// Exclusive scan of one 64-bit value per thread across the workgroup
+// (*total gets the sum): an inclusive scan inside each wavefront by
+// shuffles, then the wavefront totals through LDS (two barriers).  Several
+// small counters can ride in one value as bit fields that never overflow.
 template <int SB>
-__device__ int block_exscan(int v, int* tmp, int* total) {
-    const int t = threadIdx.x;
-    tmp[t] = v;
-    __syncthreads();
-    for (int off = 1; off < SB; off <<= 1) {
-        int x = t >= off ? tmp[t - off] : 0;
-        __syncthreads();
-        tmp[t] += x;
-        __syncthreads();
+__device__ unsigned long long block_exscan_u64(unsigned long long v, unsigned long long* wtot,
+                                               unsigned long long* total) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    unsigned long long x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int src = (lane - off) & 63;  // lanes below off ignore what they read
+        const unsigned lo = (unsigned)__shfl((int)(unsigned)x, src);
+        const unsigned hi = (unsigned)__shfl((int)(unsigned)(x >> 32), src);
+        const unsigned long long y = ((unsigned long long)hi << 32) | lo;
+        if (lane >= off) x += y;
     }
-    *total = tmp[SB - 1];
-    int ex = tmp[t] - v;
+    if (lane == 63) wtot[w] = x;
     __syncthreads();
-    return ex;
+    unsigned long long base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < SB / 64; ++i) {
+        if (i < w) base += wtot[i];
+        tot += wtot[i];
+    }
+    *total = tot;
+    __syncthreads();
+    return base + x - v;
 }
 
 // Ascending bitonic sort of skey[0, P) (P a power of two) by the workgroup.
@@ -134,7 +146,7 @@ __global__ __launch_bounds__(SB) void k_build_small(const float4* __restrict__ p
     __shared__ int seg_node[2][SMAX_SEG], seg_par[2][SMAX_SEG];
     __shared__ unsigned char seg_axis[SMAX_SEG];
     __shared__ short seg_mid[SMAX_SEG], seg_cl[SMAX_SEG], seg_cr[SMAX_SEG];
-    __shared__ int scan_tmp[SB];
+    __shared__ unsigned long long wtot[SB / 64];
     __shared__ unsigned short rnk[SMAX];
 
     const int g = blockIdx.x;
@@ -186,7 +198,7 @@ __global__ __launch_bounds__(SB) void k_build_small(const float4* __restrict__ p
         seg_par[0][0] = sg.w;
     }
     __syncthreads();
-    unsigned int* run = reinterpret_cast<unsigned int*>(skey);
+    unsigned long long* run = skey;  // the sort keys' space, reused for running counts
     const int chunk = (m + SB - 1) / SB;
     int ns = 1;
     int cur = 0;
@@ -212,9 +224,11 @@ __global__ __launch_bounds__(SB) void k_build_small(const float4* __restrict__ p
             my_pairs += (hl || hr) ? 1 : 0;
             my_children += hl + hr;
         }
-        int tot_pairs, tot_children;
-        int pair_off = block_exscan<SB>(my_pairs, scan_tmp, &tot_pairs);
-        int child_off = block_exscan<SB>(my_children, scan_tmp, &tot_children);
+        unsigned long long tot2;
+        const unsigned long long off2 =
+            block_exscan_u64<SB>((unsigned long long)my_pairs | ((unsigned long long)my_children << 32), wtot, &tot2);
+        int pair_off = (int)(unsigned)off2, child_off = (int)(off2 >> 32);
+        const int tot_pairs = (int)(unsigned)tot2, tot_children = (int)(tot2 >> 32);
         for (int q = 0; q < per; ++q) {
             const int s = t * per + q;
             if (s >= ns) break;
@@ -263,32 +277,36 @@ __global__ __launch_bounds__(SB) void k_build_small(const float4* __restrict__ p
             side[lst[cur][seg_axis[s]][j]] = (unsigned char)(j < mid ? 0 : (j == mid ? 2 : 1));
         }
         __syncthreads();
-        // stable partition of each list: running counts of left / right
-        // entries before every position, then offsets inside the segment
+        // stable partition of the three lists at once: for every position, the
+        // numbers of left and right entries of each list before it, as six
+        // 10-bit fields of one value (no field reaches 1024: at most m - 1
+        // entries move), so one scan serves all three lists
+        unsigned long long cnt6 = 0;
+        for (int q = 0; q < chunk; ++q) {
+            const int j = t * chunk + q;
+            if (j >= m) break;
+            if (sid[cur][j] < 0) continue;
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                const unsigned char sd = side[lst[cur][b][j]];
+                if (sd < 2) cnt6 += 1ull << (20 * b + 10 * sd);
+            }
+        }
+        unsigned long long tot6;
+        unsigned long long acc = block_exscan_u64<SB>(cnt6, wtot, &tot6);
+        for (int q = 0; q < chunk; ++q) {
+            const int j = t * chunk + q;
+            if (j >= m) break;
+            run[j] = acc;
+            if (sid[cur][j] < 0) continue;
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                const unsigned char sd = side[lst[cur][b][j]];
+                if (sd < 2) acc += 1ull << (20 * b + 10 * sd);
+            }
+        }
+        __syncthreads();
         for (int b = 0; b < 3; ++b) {
-            int cl = 0, cr = 0;
-            for (int q = 0; q < chunk; ++q) {
-                const int j = t * chunk + q;
-                if (j >= m) break;
-                if (sid[cur][j] < 0) continue;
-                const unsigned char sd = side[lst[cur][b][j]];
-                cl += sd == 0;
-                cr += sd == 1;
-            }
-            int tl_, tr_;
-            int a0 = block_exscan<SB>(cl, scan_tmp, &tl_);
-            int a1 = block_exscan<SB>(cr, scan_tmp, &tr_);
-            for (int q = 0; q < chunk; ++q) {
-                const int j = t * chunk + q;
-                if (j >= m) break;
-                run[2 * j] = (unsigned)a0;
-                run[2 * j + 1] = (unsigned)a1;
-                if (sid[cur][j] < 0) continue;
-                const unsigned char sd = side[lst[cur][b][j]];
-                a0 += sd == 0;
-                a1 += sd == 1;
-            }
-            __syncthreads();
             for (int q = 0; q < chunk; ++q) {
                 const int j = t * chunk + q;
                 if (j >= m) break;
@@ -304,13 +322,15 @@ __global__ __launch_bounds__(SB) void k_build_small(const float4* __restrict__ p
                     continue;
                 }
                 const int l = seg_l[cur][s], mid = seg_mid[s];
-                const int np = sd == 0 ? l + (int)(run[2 * j] - run[2 * l])
-                                       : mid + 1 + (int)(run[2 * j + 1] - run[2 * l + 1]);
+                // fields only grow along the list, so the packed difference
+                // subtracts field by field without borrows
+                const int before = (int)(((run[j] - run[l]) >> (20 * b + 10 * sd)) & 1023ull);
+                const int np = sd == 0 ? l + before : mid + 1 + before;
                 lst[cur ^ 1][b][np] = (unsigned short)p;
                 if (b == 0) sid[cur ^ 1][np] = sd == 0 ? seg_cl[s] : seg_cr[s];
             }
-            __syncthreads();
         }
+        __syncthreads();
         ns = tot_children;
         cur ^= 1;
     }

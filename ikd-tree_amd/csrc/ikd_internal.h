@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "ikd_common.h"
+#include "ikd_gload.h"
 
 namespace ikd {
 
@@ -86,13 +87,13 @@ void launch_gather_list(const float4* pts, const uint32_t* perm, const int* sid,
 void launch_small_commit(const int* pairs, const int* pair_scan, int S, DevHdr* hdr, hipStream_t s);
 
 // ---------------------------------------------------------------- refit
+void launch_refit_up(Pool pool, Params prm, const int* dirty, int* pend, int* start, DevHdr* hdr, int* viol,
+                     int* defer, hipStream_t s);
 void launch_mark_dirty(Pool pool, const int* seeds, const int* nseeds_dev, int bound, int stop_at_tdel,
-                       int* dirty, DevHdr* hdr, hipStream_t s);
+                       int* dirty, int* pend, DevHdr* hdr, hipStream_t s);
 void launch_defer_check(Pool pool, Params prm, const int* defer, int n, int4* info, int* before, int* keep,
                         hipStream_t s);
 void launch_defer_clear(Pool pool, const int* defer, int n, hipStream_t s);
-void launch_refit_level(Pool pool, Params prm, const int* dirty, DevHdr* hdr, int depth, int* viol, int* defer,
-                        hipStream_t s);
 void launch_find_top(Pool pool, const int* viol, int bound, DevHdr* hdr, int* top, hipStream_t s);
 void launch_clear_viol(Pool pool, const int* viol, int bound, DevHdr* hdr, hipStream_t s);
 

@@ -155,8 +155,8 @@ __device__ __forceinline__ void finish_query(const KnnArgs& a, const QueryCtx& c
 template <int KM>
 __device__ __forceinline__ void visit_node(const QueryCtx& c, BestList<KM>& best, int n, int& near_n, float& dn,
                                            int& far_n, float& df, unsigned int& boxes) {
-    const float4* rec = reinterpret_cast<const float4*>(c.hot[n].f);
-    const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
+    const gv4* rec = reinterpret_cast<const gv4*>(c.hot[n].f);
+    const gv4 r0 = gload4(rec), r1 = gload4(rec + 1), r2 = gload4(rec + 2), r3 = gload4(rec + 3);
     const uint32_t lk = fbits(r0.w);
     if (!(lk & LK_PDEL)) {
         float d = calc_dist(c.qx, c.qy, c.qz, r0.x, r0.y, r0.z);
@@ -745,7 +745,8 @@ __global__ __launch_bounds__(64) void k_range(RangeArgs a, int fill) {
             int e = stk[--sp * 64 + lane];
             bool whole = e < 0;
             int u = e & 0x7fffffff;
-            const float* f = hot[u].f;
+            float f[16];
+            load_record(hot[u].f, f);
             uint32_t lk = fbits(f[3]);
             if (!(lk & LK_PDEL)) {
                 bool take;

@@ -39,7 +39,7 @@ __device__ __forceinline__ float smax(float a, float b) { return a < b ? b : a; 
 
 // ============================================================ refit / Update
 __global__ void k_mark_dirty(Pool pool, const int* __restrict__ seeds, const int* nseeds_dev, int bound,
-                             int stop_at_tdel, int* __restrict__ dirty, DevHdr* hdr) {
+                             int stop_at_tdel, int* __restrict__ dirty, int* __restrict__ pend, DevHdr* hdr) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= bound) return;
     if (nseeds_dev && i >= *nseeds_dev) return;
@@ -65,6 +65,7 @@ __global__ void k_mark_dirty(Pool pool, const int* __restrict__ seeds, const int
         if (old & M_DIRTY) break;
         int pos = atomicAdd(&hdr->dirty_count, 1);
         dirty[pos] = u;
+        pend[u] = 0;
         if (u == ROOT) break;
         u = pool.par[u];
         if (step > WALK_LIMIT || u < ROOT) {
@@ -164,14 +165,50 @@ __device__ void refit_node(Pool pool, Params prm, int u, int* viol, int* defer, 
     else atomicAnd(link_ptr(pool.hot, p), ~tdbit);
 }
 
-__global__ void k_refit_level(Pool pool, Params prm, const int* __restrict__ dirty, DevHdr* hdr, int depth,
-                              int* __restrict__ viol, int* __restrict__ defer) {
-    int n = hdr->dirty_count;
+// Single-pass bottom-up refit.  The dirty set is closed under ancestors, so
+// every dirty node but the root has a dirty parent: count each node's dirty
+// children, start one lane at every dirty node without any, and let the lane
+// that finishes a node's last dirty child go on to refit the node itself
+// (the fences order each child's writes before its parent reads them).  The
+// same order as one launch per depth level, in three launches.
+__global__ void k_refit_count(Pool pool, const int* __restrict__ dirty, DevHdr* hdr, int* __restrict__ pend) {
+    const int n = hdr->dirty_count;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        int u = dirty[i];
-        if (pool.cnt[u].w != depth) continue;
-        refit_node(pool, prm, u, viol, defer, hdr);
+        const int u = dirty[i];
+        if (u != ROOT) atomicAdd(&pend[pool.par[u]], 1);
     }
+}
+
+__global__ void k_refit_starts(const int* __restrict__ dirty, DevHdr* hdr, const int* __restrict__ pend,
+                               int* __restrict__ start) {
+    const int n = hdr->dirty_count;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        start[i] = pend[dirty[i]] == 0 ? 1 : 0;
+}
+
+__global__ void k_refit_up(Pool pool, Params prm, const int* __restrict__ dirty, const int* __restrict__ start,
+                           DevHdr* hdr, int* __restrict__ pend, int* __restrict__ viol, int* __restrict__ defer) {
+    const int n = hdr->dirty_count;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (!start[i]) continue;
+        int u = dirty[i];
+        for (int step = 0; step <= WALK_LIMIT; ++step) {
+            refit_node(pool, prm, u, viol, defer, hdr);
+            if (u == ROOT) break;
+            const int p = pool.par[u];
+            __threadfence();
+            if (atomicSub(&pend[p], 1) != 1) break;  // a sibling is still on its way
+            __threadfence();
+            u = p;
+        }
+    }
+}
+
+void launch_refit_up(Pool pool, Params prm, const int* dirty, int* pend, int* start, DevHdr* hdr, int* viol,
+                     int* defer, hipStream_t s) {
+    hipLaunchKernelGGL(k_refit_count, dim3(512), dim3(256), 0, s, pool, dirty, hdr, pend);
+    hipLaunchKernelGGL(k_refit_starts, dim3(512), dim3(256), 0, s, dirty, hdr, pend, start);
+    hipLaunchKernelGGL(k_refit_up, dim3(512), dim3(256), 0, s, pool, prm, dirty, start, hdr, pend, viol, defer);
 }
 
 // Deferred rebuilds: an entry is still due if its node is linked into the
@@ -260,14 +297,10 @@ __global__ void k_clear_viol(Pool pool, const int* __restrict__ viol, DevHdr* hd
 }
 
 void launch_mark_dirty(Pool pool, const int* seeds, const int* nseeds_dev, int bound, int stop_at_tdel,
-                       int* dirty, DevHdr* hdr, hipStream_t s) {
+                       int* dirty, int* pend, DevHdr* hdr, hipStream_t s) {
     if (bound <= 0) return;
     hipLaunchKernelGGL(k_mark_dirty, dim3(blocks_for(bound, 256)), dim3(256), 0, s, pool, seeds, nseeds_dev, bound,
-                       stop_at_tdel, dirty, hdr);
-}
-void launch_refit_level(Pool pool, Params prm, const int* dirty, DevHdr* hdr, int depth, int* viol, int* defer,
-                        hipStream_t s) {
-    hipLaunchKernelGGL(k_refit_level, dim3(512), dim3(256), 0, s, pool, prm, dirty, hdr, depth, viol, defer);
+                       stop_at_tdel, dirty, pend, hdr);
 }
 void launch_find_top(Pool pool, const int* viol, int bound, DevHdr* hdr, int* top, hipStream_t s) {
     if (bound <= 0) return;
@@ -518,7 +551,8 @@ __global__ void k_box_delete_level(Pool pool, const float* __restrict__ boxes, c
         const int2 e = front[i];
         const int u = e.x;
         const float* box = boxes + 6 * e.y;
-        const float* f = pool.hot[u].f;
+        float f[16];
+        load_record(pool.hot[u].f, f);
         const uint32_t lk = fbits(f[3]);
         if (!(lk & LK_PDEL) && box_contains(box, f[0], f[1], f[2])) {
             int pos = atomicAdd(&hdr->rec_count, 1);

@@ -163,7 +163,8 @@ __global__ void k_voxel_fold(Pool pool, const float4* __restrict__ pts, const ui
                 break;
             }
             int u = lds_dyn[--sp * 64 + lane];
-            const float* f = pool.hot[u].f;
+            float f[16];
+            load_record(pool.hot[u].f, f);
             uint32_t lk = fbits(f[3]);
             if (!(lk & LK_PDEL) && box_contains(box, f[0], f[1], f[2])) {
                 ++n0;
@@ -254,11 +255,12 @@ __global__ void k_voxel_fold(Pool pool, const float4* __restrict__ pts, const ui
                 break;
             }
             int u = lds_dyn[--sp * 64 + lane];
-            float* f = pool.hot[u].f;
+            float f[16];
+            load_record(pool.hot[u].f, f);
             uint32_t lk = fbits(f[3]);
             if (!(lk & LK_PDEL) && box_contains(box, f[0], f[1], f[2])) {
                 atomicOr(&pool.meta[u], M_PDEL | M_PDS);
-                atomicOr(reinterpret_cast<uint32_t*>(&f[3]), LK_PDEL);
+                atomicOr(reinterpret_cast<uint32_t*>(&pool.hot[u].f[3]), LK_PDEL);
                 int pos = atomicAdd(&hdr->rec_count, 1);
                 if (pos < del_cap) del_out[pos] = u;
                 else atomicOr(&hdr->overflow, 4);

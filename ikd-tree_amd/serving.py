@@ -214,12 +214,24 @@ def simulate(server: Server, map_ids, scan_fn, rate_hz: float = 10.0, duration_s
     done.extend(server.drain())
     t1 = max(d.req.t_done_ns for d in done)
     lat = np.array([d.latency_ms for d in done])
+    wait = np.array([(d.req.t_start_ns - d.req.t_arrival_ns) / 1e6 for d in done])
+    run = np.array([(d.req.t_done_ns - d.req.t_start_ns) / 1e6 for d in done])
     bad = [d for d in done if d.status != 0]
+    slow = sorted(done, key=lambda d: -d.latency_ms)[:8]
+
+    def pct(a):
+        return {"p50": float(np.percentile(a, 50)), "p99": float(np.percentile(a, 99)), "max": float(a.max())}
+
     return {
         "requests": len(done),
         "failed": len(bad),
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "max": float(lat.max()), "mean": float(lat.mean())},
+        "queue_wait_ms": pct(wait),
+        "run_ms": pct(run),
+        "slowest": [{"robot": int(d.req.tag), "points": int(d.req.n), "trim": int(d.req.n_trim > 0),
+                     "wait_ms": (d.req.t_start_ns - d.req.t_arrival_ns) / 1e6,
+                     "run_ms": (d.req.t_done_ns - d.req.t_start_ns) / 1e6} for d in slow],
         "scan_points_per_s": points / ((t1 - t0) / 1e9),
         "requests_per_s": len(done) / ((t1 - t0) / 1e9),
         "wall_s": (t1 - t0) / 1e9,

@@ -55,7 +55,7 @@ int emu_shfl_i(int v, int src_lane);
 float emu_shfl_f(float v, int src_lane);
 inline int __shfl(int v, int src, int width = 64) {
     int lane = (int)(threadIdx.x % 64);
-    return emu_shfl_i(v, (lane / width) * width + (src % width));
+    return emu_shfl_i(v, (lane / width) * width + (src & (width - 1)));
 }
 inline int __shfl_xor(int v, int mask, int width = 64) {
     int lane = (int)(threadIdx.x % 64);
@@ -82,6 +82,7 @@ inline long long max(long long a, int b) { return a < b ? b : a; }
 
 // ---- atomics -------------------------------------------------------------------
 template <class T> inline T atomicAdd(T* p, T v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+template <class T> inline T atomicSub(T* p, T v) { return __atomic_fetch_sub(p, v, __ATOMIC_SEQ_CST); }
 template <class T> inline T atomicOr(T* p, T v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
 template <class T> inline T atomicAnd(T* p, T v) { return __atomic_fetch_and(p, v, __ATOMIC_SEQ_CST); }
 template <class T> inline T atomicMax(T* p, T v) {
