@@ -40,6 +40,7 @@ sample of the same queries and map.
 from __future__ import annotations
 
 import argparse
+import faulthandler
 import importlib
 import json
 import os
@@ -57,6 +58,7 @@ HBM_PEAK_GBS = 8000.0
 
 
 def main():
+    faulthandler.enable()  # a native crash prints the Python stack
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -73,6 +75,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="simulated arrival window of the serving phase")
     ap.add_argument("--lanes", type=int, default=0, help="force lanes per query (0: autotuned)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch shape or --lanes)")
+    ap.add_argument("--emu-lib", default=None, help=argparse.SUPPRESS)  # debugging: a CPU emulation build
+    ap.add_argument("--scale", type=float, default=1.0, help=argparse.SUPPRESS)  # debugging: shrink phases C/D
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (10 launches), to calibrate HBM byte counters")
     args = ap.parse_args()
@@ -92,8 +96,17 @@ def main():
     ikd = importlib.import_module("ikd-tree_amd")
     scenes = importlib.import_module("ikd-tree_amd.scenes")
     autotune = importlib.import_module("ikd-tree_amd.autotune")
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
+    global SCALE
+    SCALE = args.scale
+    if args.emu_lib:
+        # host-side debugging only: the same flow against the CPU emulation
+        # build of the library, tensors on the host, wall-clock timing
+        ikd.LIB_PATH = os.path.abspath(args.emu_lib)
+        dev = torch.device("cpu")
+        _emulate_cuda(torch)
+    else:
+        dev = torch.device("cuda:0")
+        torch.cuda.set_device(dev)
 
     scene = scenes.MappingScene(1002)
     k = args.k
@@ -228,14 +241,17 @@ def main():
         "sweep_ms": {f"lanes{a}_stack{b}_order{c}": v for (a, b, c), v in p_sweep.items()},
     }
 
+    progress("phase A done")
     # ================================================ phase B: mapping pipeline
     pipe = pipeline_latency(ikd, torch, dev, scene, args.frames, k, args.max_dist) if args.frames > 1 else None
     del tree
     torch.cuda.empty_cache()
 
+    progress("phase B done")
     # ================================================ phase C: many trees, one launch
     batching = batching_phase(ikd, torch, dev, scenes, args.trees) if args.trees > 0 else None
 
+    progress("phase C done")
     # ================================================ phase D: serving many robots
     serving = serving_phase(ikd, scenes, args.robots, args.serve_seconds) if args.robots > 0 else None
 
@@ -291,10 +307,36 @@ def main():
         "pool": map_pool,
     }
 
+    progress("phase D done")
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(seed_pts, [scans[j] for j in timed_scans][: args.cpu_scans], k,
                                               args.max_dist)
     print(json.dumps(result))
+
+
+SCALE = 1.0
+
+
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _emulate_cuda(torch):
+    """Stand-ins for the few torch.cuda calls of this script on a host-only run."""
+    class _Ev:
+        def __init__(self, enable_timing=True):
+            self.t = 0.0
+
+        def record(self, stream=None):
+            self.t = time.perf_counter()
+
+        def elapsed_time(self, other):
+            return (other.t - self.t) * 1e3
+
+    torch.cuda.Event = _Ev
+    torch.cuda.ExternalStream = lambda s: None
+    torch.cuda.synchronize = lambda *a: None
+    torch.cuda.empty_cache = lambda *a: None
 
 
 def _scene_points(scenes, seed, first_frame, n, pool):
@@ -321,8 +363,8 @@ def batching_phase(ikd, torch, dev, scenes, n_trees, seed=1004):
     from concurrent.futures import ThreadPoolExecutor
 
     rng = np.random.default_rng(seed)
-    sizes = np.exp(rng.uniform(np.log(1e4), np.log(1e6), n_trees)).astype(np.int64)
-    nqs = np.exp(rng.uniform(np.log(1e3), np.log(2e5), n_trees)).astype(np.int64)
+    sizes = (SCALE * np.exp(rng.uniform(np.log(1e4), np.log(1e6), n_trees))).astype(np.int64) + 1
+    nqs = (SCALE * np.exp(rng.uniform(np.log(1e3), np.log(2e5), n_trees))).astype(np.int64) + 1
     trees, qparts = [], []
     with ThreadPoolExecutor(max_workers=16) as pool:
         for t in range(n_trees):
@@ -458,7 +500,7 @@ def serving_phase(ikd, scenes, n_robots, seconds, seed=1005):
 
     serving = importlib.import_module("ikd-tree_amd.serving")
     rng = np.random.default_rng(seed)
-    sizes = rng.uniform(0.5e6, 3e6, n_robots).astype(np.int64)
+    sizes = (SCALE * rng.uniform(0.5e6, 3e6, n_robots)).astype(np.int64) + 1
     scs = [scenes.MappingScene(seed * 100 + r) for r in range(n_robots)]
     trees, first = [], []
     with ThreadPoolExecutor(max_workers=16) as pool:

@@ -61,13 +61,24 @@ struct BestList {
             n[i] = i < k ? -1 : -2;
         }
     }
-    // offer (dist, node); keeps the k smallest in (d2, x, y, z, id) order
+    // offer (dist, node); keeps the k smallest in (d2, x, y, z, id) order.
+    // Without an exact distance tie in the list the ranks are plain float
+    // comparisons (branch-free selects); a tie takes the full comparator.
     __device__ __forceinline__ void offer(const NodeHot* hot, const uint32_t* ids, float dist, int node) {
-        if (!after(hot, ids, d[0], n[0], dist, node)) return;
-        bool gnext[KM];
+        if (dist > d[0]) return;  // strictly worse than the k-th: the common case
+        bool tie = false;
 #pragma unroll
-        for (int i = 0; i < KM; ++i)
-            gnext[i] = (i + 1 < KM) ? after(hot, ids, d[i + 1], n[i + 1], dist, node) : false;
+        for (int i = 0; i < KM; ++i) tie = tie || d[i] == dist;
+        bool gnext[KM];
+        if (!tie) {
+#pragma unroll
+            for (int i = 0; i < KM; ++i) gnext[i] = (i + 1 < KM) ? d[i + 1] > dist : false;
+        } else {
+            if (!after(hot, ids, d[0], n[0], dist, node)) return;
+#pragma unroll
+            for (int i = 0; i < KM; ++i)
+                gnext[i] = (i + 1 < KM) ? after(hot, ids, d[i + 1], n[i + 1], dist, node) : false;
+        }
 #pragma unroll
         for (int i = 0; i < KM; ++i) {
             bool gcur = (i == 0) ? true : gnext[i - 1];

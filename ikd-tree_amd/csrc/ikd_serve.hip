@@ -172,6 +172,9 @@ struct ikd_server_s {
             in_flight_bytes += need;
             ++in_flight;
             lk.unlock();
+            // the caller may release the request as soon as it is handed back:
+            // nothing below reads it after that
+            const int map = r->map;
             r->t_start_ns = now_ns();
             int rc = run(b, r);
             r->t_done_ns = now_ns();
@@ -179,19 +182,20 @@ struct ikd_server_s {
             lk.lock();
             // hand the request back before any maintenance of its map
             done.push_back(r);
+            r = nullptr;
             cv_done.notify_all();
             bool waiting = false;
-            for (ikd_request_t* c : pending) waiting = waiting || c->map == r->map;
+            for (ikd_request_t* c : pending) waiting = waiting || c->map == map;
             if (!waiting && maintain_points != 0) {
                 // the map is idle: run its deferred rebuilds now, off the
                 // requests' latency path (a request arriving meanwhile waits)
                 lk.unlock();
                 int32_t nreb = 0;
-                ikd_rebuild_deferred(maps[r->map], maintain_points, &nreb);
+                ikd_rebuild_deferred(maps[map], maintain_points, &nreb);
                 lk.lock();
                 maintenance_runs += nreb;
             }
-            busy[r->map] = 0;
+            busy[map] = 0;
             in_flight_bytes -= need;
             --in_flight;
             cv_done.notify_all();
