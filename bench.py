@@ -73,6 +73,10 @@ def main():
     ap.add_argument("--trees", type=int, default=64, help="trees in the multi-tree batching phase (0 skips it)")
     ap.add_argument("--robots", type=int, default=16, help="maps in the serving phase (0 skips it)")
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="simulated arrival window of the serving phase")
+    ap.add_argument("--serve-defer", type=int, default=20_000,
+                    help="serving: subtrees above this many live points are rebuilt between requests")
+    ap.add_argument("--serve-chunk", type=int, default=100_000,
+                    help="serving: deferred-rebuild points per maintenance step (-1: all at once)")
     ap.add_argument("--lanes", type=int, default=0, help="force lanes per query (0: autotuned)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch shape or --lanes)")
     ap.add_argument("--emu-lib", default=None, help=argparse.SUPPRESS)  # debugging: a CPU emulation build
@@ -253,7 +257,8 @@ def main():
 
     progress("phase C done")
     # ================================================ phase D: serving many robots
-    serving = serving_phase(ikd, scenes, args.robots, args.serve_seconds) if args.robots > 0 else None
+    serving = (serving_phase(ikd, scenes, args.robots, args.serve_seconds, args.serve_defer, args.serve_chunk)
+               if args.robots > 0 else None)
 
     result = {
         "metric": METRIC,
@@ -491,7 +496,7 @@ def batching_phase(ikd, torch, dev, scenes, n_trees, seed=1004):
     return out
 
 
-def serving_phase(ikd, scenes, n_robots, seconds, seed=1005):
+def serving_phase(ikd, scenes, n_robots, seconds, defer_points, chunk_points, seed=1005):
     """Many robots, each with its own device-resident map (its own mapping
     scene, 0.5-3M points), sending scans (10k-60k points) as Poisson arrivals at
     10 Hz; each request is kNN (k=5, max_dist 5 m) + Add_Points (0.5 m voxels),
@@ -512,8 +517,10 @@ def serving_phase(ikd, scenes, n_robots, seconds, seed=1005):
             first.append(f)
     scan_len = rng.integers(10_000, 60_001, size=(n_robots, 4096))
     srv = serving.Server(slots=8, hbm_budget_bytes=64 << 30)
-    # rebuilds of subtrees above 20k points run between a map's requests
-    ids = [srv.add_map(t, defer_rebuild_points=20_000) for t in trees]
+    # rebuilds of subtrees above defer_points run between a map's requests,
+    # chunk_points at a time
+    ids = [srv.add_map(t, defer_rebuild_points=defer_points) for t in trees]
+    srv.set_maintenance(chunk_points)
 
     # one full scan per robot before the clock starts: every map's scratch
     # buffers reach their working size (allocation is not part of serving)
@@ -540,7 +547,8 @@ def serving_phase(ikd, scenes, n_robots, seconds, seed=1005):
     for t in trees:
         t.close()
     res.update({"robots": n_robots, "map_points_total": int(sizes.sum()), "slots": 8, "hbm_budget_gb": 64,
-                "rate_hz_per_robot": 10.0, "defer_rebuild_points": 20_000})
+                "rate_hz_per_robot": 10.0, "defer_rebuild_points": defer_points,
+                "maintenance_chunk_points": chunk_points})
     return res
 
 
@@ -627,6 +635,7 @@ def pipeline_latency(ikd, torch, dev, scene, frames, k, max_dist):
         "inserted_points_per_s": nq / sum(ins_lat),
         "add_points_returned": int(added),
         "pool": tree.pool_stats(),
+        "counters": tree.counters(),
     }
 
 

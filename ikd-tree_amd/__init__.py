@@ -40,7 +40,7 @@ EXPORTS = (
     "ikd_nearest_search_device", "ikd_nearest_search_counters", "ikd_box_search", "ikd_radius_search",
     "ikd_add_points", "ikd_add_points_device", "ikd_delete_points", "ikd_delete_point_boxes", "ikd_size",
     "ikd_validnum", "ikd_tree_range", "ikd_root_alpha", "ikd_flatten", "ikd_synchronize", "ikd_stream",
-    "ikd_pool_stats", "ikd_set_tuning", "ikd_dump_nodes", "ikd_last_error",
+    "ikd_pool_stats", "ikd_get_counters", "ikd_set_tuning", "ikd_dump_nodes", "ikd_last_error",
     "ikd_batch_create", "ikd_batch_destroy", "ikd_batch_nearest_search_device", "ikd_batch_synchronize",
     "ikd_batch_stream", "ikd_batch_set_tuning", "ikd_batch_last_error", "ikd_batch_radius_search_device",
     "ikd_now_ns", "ikd_server_create", "ikd_server_add_map", "ikd_server_submit", "ikd_server_wait",
@@ -52,6 +52,15 @@ IKD_TUNE_KNN_STACK = 2
 IKD_TUNE_LANES_PER_QUERY = 3
 IKD_TUNE_SMALL_BUILD = 4
 IKD_TUNE_DEFER_REBUILD = 5
+
+
+class Counters(ctypes.Structure):
+    """ikd_counters_t: a tree's event counts since creation."""
+    _fields_ = [(name, ctypes.c_int64) for name in
+                ("rebuilds", "rebuilt_points", "compactions", "pool_grows", "scratch_grows", "host_waits")]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class IkdError(RuntimeError):
@@ -104,6 +113,7 @@ def load_library(path: str | None = None):
         "ikd_synchronize": [vp],
         "ikd_stream": [vp, ctypes.POINTER(vp)],
         "ikd_pool_stats": [vp, vp, vp, vp, vp],
+        "ikd_get_counters": [vp, vp],
         "ikd_set_tuning": [vp, i32, i32],
         "ikd_dump_nodes": [vp, vp, i64, vp],
         "ikd_rebuild_deferred": [vp, i64, vp],
@@ -325,6 +335,11 @@ class KD_TREE:
                                              ctypes.byref(reb)))
         return {"used_nodes": used.value, "capacity_nodes": cap.value, "max_depth": depth.value,
                 "rebuilds": reb.value}
+
+    def counters(self):
+        c = Counters()
+        self._check(self._lib.ikd_get_counters(self._h, ctypes.byref(c)))
+        return c.as_dict()
 
     def set_tuning(self, key: int, value: int):
         self._check(self._lib.ikd_set_tuning(self._h, key, value))

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -164,6 +165,8 @@ struct ikd_tree_s {
     int knn_lanes = -1;   // lanes per query: -1 picks from the batch size; 0 persistent; else 1, 2, 4, 8, 16
     int small_build = 1;  // build subtrees of <= SMALL_MAX points with one workgroup each
     hipEvent_t ev = nullptr;  // orders batch launches against this tree's stream
+    ikd_counters_t ctr{};     // event counts since creation (ikd_get_counters)
+    hipMemPool_t mp = nullptr;  // stream-ordered allocations (null: hipMalloc / hipFree)
     // scratch
     DevBuf b[80];
 };
@@ -178,6 +181,8 @@ struct ikd_batch_s {
     int knn_lanes = -1;
     int knn_stack = 0;
     int* h_flag = nullptr;  // pinned mirror of the overflow flag
+    ikd_counters_t ctr{};   // only scratch_grows and host_waits move
+    hipMemPool_t mp = nullptr;
     DevBuf b[8];
 };
 
@@ -207,17 +212,67 @@ static_assert(B_COUNT <= 80, "scratch table");
         if (r_ != IKD_OK) return r_; \
     } while (0)
 
+// The library's device memory pool on device `dev`: stream-ordered
+// allocation, so growing a buffer never waits for the device (hipFree waits
+// for every stream of the device, which stalls every other tree's work
+// queued behind it).  Freed memory stays in the pool for reuse.  Null when
+// the runtime offers no memory pools.
+hipMemPool_t library_pool(int dev) {
+    static std::mutex mu;
+    static hipMemPool_t pools[64];
+    static bool tried[64];
+    if (dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> g(mu);
+    if (!tried[dev]) {
+        tried[dev] = true;
+        hipMemPoolProps props{};
+        props.allocType = hipMemAllocationTypePinned;
+        props.handleTypes = hipMemHandleTypeNone;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        hipMemPool_t mp = nullptr;
+        if (hipMemPoolCreate(&mp, &props) == hipSuccess) {
+            uint64_t keep = UINT64_MAX;
+            if (hipMemPoolSetAttribute(mp, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess) {
+                pools[dev] = mp;
+            } else {
+                hipMemPoolDestroy(mp);
+            }
+        }
+        (void)hipGetLastError();
+    }
+    return pools[dev];
+}
+
+// Device memory ordered on the owner's stream: usable by work queued after
+// the call on that stream (other streams order themselves with events).
+template <class Owner>
+hipError_t dev_alloc(Owner* t, void** p, size_t bytes) {
+    if (t->mp) return hipMallocFromPoolAsync(p, bytes, t->mp, t->st);
+    return hipMalloc(p, bytes);
+}
+// Frees after the work already queued on the owner's stream.
+template <class Owner>
+void dev_free(Owner* t, void* p) {
+    if (!p) return;
+    if (t->mp) {
+        hipFreeAsync(p, t->st);
+        return;
+    }
+    hipStreamSynchronize(t->st);  // queued kernels may still read the buffer
+    hipFree(p);
+}
+
 template <class T, class Owner>
 int ensure(Owner* t, int slot, long long count, T** out) {
     size_t need = (size_t)std::max<long long>(count, 1) * sizeof(T);
     DevBuf& d = t->b[slot];
     if (d.bytes < need) {
-        if (d.p) {
-            hipStreamSynchronize(t->st);  // queued kernels may still read the old buffer
-            hipFree(d.p);
-        }
-        size_t nb = std::max(need, d.bytes + d.bytes / 2);
-        if (hipMalloc(&d.p, nb) != hipSuccess) {
+        dev_free(t, d.p);
+        d.p = nullptr;
+        size_t nb = std::max(need, 2 * d.bytes);
+        t->ctr.scratch_grows += 1;
+        if (dev_alloc(t, &d.p, nb) != hipSuccess) {
             d.p = nullptr;
             d.bytes = 0;
             t->err = "device allocation of scratch failed";
@@ -229,9 +284,16 @@ int ensure(Owner* t, int slot, long long count, T** out) {
     return IKD_OK;
 }
 
+// Host waits for the tree's stream; counted because each one is a round trip
+// on the latency path of an update.
+hipError_t wait_stream(ikd_tree_t t) {
+    t->ctr.host_waits += 1;
+    return hipStreamSynchronize(t->st);
+}
+
 int sync_hdr(ikd_tree_t t) {
     HIPCHK(hipMemcpyAsync(t->h_hdr, t->d_hdr, sizeof(DevHdr), hipMemcpyDeviceToHost, t->st));
-    HIPCHK(hipStreamSynchronize(t->st));
+    HIPCHK(wait_stream(t));
     t->pool_used = 2ll * t->h_hdr->pair_used;
     t->max_depth = t->h_hdr->max_depth;
     if (t->h_hdr->overflow) {
@@ -244,7 +306,7 @@ int sync_hdr(ikd_tree_t t) {
 int read_root(ikd_tree_t t) {
     hipLaunchKernelGGL(k_root_info, dim3(1), dim3(1), 0, t->st, t->pool, t->d_root);
     HIPCHK(hipMemcpyAsync(t->h_root, t->d_root, sizeof(RootInfo), hipMemcpyDeviceToHost, t->st));
-    HIPCHK(hipStreamSynchronize(t->st));
+    HIPCHK(wait_stream(t));
     return IKD_OK;
 }
 bool has_root(ikd_tree_t t) { return (t->h_root->child.z & (int)LK_HASL) != 0; }
@@ -313,20 +375,19 @@ int exscan_i64(Owner* t, const long long* in, long long* out, int n) {
 
 // ------------------------------------------------------------- node pool
 int alloc_pool(ikd_tree_t t, long long cap, Pool* p) {
-    if (hipMalloc(&p->hot, cap * sizeof(NodeHot)) != hipSuccess) return IKD_ERR_NOMEM;
-    if (hipMalloc(&p->ids, cap * sizeof(uint32_t)) != hipSuccess) return IKD_ERR_NOMEM;
-    if (hipMalloc(&p->par, cap * sizeof(int32_t)) != hipSuccess) return IKD_ERR_NOMEM;
-    if (hipMalloc(&p->cnt, cap * sizeof(int4)) != hipSuccess) return IKD_ERR_NOMEM;
-    if (hipMalloc(&p->meta, cap * sizeof(uint32_t)) != hipSuccess) return IKD_ERR_NOMEM;
-    (void)t;
+    if (dev_alloc(t, (void**)&p->hot, cap * sizeof(NodeHot)) != hipSuccess) return IKD_ERR_NOMEM;
+    if (dev_alloc(t, (void**)&p->ids, cap * sizeof(uint32_t)) != hipSuccess) return IKD_ERR_NOMEM;
+    if (dev_alloc(t, (void**)&p->par, cap * sizeof(int32_t)) != hipSuccess) return IKD_ERR_NOMEM;
+    if (dev_alloc(t, (void**)&p->cnt, cap * sizeof(int4)) != hipSuccess) return IKD_ERR_NOMEM;
+    if (dev_alloc(t, (void**)&p->meta, cap * sizeof(uint32_t)) != hipSuccess) return IKD_ERR_NOMEM;
     return IKD_OK;
 }
-void free_pool(Pool* p) {
-    hipFree(p->hot);
-    hipFree(p->ids);
-    hipFree(p->par);
-    hipFree(p->cnt);
-    hipFree(p->meta);
+void free_pool(ikd_tree_t t, Pool* p) {
+    dev_free(t, p->hot);
+    dev_free(t, p->ids);
+    dev_free(t, p->par);
+    dev_free(t, p->cnt);
+    dev_free(t, p->meta);
     *p = Pool{};
 }
 
@@ -340,9 +401,11 @@ int reserve_nodes(ikd_tree_t t, long long extra) {
     }
     long long ncap = std::max(need + need / 2, 2 * t->cap);
     ncap = std::min(ncap, 1ll << 28);
+    if (t->cap > 0) t->ctr.pool_grows += 1;
     Pool np{};
     if (alloc_pool(t, ncap, &np) != IKD_OK) {
-        free_pool(&np);
+        (void)hipGetLastError();
+        free_pool(t, &np);
         t->err = "node pool allocation failed";
         return IKD_ERR_NOMEM;
     }
@@ -353,8 +416,7 @@ int reserve_nodes(ikd_tree_t t, long long extra) {
         HIPCHK(hipMemcpyAsync(np.par, t->pool.par, used * sizeof(int32_t), hipMemcpyDeviceToDevice, t->st));
         HIPCHK(hipMemcpyAsync(np.cnt, t->pool.cnt, used * sizeof(int4), hipMemcpyDeviceToDevice, t->st));
         HIPCHK(hipMemcpyAsync(np.meta, t->pool.meta, used * sizeof(uint32_t), hipMemcpyDeviceToDevice, t->st));
-        HIPCHK(hipStreamSynchronize(t->st));
-        free_pool(&t->pool);
+        free_pool(t, &t->pool);  // after the copies
     }
     t->pool = np;
     t->cap = ncap;
@@ -434,7 +496,7 @@ int run_build(ikd_tree_t t, const float4* pts, const int* seg_of, int m, const i
     HIPCHK(hipMemcpyAsync(segd[0], d_seg_depth, S * sizeof(int), hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(&t->d_hdr->seg_count[0], &S, sizeof(int), hipMemcpyHostToDevice, st));
     // a pageable host source: make sure it is consumed before S goes away
-    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(wait_stream(t));
 
     int levels = bits_for(maxlen);  // floor(log2(maxlen)) + 1
     // A segment of x points leaves children of at most x/2: from level Ls on
@@ -615,6 +677,7 @@ int rebuild_listed(ikd_tree_t t, const std::vector<int4>& hinfo, const std::vect
         roots.push_back(u);
         total_nodes += hbefore[i];
         total_live += live;
+        t->ctr.rebuilt_points += live;
         maxlen = std::max(maxlen, live);
     }
     if (!roots.empty()) {
@@ -653,7 +716,7 @@ int rebuild_tops(ikd_tree_t t) {
         std::vector<int> hbefore(ntop);
         HIPCHK(hipMemcpyAsync(hinfo.data(), info, ntop * sizeof(int4), hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(hbefore.data(), before, ntop * sizeof(int), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(wait_stream(t));
         CHK(rebuild_listed(t, hinfo, hbefore));
         t->rebuilds += ntop;
         // The refit already handed every ancestor the post-rebuild counts, so
@@ -673,6 +736,7 @@ int compact_if_needed(ikd_tree_t t, bool force) {
     bool deep = t->max_depth + 2 > MAX_DEPTH;
     bool sparse = t->pool_used > 4 * live + (1ll << 22);
     if (!force && !deep && !sparse) return IKD_OK;
+    t->ctr.compactions += 1;
     if (!has_root(t)) return reset_pool(t);
     float4* pts;
     int* segof;
@@ -878,7 +942,7 @@ int add_points_ds_chunk(ikd_tree_t t, const float* d_xyz, const uint32_t* d_ids,
     launch_compact_pts(insfl, scan, inspt, nv, t->d_hdr, cpts, counts + 1, st);
     int hc = 0;
     HIPCHK(hipMemcpyAsync(&hc, counts + 1, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(wait_stream(t));
     return insert_points(t, cpts, hc, dseeds, nullptr, ndel);
 }
 
@@ -1076,6 +1140,12 @@ int ikd_create(float delete_param, float balance_param, float box_length, ikd_tr
     *out = nullptr;
     ikd_tree_t t = new ikd_tree_s();
     t->prm = Params{delete_param, balance_param, box_length};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        delete t;
+        return IKD_ERR_HIP;
+    }
+    t->mp = library_pool(dev);
     if (hipStreamCreateWithFlags(&t->st, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&t->d_hdr, sizeof(DevHdr)) != hipSuccess ||
         hipHostMalloc(&t->h_hdr, sizeof(DevHdr), hipHostMallocDefault) != hipSuccess ||
@@ -1098,10 +1168,11 @@ int ikd_create(float delete_param, float balance_param, float box_length, ikd_tr
 
 int ikd_destroy(ikd_tree_t t) {
     CHECK_T();
-    if (t->st) hipStreamSynchronize(t->st);
-    if (t->cap > 0) free_pool(&t->pool);
-    for (auto& b : t->b)
-        if (b.p) hipFree(b.p);
+    if (t->st) {
+        if (t->cap > 0) free_pool(t, &t->pool);
+        for (auto& b : t->b) dev_free(t, b.p);
+        hipStreamSynchronize(t->st);
+    }
     if (t->d_hdr) hipFree(t->d_hdr);
     if (t->h_hdr) hipHostFree(t->h_hdr);
     if (t->d_root) hipFree(t->d_root);
@@ -1155,7 +1226,7 @@ int ikd_build(ikd_tree_t t, const float* xyz, const uint32_t* ids, int64_t n) {
         int* dsegd;
         CHK(upload(t, B_ISEG, &s0, 1, &dseg));
         CHK(upload(t, B_ISEGD, &d0, 1, &dsegd));
-        HIPCHK(hipStreamSynchronize(t->st));
+        HIPCHK(wait_stream(t));
         CHK(run_build(t, pts, segof, (int)n, dseg, dsegd, 1, (int)n));
         hipLaunchKernelGGL(k_link_bits, dim3(1), dim3(1), 0, t->st, t->pool.hot, SENTINEL, LK_HASL, LK_TDL);
     }
@@ -1274,7 +1345,7 @@ int ikd_add_points(ikd_tree_t t, const float* xyz, const uint32_t* ids, int64_t 
     uint32_t* di = nullptr;
     CHK(upload(t, B_ADDX, xyz, 3 * n, &dx));
     if (ids) CHK(upload(t, B_ADDI, ids, n, &di));
-    HIPCHK(hipStreamSynchronize(t->st));
+    HIPCHK(wait_stream(t));
     return ikd_add_points_device(t, dx, di, n, downsample_on, out_added);
 }
 
@@ -1489,7 +1560,7 @@ int ikd_rebuild_deferred(ikd_tree_t t, int64_t max_points, int32_t* out_rebuilt)
     HIPCHK(hipMemcpyAsync(hkeep.data(), keep, n * sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(hinf.data(), info, n * sizeof(int4), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(hbef.data(), before, n * sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(wait_stream(t));
     // take due entries in list order while the point budget lasts (at least one)
     std::vector<int4> take;
     std::vector<int> take_before, rest;
@@ -1512,7 +1583,7 @@ int ikd_rebuild_deferred(ikd_tree_t t, int64_t max_points, int32_t* out_rebuilt)
         hipLaunchKernelGGL(k_link_defer_flags, dim3(nblk(cnt)), dim3(BS), 0, st, t->pool, defer, cnt);
     }
     HIPCHK(hipMemcpyAsync(&t->d_hdr->defer_count, &cnt, sizeof(int), hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(wait_stream(t));
     if (take.empty()) return IKD_OK;
     CHK(rebuild_listed(t, take, take_before));
     t->rebuilds += (long long)take.size();
@@ -1568,6 +1639,14 @@ int ikd_pool_stats(ikd_tree_t t, int64_t* used, int64_t* capacity, int32_t* max_
     return IKD_OK;
 }
 
+int ikd_get_counters(ikd_tree_t t, ikd_counters_t* out) {
+    CHECK_T();
+    if (!out) return IKD_ERR_ARG;
+    *out = t->ctr;
+    out->rebuilds = t->rebuilds;
+    return IKD_OK;
+}
+
 const char* ikd_last_error(ikd_tree_t t) {
     if (!t) return "null tree handle";
     return t->err.c_str();
@@ -1616,6 +1695,12 @@ int ikd_batch_create(ikd_batch_t* out) {
     if (!out) return IKD_ERR_ARG;
     *out = nullptr;
     ikd_batch_t b = new ikd_batch_s();
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        delete b;
+        return IKD_ERR_HIP;
+    }
+    b->mp = library_pool(dev);
     if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&b->ev, hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc(&b->h_flag, sizeof(int), hipHostMallocDefault) != hipSuccess) {
@@ -1638,9 +1723,10 @@ int ikd_batch_create(ikd_batch_t* out) {
 
 int ikd_batch_destroy(ikd_batch_t b) {
     if (!b) return IKD_ERR_ARG;
-    if (b->st) hipStreamSynchronize(b->st);
-    for (auto& d : b->b)
-        if (d.p) hipFree(d.p);
+    if (b->st) {
+        for (auto& d : b->b) dev_free(b, d.p);
+        hipStreamSynchronize(b->st);
+    }
     if (b->h_flag) hipHostFree(b->h_flag);
     if (b->ev) hipEventDestroy(b->ev);
     if (b->st) hipStreamDestroy(b->st);

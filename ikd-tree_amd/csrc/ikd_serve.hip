@@ -137,6 +137,7 @@ struct ikd_server_s {
             if (rc != IKD_OK) return rc;
             r->added = added;
         }
+        r->t_insert_done_ns = now_ns();
         if (r->n_trim > 0) {
             int32_t del = 0;
             rc = ikd_delete_point_boxes(t, r->trim_boxes, r->n_trim, &del);
@@ -176,24 +177,37 @@ struct ikd_server_s {
             // nothing below reads it after that
             const int map = r->map;
             r->t_start_ns = now_ns();
+            ikd_counters_t c0{}, c1{};
+            ikd_get_counters(maps[map], &c0);
             int rc = run(b, r);
+            ikd_get_counters(maps[map], &c1);
             r->t_done_ns = now_ns();
+            r->delta.rebuilds = c1.rebuilds - c0.rebuilds;
+            r->delta.rebuilt_points = c1.rebuilt_points - c0.rebuilt_points;
+            r->delta.compactions = c1.compactions - c0.compactions;
+            r->delta.pool_grows = c1.pool_grows - c0.pool_grows;
+            r->delta.scratch_grows = c1.scratch_grows - c0.scratch_grows;
+            r->delta.host_waits = c1.host_waits - c0.host_waits;
             if (rc != IKD_OK && r->status == IKD_OK) r->status = rc;
             lk.lock();
             // hand the request back before any maintenance of its map
             done.push_back(r);
             r = nullptr;
             cv_done.notify_all();
-            bool waiting = false;
-            for (ikd_request_t* c : pending) waiting = waiting || c->map == map;
-            if (!waiting && maintain_points != 0) {
-                // the map is idle: run its deferred rebuilds now, off the
-                // requests' latency path (a request arriving meanwhile waits)
+            // While the map is idle, run its deferred rebuilds off the
+            // requests' latency path, max_points at a time, and give the map
+            // back as soon as a request for it arrives (it waits at most one
+            // chunk).
+            while (maintain_points != 0) {
+                bool waiting = false;
+                for (ikd_request_t* c : pending) waiting = waiting || c->map == map;
+                if (waiting) break;
                 lk.unlock();
                 int32_t nreb = 0;
-                ikd_rebuild_deferred(maps[map], maintain_points, &nreb);
+                int mrc = ikd_rebuild_deferred(maps[map], maintain_points, &nreb);
                 lk.lock();
                 maintenance_runs += nreb;
+                if (mrc != IKD_OK || nreb == 0 || maintain_points < 0) break;
             }
             busy[map] = 0;
             in_flight_bytes -= need;

@@ -32,6 +32,7 @@ class Request(ctypes.Structure):
         ("out_counts", ctypes.c_void_p), ("t_arrival_ns", ctypes.c_int64), ("tag", ctypes.c_int64),
         ("t_start_ns", ctypes.c_int64), ("t_done_ns", ctypes.c_int64),
         ("added", ctypes.c_int32), ("trimmed", ctypes.c_int32), ("status", ctypes.c_int32),
+        ("t_insert_done_ns", ctypes.c_int64), ("delta", _pkg.Counters),
     ]
 
 
@@ -231,7 +232,16 @@ def simulate(server: Server, map_ids, scan_fn, rate_hz: float = 10.0, duration_s
         "run_ms": pct(run),
         "slowest": [{"robot": int(d.req.tag), "points": int(d.req.n), "trim": int(d.req.n_trim > 0),
                      "wait_ms": (d.req.t_start_ns - d.req.t_arrival_ns) / 1e6,
-                     "run_ms": (d.req.t_done_ns - d.req.t_start_ns) / 1e6} for d in slow],
+                     "run_ms": (d.req.t_done_ns - d.req.t_start_ns) / 1e6,
+                     "insert_done_ms": (d.req.t_insert_done_ns - d.req.t_start_ns) / 1e6,
+                     "events": d.req.delta.as_dict()} for d in slow],
+        "events": {
+            "requests_with_compaction": sum(1 for d in done if d.req.delta.compactions > 0),
+            "requests_with_pool_grow": sum(1 for d in done if d.req.delta.pool_grows > 0),
+            "requests_with_scratch_grow": sum(1 for d in done if d.req.delta.scratch_grows > 0),
+            "rebuilt_points_per_request": float(np.mean([d.req.delta.rebuilt_points for d in done])),
+            "host_waits_per_request": float(np.mean([d.req.delta.host_waits for d in done])),
+        },
         "scan_points_per_s": points / ((t1 - t0) / 1e9),
         "requests_per_s": len(done) / ((t1 - t0) / 1e9),
         "wall_s": (t1 - t0) / 1e9,

@@ -42,6 +42,17 @@ extern "C" {
 
 typedef struct ikd_tree_s* ikd_tree_t;
 
+/* Event counts of one tree since ikd_create, kept on the host (no
+ * counterpart in the reference; reading them waits for nothing). */
+typedef struct ikd_counters {
+    int64_t rebuilds;       /* subtrees rebuilt (Rebuild, ikd_Tree.cpp:500-533), deferred ones included */
+    int64_t rebuilt_points; /* live points those rebuilds re-sorted */
+    int64_t compactions;    /* whole-tree rebuilds: pool mostly garbage, or the tree deeper than 62 levels */
+    int64_t pool_grows;     /* node-pool reallocations */
+    int64_t scratch_grows;  /* scratch-buffer allocations and reallocations */
+    int64_t host_waits;     /* host waits on the tree's stream inside the library's calls */
+} ikd_counters_t;
+
 /* KD_TREE(float delete_param=0.5, float balance_param=0.6, float box_length=0.2)
  * ikd_Tree.h:226, ikd_Tree.cpp:9-17.  Uses the current HIP device. */
 int ikd_create(float delete_param, float balance_param, float box_length, ikd_tree_t* out);
@@ -134,6 +145,8 @@ int ikd_stream(ikd_tree_t t, void** out_stream);
  * capacity, deepest node, rebuilds run so far. */
 int ikd_pool_stats(ikd_tree_t t, int64_t* used_nodes, int64_t* capacity_nodes, int32_t* max_depth,
                    int64_t* rebuilds);
+/* Diagnostics (no counterpart in the reference): the tree's event counts. */
+int ikd_get_counters(ikd_tree_t t, ikd_counters_t* out);
 
 /* Launch tuning (no counterpart in the reference; results never depend on it):
  *  IKD_TUNE_QUERY_ORDER 1 sorts batches of >= 4096 kNN queries into
@@ -247,6 +260,8 @@ typedef struct ikd_request {
     /* set by the server */
     int64_t t_start_ns, t_done_ns;
     int32_t added, trimmed, status;
+    int64_t t_insert_done_ns; /* when Add_Points (and the kNN before it) had finished */
+    ikd_counters_t delta;     /* the map's counters over this request */
 } ikd_request_t;
 
 /* Monotonic clock of the server (ns). */
@@ -260,9 +275,11 @@ int ikd_server_submit(ikd_server_t s, ikd_request_t* req);
  * until nothing is pending or running; 0: do not wait). */
 int ikd_server_wait(ikd_server_t s, ikd_request_t** out, int32_t max_out, int32_t timeout_ms, int32_t* out_n);
 int ikd_server_pending(ikd_server_t s, int64_t* out_pending, int64_t* out_in_flight);
-/* After a request, when no other request for the same map is pending, the
- * worker runs ikd_rebuild_deferred(map, max_points) before taking new work
- * (-1, the default: no limit; 0: never).  Deferral itself is per tree
+/* After a request, while no other request for the same map is pending, the
+ * worker runs ikd_rebuild_deferred(map, max_points) repeatedly until the
+ * map's deferred list is empty, then takes new work; a request for the map
+ * that arrives meanwhile waits for at most one such chunk (-1, the default:
+ * one call with no limit; 0: never).  Deferral itself is per tree
  * (IKD_TUNE_DEFER_REBUILD). */
 int ikd_server_set_maintenance(ikd_server_t s, int64_t max_points);
 /* Deferred rebuilds the workers have run so far. */

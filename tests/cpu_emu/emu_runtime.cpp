@@ -234,6 +234,22 @@ hipError_t hipFree(void* p) {
     std::free(p);
     return hipSuccess;
 }
+struct emu_pool { int dummy; };
+hipError_t hipMemPoolCreate(hipMemPool_t* mp, const hipMemPoolProps*) {
+    *mp = new emu_pool{0};
+    return hipSuccess;
+}
+hipError_t hipMemPoolDestroy(hipMemPool_t mp) {
+    delete mp;
+    return hipSuccess;
+}
+hipError_t hipMemPoolSetAttribute(hipMemPool_t, hipMemPoolAttr, void*) { return hipSuccess; }
+hipError_t hipMallocFromPoolAsync(void** p, size_t n, hipMemPool_t, hipStream_t) { return hipMalloc(p, n); }
+hipError_t hipFreeAsync(void* p, hipStream_t) { return hipFree(p); }
+hipError_t hipGetDevice(int* dev) {
+    *dev = 0;
+    return hipSuccess;
+}
 hipError_t hipHostMalloc(void** p, size_t n, unsigned) { return hipMalloc(p, n); }
 hipError_t hipHostFree(void* p) { return hipFree(p); }
 hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) {

@@ -118,6 +118,28 @@ hipError_t hipFree(void* p);
 hipError_t hipHostMalloc(void** p, size_t n, unsigned flags);
 template <class T> hipError_t hipHostMalloc(T** p, size_t n, unsigned flags) { return hipHostMalloc((void**)p, n, flags); }
 hipError_t hipHostFree(void* p);
+// stream-ordered pools: streams run synchronously here, so these are malloc/free
+struct emu_pool;
+typedef emu_pool* hipMemPool_t;
+enum hipMemAllocationType { hipMemAllocationTypePinned = 1 };
+enum hipMemAllocationHandleType { hipMemHandleTypeNone = 0 };
+enum hipMemLocationType { hipMemLocationTypeDevice = 1 };
+struct hipMemLocation { hipMemLocationType type; int id; };
+struct hipMemPoolProps {
+    hipMemAllocationType allocType;
+    hipMemAllocationHandleType handleTypes;
+    hipMemLocation location;
+    void* win32SecurityAttributes;
+    size_t maxSize;
+    unsigned char reserved[56];
+};
+enum hipMemPoolAttr { hipMemPoolAttrReleaseThreshold = 4 };
+hipError_t hipMemPoolCreate(hipMemPool_t* mp, const hipMemPoolProps* props);
+hipError_t hipMemPoolDestroy(hipMemPool_t mp);
+hipError_t hipMemPoolSetAttribute(hipMemPool_t mp, hipMemPoolAttr attr, void* value);
+hipError_t hipMallocFromPoolAsync(void** p, size_t n, hipMemPool_t mp, hipStream_t st);
+hipError_t hipFreeAsync(void* p, hipStream_t st);
+hipError_t hipGetDevice(int* dev);
 hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind k);
 hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t st);
 hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t st);
