@@ -52,6 +52,7 @@ IKD_TUNE_KNN_STACK = 2
 IKD_TUNE_LANES_PER_QUERY = 3
 IKD_TUNE_SMALL_BUILD = 4
 IKD_TUNE_DEFER_REBUILD = 5
+IKD_TUNE_FOLD_LANES = 6
 
 
 class Counters(ctypes.Structure):

@@ -164,6 +164,7 @@ struct ikd_tree_s {
     int knn_stack = 0;    // minimum LDS stack entries per lane (0: the tree depth + 2)
     int knn_lanes = -1;   // lanes per query: -1 picks from the batch size; 0 persistent; else 1, 2, 4, 8, 16
     int small_build = 1;  // build subtrees of <= SMALL_MAX points with one workgroup each
+    int fold_lanes = 0;   // lanes per voxel in the downsample fold: 0 picks by map size; else 1, 4, 8, 16
     hipEvent_t ev = nullptr;  // orders batch launches against this tree's stream
     ikd_counters_t ctr{};     // event counts since creation (ikd_get_counters)
     hipMemPool_t mp = nullptr;  // stream-ordered allocations (null: hipMalloc / hipFree)
@@ -912,12 +913,22 @@ int add_points_ds_chunk(ikd_tree_t t, const float* d_xyz, const uint32_t* d_ids,
     // With conflict checks on, a first pass only decides whether the batch is
     // independent; the deleting pass runs only when it is.
     const int passes = (check && apply) ? 2 : 1;
+    // lanes per voxel: teams split the walks of dense voxels (maps of raw
+    // scans hold up to thousands of points per voxel); the team deques index
+    // their LDS columns with a power-of-two mask
+    const int fold_lanes = t->fold_lanes > 0 ? t->fold_lanes : (t->pool_used >= (1 << 16) ? 8 : 1);
+    int fold_stack = std::min(t->max_depth + 2, MAX_DEPTH);
+    if (fold_lanes > 1) {
+        int p2 = 1;
+        while (p2 < fold_stack) p2 <<= 1;
+        fold_stack = p2;
+    }
     for (int pass = 0; pass < passes; ++pass) {
         const bool deleting = apply && pass == passes - 1;
         hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, t->d_hdr, 2 | 8);
         launch_voxel_fold2(t->pool, pts, vb, kb, vstart, n, n, t->prm.box_length, check, delbox, delfl, inspt, insfl,
-                           t->d_hdr, t->pool_used + 16, deleting ? dseeds : nullptr, (int)dcap,
-                           std::min(t->max_depth + 2, MAX_DEPTH), st);
+                           t->d_hdr, t->pool_used + 16, deleting ? dseeds : nullptr, (int)dcap, fold_stack, fold_lanes,
+                           st);
         int hbad = 0;
         HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
         CHK(sync_hdr(t));
@@ -1623,6 +1634,10 @@ int ikd_set_tuning(ikd_tree_t t, int key, int value) {
         case IKD_TUNE_LANES_PER_QUERY:
             if (value < -1 || value > 16 || (value > 0 && (value & (value - 1)))) return IKD_ERR_ARG;
             t->knn_lanes = value;
+            return IKD_OK;
+        case IKD_TUNE_FOLD_LANES:
+            if (!(value == 0 || value == 1 || value == 4 || value == 8 || value == 16)) return IKD_ERR_ARG;
+            t->fold_lanes = value;
             return IKD_OK;
         default:
             return IKD_ERR_ARG;

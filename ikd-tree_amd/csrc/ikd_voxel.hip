@@ -277,6 +277,258 @@ __global__ void k_voxel_fold(Pool pool, const float4* __restrict__ pts, const ui
     }
 }
 
+// ---------------------------------------------------------------- team fold
+// The same fold with T lanes per voxel.  Maps built from raw scans hold
+// hundreds to thousands of points in one voxel near the trajectory, and one
+// lane walking all of them sets the launch time; a team splits the walk.
+// Every lane runs its own depth-first walk over a deque in LDS; a lane that
+// runs dry takes the shallowest entry of a busy teammate.  The walk order is
+// no longer pre-order, so the tie on distance to the centre (the reference
+// keeps the first point of Search_by_range's pre-order, own-left-right) is
+// broken by each node's pre-order rank, read off its path from the root.
+
+// Pre-order position of node u: the sides (left 0, right 1) of its path from
+// the root as bits from the top, and its depth.  Node a precedes node b in
+// pre-order iff (path_a, depth_a) < (path_b, depth_b) lexicographically: an
+// ancestor's path is a prefix (aligned, no larger; ties go to the shallower),
+// and below the first differing side the left subtree comes first.
+__device__ void preorder_pos(const Pool& pool, int u, unsigned long long& path, int& depth) {
+    unsigned long long bits = 0;
+    int d = 0;
+    for (int v = u; v != ROOT && v > SENTINEL + 1 && d < 64; v = pool.par[v]) {
+        bits = (bits >> 1) | ((unsigned long long)(v & 1) << 63);
+        ++d;
+    }
+    path = bits;
+    depth = d;
+}
+__device__ bool preorder_less(const Pool& pool, int a, int b) {
+    unsigned long long pa, pb;
+    int da, db;
+    preorder_pos(pool, a, pa, da);
+    preorder_pos(pool, b, pb, db);
+    return pa < pb || (pa == pb && da < db);
+}
+
+__device__ __forceinline__ void team_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Visits every node of the tree whose point is live and inside box, once per
+// team, calling visit(node, record) on the lane that reaches it.  Every lane
+// of the wavefront calls this (active marks the teams with work); stk is the
+// lane's column of a [cap][64] LDS array, cap a power of two.
+template <int T, class Visit>
+__device__ void team_box_walk(const Pool& pool, const float* box, bool active, int* lds, int cap, int (*dn)[T],
+                              long long max_steps, DevHdr* hdr, Visit visit) {
+    const int lane = threadIdx.x;
+    const int team = lane / T;
+    const unsigned long long team_mask = (T == 64) ? ~0ull : (((1ull << T) - 1ull) << (team * T));
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int mask = cap - 1;
+    int head = 0, tail = 0;  // the lane's deque [head, tail)
+    int n = -1;
+    if (active && lane % T == 0) {
+        const uint32_t lk0 = fbits(pool.hot[SENTINEL].f[3]);
+        if ((lk0 & LK_HASL) && !(lk0 & LK_TDL) && !box_disjoint(box, pool.hot[SENTINEL].f + 4)) n = ROOT;
+    }
+    long long iter = 0;
+    while (true) {
+        if (n < 0 && tail != head) {
+            --tail;
+            n = lds[(tail & mask) * 64 + lane];
+        }
+        const unsigned long long imask = __ballot(active && n < 0) & team_mask;
+        const unsigned long long dmask = __ballot(tail != head) & team_mask;
+        // the fences run on every lane whenever any team hands work over, so
+        // the wave barriers are never inside team-divergent control flow
+        const bool donate = imask != 0 && dmask != 0;
+        if (__ballot(donate) != 0) {
+            const int ni = __popcll(imask), nd = __popcll(dmask);
+            if (donate && tail != head) {
+                const int r = __popcll(dmask & below);
+                if (r < ni) {
+                    dn[team][r] = lds[(head & mask) * 64 + lane];
+                    ++head;
+                }
+            }
+            team_lds_fence();
+            if (donate && active && n < 0) {
+                const int r = __popcll(imask & below);
+                if (r < nd) n = dn[team][r];
+            }
+            team_lds_fence();
+        }
+        if (__ballot(n >= 0 || tail != head) == 0) break;
+        if (++iter > max_steps) {
+            if (lane == 0) atomicOr(&hdr->overflow, 4);
+            break;
+        }
+        if (n >= 0) {
+            float f[16];
+            load_record(pool.hot[n].f, f);
+            const uint32_t lk = fbits(f[3]);
+            if (!(lk & LK_PDEL) && box_contains(box, f[0], f[1], f[2])) visit(n, f);
+            const int pair = (int)lk_pair(lk);
+            int next = -1;
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                if (!(lk & lk_has(s)) || (lk & lk_td(s))) continue;
+                if (box_disjoint(box, f + 4 + 6 * s)) continue;
+                if (next < 0) {
+                    next = 2 * pair + s;
+                } else if (tail - head < cap) {
+                    lds[(tail & mask) * 64 + lane] = 2 * pair + s;
+                    ++tail;
+                } else {
+                    atomicOr(&hdr->overflow, 4);
+                }
+            }
+            n = next;
+        }
+    }
+}
+
+template <int T>
+__global__ __launch_bounds__(64) void k_voxel_fold_team(Pool pool, const float4* __restrict__ pts,
+                                                        const uint32_t* __restrict__ order,
+                                                        const unsigned long long* __restrict__ keys,
+                                                        const int* __restrict__ vstart, int n, int bound, float ds,
+                                                        int check, float* __restrict__ del_boxes,
+                                                        int* __restrict__ del_flag, float4* __restrict__ ins_pts,
+                                                        int* __restrict__ ins_flag, DevHdr* hdr, long long max_steps,
+                                                        int* __restrict__ del_out, int del_cap, int stack_cap) {
+    constexpr int TEAMS = 64 / T;
+    extern __shared__ int lds_dyn[];  // per-lane deques, [stack_cap][64]
+    __shared__ int dn[TEAMS][T];
+    const int lane = threadIdx.x;
+    const int team = lane / T, tl = lane % T;
+    const int v = blockIdx.x * TEAMS + team;
+    const bool valid = v < bound && v < hdr->group_count;
+    const int vv = valid ? v : 0;
+    const int j0 = valid ? vstart[vv] : 0, j1 = valid ? vstart[vv + 1] : 0;
+    const float4 P0 = pts[order[valid ? j0 : 0]];
+    VoxGeo g;
+    g.ds = ds;
+    g.fidx[0] = floorf(P0.x / ds);
+    g.fidx[1] = floorf(P0.y / ds);
+    g.fidx[2] = floorf(P0.z / ds);
+    float box[6];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        box[a] = g.fidx[a] * ds;
+        box[3 + a] = box[a] + ds;
+    }
+    const float mx = (float)((double)box[0] + (double)(box[3] - box[0]) / 2.0);
+    const float my = (float)((double)box[1] + (double)(box[4] - box[1]) / 2.0);
+    const float mz = (float)((double)box[2] + (double)(box[5] - box[2]) / 2.0);
+    const int first_idx = (int)order[valid ? j0 : 0];
+
+    // ---- S0: live map points in the box; the nearest to the centre, first in pre-order on ties
+    int n0 = 0, best_s = -1, split = 0x7fffffff;
+    float best_sd = INFINITY;
+    team_box_walk<T>(pool, box, valid, lds_dyn, stack_cap, dn, max_steps, hdr, [&](int u, const float* f) {
+        ++n0;
+        const float d = calc_dist(f[0], f[1], f[2], mx, my, mz);
+        if (best_s < 0 || d < best_sd || (d == best_sd && preorder_less(pool, u, best_s))) {
+            best_sd = d;
+            best_s = u;
+        }
+        if (check) {
+            const int other = foreign_voxel_first_index(g, f[0], f[1], f[2], keys, order, n);
+            if (other != 0x7fffffff) split = min(split, 2 * max(other, first_idx) + 1);
+        }
+    });
+#pragma unroll
+    for (int off = T / 2; off >= 1; off >>= 1) {
+        n0 += __shfl_xor(n0, off, T);
+        split = min(split, __shfl_xor(split, off, T));
+        const float od = __shfl_xor(best_sd, off, T);
+        const int os = __shfl_xor(best_s, off, T);
+        if (os >= 0 && (best_s < 0 || od < best_sd || (od == best_sd && preorder_less(pool, os, best_s)))) {
+            best_sd = od;
+            best_s = os;
+        }
+    }
+
+    // ---- the ordered fold over this voxel's batch points (every lane of the team alike)
+    bool have_new = false;
+    float cx = 0, cy = 0, cz = 0;
+    uint32_t cid = 0;
+    float ex = 0, ey = 0, ez = 0;  // the single original point when |S0| == 1
+    if (best_s >= 0) {
+        const float* f = pool.hot[best_s].f;
+        ex = f[0];
+        ey = f[1];
+        ez = f[2];
+    }
+    const uint32_t eid = best_s >= 0 ? pool.ids[best_s] : 0u;
+    int added = 0;
+    for (int j = j0; j < j1; ++j) {
+        const int idx = (int)order[j];
+        const float4 P = pts[idx];
+        if (check) {
+            if (!box_contains(box, P.x, P.y, P.z)) split = min(split, 2 * idx);
+            const int other = foreign_voxel_first_index(g, P.x, P.y, P.z, keys, order, n);
+            if (other != 0x7fffffff) split = min(split, 2 * idx);
+        }
+        const float dp = calc_dist(P.x, P.y, P.z, mx, my, mz);
+        float bx = P.x, by = P.y, bz = P.z;
+        uint32_t bid = fbits(P.w);
+        int s_count;
+        if (j == j0) {
+            s_count = n0;
+            if (n0 > 0 && best_sd < dp) {
+                bx = ex;
+                by = ey;
+                bz = ez;
+                bid = eid;
+            }
+        } else {
+            s_count = 1;
+            const float qx = have_new ? cx : ex, qy = have_new ? cy : ey, qz = have_new ? cz : ez;
+            const uint32_t qid = have_new ? cid : eid;
+            const float de = calc_dist(qx, qy, qz, mx, my, mz);
+            if (de < dp) {
+                bx = qx;
+                by = qy;
+                bz = qz;
+                bid = qid;
+            }
+        }
+        if (s_count > 1 || same_point(P.x, P.y, P.z, bx, by, bz)) {
+            have_new = true;
+            cx = bx;
+            cy = by;
+            cz = bz;
+            cid = bid;
+            ++added;
+        }
+    }
+    if (valid && tl == 0) {
+        if (split != 0x7fffffff) atomicMin(&hdr->split_index, split);
+        atomicAdd(&hdr->added, added);
+        del_flag[v] = (have_new && n0 > 0) ? 1 : 0;
+        ins_flag[v] = have_new ? 1 : 0;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) del_boxes[6 * v + a] = box[a];
+        ins_pts[v] = make_float4(cx, cy, cz, bitsf(cid));
+    }
+
+    // The voxel's original points S0 are deleted (downsample labels) when a
+    // new point replaces them: Delete_by_range(voxel, downsample) of cpp:446.
+    // Voxel boxes are disjoint here, so each S0 node is written by one lane.
+    const bool del = valid && del_out && have_new && n0 > 0 && split == 0x7fffffff;
+    team_box_walk<T>(pool, box, del, lds_dyn, stack_cap, dn, max_steps, hdr, [&](int u, const float*) {
+        atomicOr(&pool.meta[u], M_PDEL | M_PDS);
+        atomicOr(reinterpret_cast<uint32_t*>(&pool.hot[u].f[3]), LK_PDEL);
+        const int pos = atomicAdd(&hdr->rec_count, 1);
+        if (pos < del_cap) del_out[pos] = u;
+        else atomicOr(&hdr->overflow, 4);
+    });
+}
+
 // stream compaction of the flagged voxel boxes / points
 __global__ void k_compact_boxes(const int* __restrict__ flag, const int* __restrict__ scan, const float* __restrict__ in,
                                 int bound, DevHdr* hdr, float* __restrict__ out, int* __restrict__ out_count) {
@@ -317,11 +569,23 @@ void launch_voxel_starts(const int* flag, const int* scan, int n, int* vstart, D
 void launch_voxel_fold2(Pool pool, const float4* pts, const uint32_t* order, const unsigned long long* keys,
                         const int* vstart, int n, int bound, float ds, int check, float* del_boxes, int* del_flag,
                         float4* ins_pts, int* ins_flag, DevHdr* hdr, long long max_steps, int* del_out, int del_cap,
-                        int stack_cap, hipStream_t s) {
+                        int stack_cap, int lanes, hipStream_t s) {
     if (bound <= 0) return;
-    hipLaunchKernelGGL(k_voxel_fold, dim3(blocks_for(bound, 64)), dim3(64), (size_t)stack_cap * 64 * sizeof(int), s,
-                       pool, pts, order, keys, vstart, n, bound, ds, check, del_boxes, del_flag, ins_pts, ins_flag, hdr,
-                       max_steps, del_out, del_cap, stack_cap);
+    const size_t lds = (size_t)stack_cap * 64 * sizeof(int);
+#define IKD_FOLD_TEAM(T)                                                                                            \
+    hipLaunchKernelGGL(k_voxel_fold_team<T>, dim3(blocks_for(bound, 64 / (T))), dim3(64), lds, s, pool, pts, order, \
+                       keys, vstart, n, bound, ds, check, del_boxes, del_flag, ins_pts, ins_flag, hdr, max_steps,     \
+                       del_out, del_cap, stack_cap)
+    switch (lanes) {
+        case 4: IKD_FOLD_TEAM(4); break;
+        case 8: IKD_FOLD_TEAM(8); break;
+        case 16: IKD_FOLD_TEAM(16); break;
+        default:
+            hipLaunchKernelGGL(k_voxel_fold, dim3(blocks_for(bound, 64)), dim3(64), lds, s, pool, pts, order, keys,
+                               vstart, n, bound, ds, check, del_boxes, del_flag, ins_pts, ins_flag, hdr, max_steps,
+                               del_out, del_cap, stack_cap);
+    }
+#undef IKD_FOLD_TEAM
 }
 void launch_compact_boxes(const int* flag, const int* scan, const float* in, int bound, DevHdr* hdr, float* out,
                           int* out_count, hipStream_t s) {

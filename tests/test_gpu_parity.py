@@ -288,6 +288,72 @@ def test_add_points_downsample_vs_oracle(ikd, ds):
         assert_same_multiset(g.flatten()[0], o.flatten()[0], f"downsampled live set, batch {it}")
 
 
+def _dense_voxel_cloud(rng):
+    """Thousands of points in a few 0.5 m voxels, exact duplicates, and pairs of
+    distinct points at the same distance from their voxel's centre (the
+    reference keeps the first of them in Search_by_range's pre-order)."""
+    dense = rng.uniform(0.0, 2.0, (12000, 3)).astype(np.float32)
+    c = (np.floor(rng.uniform(4, 8, (300, 3))) * 0.5 + 0.25).astype(np.float32)  # voxel centres beside the cube
+    e = np.float32(0.125)
+    tie_a = c + np.array([e, 0, 0], np.float32)
+    tie_b = c - np.array([e, 0, 0], np.float32)
+    tie_c = c + np.array([0, 0, e], np.float32)
+    return np.concatenate([dense, tie_a, tie_b, tie_c, dense[:500]])
+
+
+def _voxel_signature(pts, ds):
+    """(voxel index, squared distance to the voxel centre) of every point, sorted:
+    equal for two live sets that keep, in every voxel, points at the same
+    distances from its centre (ties between distinct equidistant points aside)."""
+    v = np.floor(pts / np.float32(ds)).astype(np.int64)
+    c = v.astype(np.float64) * ds + ds / 2
+    d = ((pts.astype(np.float64) - c) ** 2).sum(axis=1)
+    return sorted(zip(map(tuple, v.tolist()), d.tolist()))
+
+
+def test_downsample_dense_voxels_vs_oracle(ikd):
+    """Dense voxels with exact distance ties.  Which of several equidistant
+    points a voxel keeps follows the pre-order of the tree at the start of the
+    batch; the reference rebuilds inside its per-point loop, which can reorder
+    the tied points first, so the comparison is per voxel and distance."""
+    rng = np.random.default_rng(61)
+    base = _dense_voxel_cloud(rng)
+    g, o = _pair(ikd, base, params=(0.5, 0.6, 0.5))
+    for it in range(4):
+        batch = _dense_batch(rng, it)
+        assert g.Add_Points(batch, True) == o.Add_Points(batch, True), f"Add_Points return, batch {it}"
+        assert g.validnum() == o.validnum()
+        assert _voxel_signature(g.flatten()[0], 0.5) == _voxel_signature(o.flatten()[0], 0.5), f"batch {it}"
+
+
+@pytest.mark.parametrize("lanes", [4, 8, 16])
+def test_downsample_fold_lanes_identical(ikd, lanes):
+    """Teams of lanes per voxel keep exactly the points (and ids) one lane keeps,
+    equidistant ties included."""
+    rng = np.random.default_rng(62)
+    base = _dense_voxel_cloud(rng)
+    out = []
+    for ln in (1, lanes):
+        g = ikd.KD_TREE(0.5, 0.6, 0.5)
+        g.Build(base)
+        g.set_tuning(ikd.IKD_TUNE_FOLD_LANES, ln)
+        r = np.random.default_rng(63)
+        # one call after a Build: the tree the fold walks is the same in both
+        added = g.Add_Points(np.concatenate([_dense_batch(r, 0), _dense_batch(r, 1)]), True)
+        x, ids = g.flatten()
+        order = np.lexsort((ids, x[:, 2], x[:, 1], x[:, 0]))
+        out.append((added, x[order], ids[order]))
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
+
+
+def _dense_batch(rng, it):
+    if it == 1:  # new points farther from the centres than the tied points
+        cc = (np.floor(rng.uniform(4, 8, (400, 3))) * 0.5 + 0.25).astype(np.float32)
+        return cc + np.float32(0.24)
+    return rng.uniform(-0.5, 4.5, (2000, 3)).astype(np.float32)
+
+
 def test_delete_points(ikd):
     rng = np.random.default_rng(70)
     pts = uniform_cloud(rng, 20000)
