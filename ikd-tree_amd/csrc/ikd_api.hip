@@ -101,10 +101,6 @@ __global__ void k_node_info(Pool pool, const int* __restrict__ top, int n, int4*
     nodes_before[i] = top[2 * i + 1];
 }
 
-__global__ void k_seg_of_from_keys(const unsigned long long* __restrict__ keys, int n, int* __restrict__ seg_of) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) seg_of[i] = (int)(keys[i] >> 32);
-}
 
 __global__ void k_unpack_points(const float4* __restrict__ in, int n, float* __restrict__ xyz,
                                 uint32_t* __restrict__ ids) {
@@ -195,7 +191,7 @@ enum Buf {
     B_SEEDS, B_DIRTY, B_VIOL, B_TOP, B_INFO, B_FRONT0, B_FRONT1, B_EKEY0, B_EKEY1, B_RECS, B_GFLAG, B_GSCAN, B_ATTACH,
     B_PFLAG, B_PSCAN, B_ISEG, B_ISEGD, B_IO0, B_IO1, B_IO2, B_IO3, B_IO4, B_VSTART, B_DELBOX, B_DELFL, B_INSPT,
     B_INSFL, B_CBOX, B_CPTS, B_MISC, B_MORTK, B_MORTV, B_MORTK2, B_MORTV2, B_CNT64, B_OFF64, B_ADDX, B_ADDI, B_PNEXT, B_SPAIRS, B_SPSCAN,
-    B_DEFER, B_DINFO, B_DBEFORE, B_DKEEP, B_PTS3, B_PEND, B_START, B_COUNT
+    B_DEFER, B_DINFO, B_DBEFORE, B_DKEEP, B_PTS3, B_PEND, B_START, B_INV0, B_COUNT
 };
 static_assert(B_COUNT <= 80, "scratch table");
 
@@ -449,7 +445,7 @@ int run_build(ikd_tree_t t, const float4* pts, const int* seg_of, int m, const i
         CHK(ensure(t, B_SPSCAN, S, &pscan));
         launch_small_pairs(d_seg, S, nullptr, pairs, st);
         CHK(exscan_i32(t, pairs, pscan, S));
-        launch_build_small(pts, d_seg, d_seg_depth, pscan, S, nullptr, nullptr, t->pool, t->d_hdr, st);
+        launch_build_small(pts, d_seg, d_seg_depth, pscan, S, nullptr, nullptr, SmallLists{}, t->pool, t->d_hdr, st);
         launch_small_commit(pairs, pscan, S, t->d_hdr, st);
         HIPCHK(hipGetLastError());
         return IKD_OK;
@@ -551,11 +547,19 @@ int run_build(ikd_tree_t t, const float4* pts, const int* seg_of, int m, const i
         CHK(ensure(t, B_PTS3, m, &gpts));
         CHK(ensure(t, B_SPAIRS, bound, &pairs));
         CHK(ensure(t, B_SPSCAN, bound, &pscan));
-        launch_gather_list(pts, perm[cur][0], sid[cur], m, gpts, st);
+        int* inv0;
+        CHK(ensure(t, B_INV0, m, &inv0));
+        launch_gather_list(pts, perm[cur][0], sid[cur], m, gpts, inv0, st);
         const int* count_dev = &t->d_hdr->seg_count[cur];
         launch_small_pairs(seg[cur], bound, count_dev, pairs, st);
         CHK(exscan_i32(t, pairs, pscan, bound));
-        launch_build_small(gpts, seg[cur], segd[cur], pscan, bound, count_dev, perm[cur][0], t->pool, t->d_hdr, st);
+        // the level-synchronous lists are already sorted per segment
+        SmallLists pre;
+        pre.list1 = perm[cur][1];
+        pre.list2 = perm[cur][2];
+        pre.inv0 = inv0;
+        launch_build_small(gpts, seg[cur], segd[cur], pscan, bound, count_dev, perm[cur][0], pre, t->pool, t->d_hdr,
+                           st);
         launch_small_commit(pairs, pscan, bound, t->d_hdr, st);
     }
     HIPCHK(hipGetLastError());
@@ -564,46 +568,45 @@ int run_build(ikd_tree_t t, const float4* pts, const int* seg_of, int m, const i
 
 // ---------------------------------------------------------- enumerate live
 // Live points of the subtrees rooted at roots[] (host list), grouped by root
-// in root order, each group ordered by node index.  Returns the points in
-// B_PTS2 (float4) and their root index in B_SEGOF.
-int enumerate_live(ikd_tree_t t, const std::vector<int>& roots, long long total_nodes, long long total_live,
-                   float4** out_pts, int** out_seg) {
+// in root order, each group in the reference's flatten order (own, left,
+// right).  live[i] is root i's live point count (its group's size).  Returns
+// the points in B_PTS2 (float4) and their root index in B_SEGOF.
+int enumerate_live(ikd_tree_t t, const std::vector<int>& roots, const std::vector<long long>& live,
+                   long long total_nodes, long long total_live, float4** out_pts, int** out_seg) {
     hipStream_t st = t->st;
     int R = (int)roots.size();
     long long cap = std::max<long long>(total_nodes, 1);
-    int2 *f0, *f1;
-    unsigned long long *k0, *k1;
+    int4 *f0, *f1;
     CHK(ensure(t, B_FRONT0, cap, &f0));
     CHK(ensure(t, B_FRONT1, cap, &f1));
-    CHK(ensure(t, B_EKEY0, std::max<long long>(total_live, 1), &k0));
-    CHK(ensure(t, B_EKEY1, std::max<long long>(total_live, 1), &k1));
-    std::vector<int2> h(R);
-    for (int i = 0; i < R; ++i) h[i] = make_int2(roots[i], i);
-    HIPCHK(hipMemcpyAsync(f0, h.data(), R * sizeof(int2), hipMemcpyHostToDevice, st));
+    const int n = (int)total_live;
+    float4* pts;
+    int* segof;
+    CHK(ensure(t, B_PTS2, std::max(n, 1), &pts));
+    CHK(ensure(t, B_SEGOF, std::max(n, 1), &segof));
+    std::vector<int4> h(R);
+    long long base = 0;
+    for (int i = 0; i < R; ++i) {
+        h[i] = make_int4(roots[i], (int)base, i, 0);
+        base += live[i];
+    }
+    HIPCHK(hipMemcpyAsync(f0, h.data(), R * sizeof(int4), hipMemcpyHostToDevice, st));
     int init[3] = {R, 0, 0};
     HIPCHK(hipMemcpyAsync(t->d_hdr->front3, init, sizeof(init), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, t->d_hdr, 2);
     int levels = t->max_depth + 2;
     for (int L = 0; L < levels; ++L) {
-        int2* fin = (L & 1) ? f1 : f0;
-        int2* fout = (L & 1) ? f0 : f1;
-        launch_enum_level(t->pool, fin, L, fout, k0, t->d_hdr, (int)cap, (int)std::max<long long>(total_live, 1), st);
+        int4* fin = (L & 1) ? f1 : f0;
+        int4* fout = (L & 1) ? f0 : f1;
+        launch_enum_preorder(t->pool, fin, L, fout, pts, segof, t->d_hdr, (int)cap, std::max(n, 1), st);
     }
-    CHK(sync_hdr(t));
-    if (t->h_hdr->rec_count != total_live) {
+    CHK(sync_hdr(t));  // also keeps h and init alive until the copies ran
+    if (t->h_hdr->rec_count != total_live || (t->h_hdr->overflow & 32)) {
         char msg[160];
         snprintf(msg, sizeof msg, "enumerate found %d live points, counts say %lld", t->h_hdr->rec_count, total_live);
         t->err = msg;
         return IKD_ERR_INTERNAL;
     }
-    int n = (int)total_live;
-    CHK(sort_keys(t, k0, k1, n, 32 + bits_for(R)));
-    float4* pts;
-    int* segof;
-    CHK(ensure(t, B_PTS2, std::max(n, 1), &pts));
-    CHK(ensure(t, B_SEGOF, std::max(n, 1), &segof));
-    launch_gather_points(t->pool, k1, n, pts, st);
-    if (n > 0) hipLaunchKernelGGL(k_seg_of_from_keys, dim3(nblk(n)), dim3(BS), 0, st, k1, n, segof);
     HIPCHK(hipGetLastError());
     *out_pts = pts;
     *out_seg = segof;
@@ -658,6 +661,7 @@ int rebuild_listed(ikd_tree_t t, const std::vector<int4>& hinfo, const std::vect
     hipStream_t st = t->st;
     const int ntop = (int)hinfo.size();
     std::vector<int> roots;
+    std::vector<long long> lives;
     std::vector<int4> segs;
     std::vector<int> segd;
     long long total_nodes = 0, total_live = 0;
@@ -676,6 +680,7 @@ int rebuild_listed(ikd_tree_t t, const std::vector<int4>& hinfo, const std::vect
         segs.push_back(make_int4((int)total_live, (int)(total_live + live - 1), u, parent));
         segd.push_back(depth);
         roots.push_back(u);
+        lives.push_back(live);
         total_nodes += hbefore[i];
         total_live += live;
         t->ctr.rebuilt_points += live;
@@ -684,7 +689,7 @@ int rebuild_listed(ikd_tree_t t, const std::vector<int4>& hinfo, const std::vect
     if (!roots.empty()) {
         float4* pts;
         int* segof;
-        CHK(enumerate_live(t, roots, total_nodes, total_live, &pts, &segof));
+        CHK(enumerate_live(t, roots, lives, total_nodes, total_live, &pts, &segof));
         int4* dseg;
         int* dsegd;
         CHK(ensure(t, B_ISEG, (long long)segs.size(), &dseg));
@@ -742,7 +747,8 @@ int compact_if_needed(ikd_tree_t t, bool force) {
     float4* pts;
     int* segof;
     std::vector<int> roots = {ROOT};
-    CHK(enumerate_live(t, roots, t->h_root->cnt.x, live, &pts, &segof));
+    std::vector<long long> lives = {live};
+    CHK(enumerate_live(t, roots, lives, t->h_root->cnt.x, live, &pts, &segof));
     // move the points out of the scratch slot the build reuses
     float4* keep;
     CHK(ensure(t, B_PTS, live, &keep));
@@ -1456,7 +1462,8 @@ int ikd_flatten(ikd_tree_t t, float* out_xyz, uint32_t* out_ids, int64_t capacit
     float4* pts;
     int* segof;
     std::vector<int> roots = {ROOT};
-    CHK(enumerate_live(t, roots, t->h_root->cnt.x, live, &pts, &segof));
+    std::vector<long long> lives = {live};
+    CHK(enumerate_live(t, roots, lives, t->h_root->cnt.x, live, &pts, &segof));
     float* dx;
     uint32_t* di;
     CHK(ensure(t, B_IO0, 3 * live, &dx));

@@ -201,16 +201,24 @@ static inline int blocks_for(long long n, int bs) { return (int)((n + bs - 1) / 
 // points are contiguous for the one-workgroup build of the levels below.
 // Positions whose point is already a node of an upper level (sid < 0) are
 // never written by the partitions and hold no valid index: skip them.
+// out[j] = the point at list position j; inv[point] = j (positions already
+// placed as nodes are skipped: no kernel wrote their list entries).
 __global__ void k_gather_list(const float4* __restrict__ pts, const uint32_t* __restrict__ perm,
-                              const int* __restrict__ sid, int m, float4* __restrict__ out) {
+                              const int* __restrict__ sid, int m, float4* __restrict__ out, int* __restrict__ inv) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= m) return;
-    out[j] = sid[j] >= 0 ? pts[perm[j]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (sid[j] < 0) {
+        out[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const uint32_t p = perm[j];
+    out[j] = pts[p];
+    if (inv) inv[p] = j;
 }
-void launch_gather_list(const float4* pts, const uint32_t* perm, const int* sid, int m, float4* out,
+void launch_gather_list(const float4* pts, const uint32_t* perm, const int* sid, int m, float4* out, int* inv,
                         hipStream_t s) {
     if (m <= 0) return;
-    hipLaunchKernelGGL(k_gather_list, dim3(blocks_for(m, 256)), dim3(256), 0, s, pts, perm, sid, m, out);
+    hipLaunchKernelGGL(k_gather_list, dim3(blocks_for(m, 256)), dim3(256), 0, s, pts, perm, sid, m, out, inv);
 }
 
 void launch_build_keys(const float4* pts, const int* seg_of, int m, int axis, unsigned long long* keys,

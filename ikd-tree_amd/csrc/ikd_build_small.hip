@@ -135,7 +135,8 @@ __global__ __launch_bounds__(SB) void k_build_small(const float4* __restrict__ p
                                                      const int* __restrict__ seg_depth,
                                                      const int* __restrict__ pair_scan,
                                                      const int* __restrict__ count_dev,
-                                                     const uint32_t* __restrict__ orig, Pool pool, DevHdr* hdr) {
+                                                     const uint32_t* __restrict__ orig, SmallLists pre, Pool pool,
+                                                     DevHdr* hdr) {
     constexpr int SMAX_SEG = SMAX / 2 + 1;
     __shared__ float px[SMAX], py[SMAX], pz[SMAX];
     __shared__ unsigned long long skey[SMAX];  // sort keys, then running counts
@@ -168,6 +169,15 @@ __global__ __launch_bounds__(SB) void k_build_small(const float4* __restrict__ p
     }
     int P = 1;
     while (P < m) P <<= 1;
+    if (pre.inv0) {
+        // the caller's lists are already sorted (coordinate, original index)
+        // per segment, and the points were gathered in x order
+        for (int i = t; i < m; i += SB) {
+            lst[0][0][i] = (unsigned short)i;
+            lst[0][1][i] = (unsigned short)(pre.inv0[pre.list1[l0 + i]] - l0);
+            lst[0][2][i] = (unsigned short)(pre.inv0[pre.list2[l0 + i]] - l0);
+        }
+    } else {
     // tie-break rank of every point: its order by original index, or its position
     if (orig) {
         for (int i = t; i < P; i += SB)
@@ -189,6 +199,7 @@ __global__ __launch_bounds__(SB) void k_build_small(const float4* __restrict__ p
         block_bitonic<SB>(skey, P);
         for (int i = t; i < m; i += SB) lst[0][a][i] = (unsigned short)(skey[i] & 0xffffu);
         __syncthreads();
+    }
     }
     for (int i = t; i < m; i += SB) sid[0][i] = 0;
     if (t == 0) {
@@ -343,12 +354,13 @@ void launch_small_pairs(const int4* seg, int S, const int* count_dev, int* pairs
     hipLaunchKernelGGL(k_small_pairs, dim3(blocks_for(S, 256)), dim3(256), 0, s, seg, S, count_dev, pairs);
 }
 void launch_build_small(const float4* pts, const int4* seg, const int* seg_depth, const int* pair_scan, int S,
-                        const int* count_dev, const uint32_t* orig, Pool pool, DevHdr* hdr, hipStream_t s) {
+                        const int* count_dev, const uint32_t* orig, SmallLists pre, Pool pool, DevHdr* hdr,
+                        hipStream_t s) {
     if (S <= 0) return;
     hipLaunchKernelGGL((k_build_small<64, TINY_MAX>), dim3(S), dim3(64), 0, s, pts, seg, seg_depth, pair_scan,
-                       count_dev, orig, pool, hdr);
+                       count_dev, orig, pre, pool, hdr);
     hipLaunchKernelGGL((k_build_small<256, SMALL_MAX>), dim3(S), dim3(256), 0, s, pts, seg, seg_depth, pair_scan,
-                       count_dev, orig, pool, hdr);
+                       count_dev, orig, pre, pool, hdr);
 }
 void launch_small_commit(const int* pairs, const int* pair_scan, int S, DevHdr* hdr, hipStream_t s) {
     if (S <= 0) return;

@@ -81,9 +81,19 @@ void launch_build_flags(const BuildLevel& L, hipStream_t s);
 void launch_build_scatter(const BuildLevel& L, hipStream_t s);
 
 void launch_small_pairs(const int4* seg, int S, const int* count_dev, int* pairs, hipStream_t s);
+// Presorted lists for the small build (all null: the kernel sorts): the
+// segments' y- and z-ordered lists of point indices (list1, list2) and, for
+// every point, its position in the x-ordered list the points were gathered in.
+struct SmallLists {
+    const uint32_t* list1 = nullptr;
+    const uint32_t* list2 = nullptr;
+    const int* inv0 = nullptr;
+};
 void launch_build_small(const float4* pts, const int4* seg, const int* seg_depth, const int* pair_scan, int S,
-                        const int* count_dev, const uint32_t* orig, Pool pool, DevHdr* hdr, hipStream_t s);
-void launch_gather_list(const float4* pts, const uint32_t* perm, const int* sid, int m, float4* out, hipStream_t s);
+                        const int* count_dev, const uint32_t* orig, SmallLists pre, Pool pool, DevHdr* hdr,
+                        hipStream_t s);
+void launch_gather_list(const float4* pts, const uint32_t* perm, const int* sid, int m, float4* out, int* inv,
+                        hipStream_t s);
 void launch_small_commit(const int* pairs, const int* pair_scan, int S, DevHdr* hdr, hipStream_t s);
 
 // ---------------------------------------------------------------- refit
@@ -98,9 +108,8 @@ void launch_find_top(Pool pool, const int* viol, int bound, DevHdr* hdr, int* to
 void launch_clear_viol(Pool pool, const int* viol, int bound, DevHdr* hdr, hipStream_t s);
 
 // ---------------------------------------------------------------- enumerate
-void launch_enum_level(Pool pool, const int2* front, int lvl, int2* front_next, unsigned long long* out,
-                       DevHdr* hdr, int bound, int out_cap, hipStream_t s);
-void launch_gather_points(Pool pool, const unsigned long long* keys, int n, float4* out, hipStream_t s);
+void launch_enum_preorder(Pool pool, const int4* front, int lvl, int4* front_next, float4* out_pts, int* out_seg,
+                          DevHdr* hdr, int bound, int out_cap, hipStream_t s);
 
 // ---------------------------------------------------------------- insert
 void launch_insert_descend(Pool pool, const float4* pts, int n, unsigned long long* keys, uint32_t* vals,

@@ -312,52 +312,57 @@ void launch_clear_viol(Pool pool, const int* viol, int bound, DevHdr* hdr, hipSt
 }
 
 // ======================================================= enumerate (flatten)
-// Breadth-first walk from a set of subtree roots, stopping at tree_deleted
-// nodes; emits (segment << 32 | node) for every live node.  Three rotating
-// frontier counters let a level reset the counter of the level after next.
-__global__ void k_enum_level(Pool pool, const int2* __restrict__ front, int lvl, int2* __restrict__ front_next,
-                             unsigned long long* __restrict__ out, DevHdr* hdr, int bound, int out_cap) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) hdr->front3[(lvl + 2) % 3] = 0;
-    if (i >= bound) return;
-    if (i >= hdr->front3[lvl % 3]) return;
-    int2 e = front[i];
-    int u = e.x;
-    uint32_t m = pool.meta[u];
-    if (m & M_TDEL) return;
-    if (!(m & M_PDEL)) {
-        int pos = atomicAdd(&hdr->rec_count, 1);
-        if (pos < out_cap) out[pos] = ((unsigned long long)(uint32_t)e.y << 32) | (uint32_t)u;
-    }
-    uint32_t lk = load_link(pool.hot, u);
-    int pair = (int)lk_pair(lk);
+// Live points of several subtrees in the reference's flatten order (own
+// point, left subtree, right subtree; ikd_Tree.cpp:1326-1333), one depth
+// level per launch.  A frontier entry (node, base, segment) knows where the
+// first live point of its subtree goes: its own point takes base when live,
+// the left subtree starts after it and the right subtree after the left's
+// live count (TreeSize - invalid_point_num of the left child).  Tree-deleted
+// subtrees hold no live points and are not entered.  Three rotating frontier
+// counters let a level reset the counter of the level after next.
+__global__ void k_enum_preorder(Pool pool, const int4* __restrict__ front, int lvl, int4* __restrict__ front_next,
+                                float4* __restrict__ out_pts, int* __restrict__ out_seg, DevHdr* hdr, int bound,
+                                int out_cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) hdr->front3[(lvl + 2) % 3] = 0;
+    const int nf = min(hdr->front3[lvl % 3], bound);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nf; i += gridDim.x * blockDim.x) {
+        const int4 e = front[i];
+        const int u = e.x;
+        const uint32_t m = pool.meta[u];
+        if (m & M_TDEL) continue;
+        int next = e.y;
+        if (!(m & M_PDEL)) {
+            if (next < out_cap) {
+                const float4 p = *reinterpret_cast<const float4*>(pool.hot[u].f);
+                out_pts[next] = make_float4(p.x, p.y, p.z, bitsf(pool.ids[u]));
+                out_seg[next] = e.z;
+                atomicAdd(&hdr->rec_count, 1);
+            } else {
+                atomicOr(&hdr->overflow, 32);
+            }
+            ++next;
+        }
+        const uint32_t lk = load_link(pool.hot, u);
+        const int pair = (int)lk_pair(lk);
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        if ((lk & lk_has(s)) && !(lk & lk_td(s))) {
-            int pos = atomicAdd(&hdr->front3[(lvl + 1) % 3], 1);
-            if (pos < bound) front_next[pos] = make_int2(2 * pair + s, e.y);
+        for (int s = 0; s < 2; ++s) {
+            if (!(lk & lk_has(s)) || (lk & lk_td(s))) continue;
+            const int c = 2 * pair + s;
+            const int4 cc = pool.cnt[c];
+            const int pos = atomicAdd(&hdr->front3[(lvl + 1) % 3], 1);
+            if (pos < bound) front_next[pos] = make_int4(c, next, e.z, 0);
             else atomicOr(&hdr->overflow, 32);
+            next += cc.x - cc.y;
         }
     }
 }
 
-__global__ void k_gather_points(Pool pool, const unsigned long long* __restrict__ keys, int n,
-                                float4* __restrict__ out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int u = (int)(uint32_t)(keys[i] & 0xffffffffull);
-    float4 p = *reinterpret_cast<const float4*>(pool.hot[u].f);
-    out[i] = make_float4(p.x, p.y, p.z, bitsf(pool.ids[u]));
-}
-
-void launch_enum_level(Pool pool, const int2* front, int lvl, int2* front_next, unsigned long long* out,
-                       DevHdr* hdr, int bound, int out_cap, hipStream_t s) {
-    hipLaunchKernelGGL(k_enum_level, dim3(blocks_for(bound > 0 ? bound : 1, 256)), dim3(256), 0, s, pool, front, lvl,
-                       front_next, out, hdr, bound, out_cap);
-}
-void launch_gather_points(Pool pool, const unsigned long long* keys, int n, float4* out, hipStream_t s) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_gather_points, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, keys, n, out);
+void launch_enum_preorder(Pool pool, const int4* front, int lvl, int4* front_next, float4* out_pts, int* out_seg,
+                          DevHdr* hdr, int bound, int out_cap, hipStream_t s) {
+    // a grid-stride loop: the frontier size is known only on the device
+    const int blocks = std::min(blocks_for(bound > 0 ? bound : 1, 256), 2048);
+    hipLaunchKernelGGL(k_enum_preorder, dim3(blocks), dim3(256), 0, s, pool, front, lvl, front_next, out_pts, out_seg,
+                       hdr, bound, out_cap);
 }
 
 // ================================================================== insert
