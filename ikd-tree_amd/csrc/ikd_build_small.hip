@@ -105,26 +105,55 @@ __device__ unsigned long long block_exscan_u64(unsigned long long v, unsigned lo
     return base + x - v;
 }
 
-// Ascending bitonic sort of skey[0, P) (P a power of two) by the workgroup.
-template <int SB>
-__device__ void block_bitonic(unsigned long long* skey, int P) {
+// Ascending bitonic sort of P keys (P a power of two, at most E * SB) held in
+// registers: key i lives in x[i / SB] of thread i % SB.  Stages whose partner
+// is in the same wavefront exchange by shuffles, stages across the thread's
+// own registers by swaps, and only the stages across wavefronts go through
+// LDS (xch, SB * E entries).  Keys must be distinct.
+template <int SB, int E>
+__device__ void reg_bitonic(unsigned long long (&x)[E], int P, unsigned long long* xch) {
     const int t = threadIdx.x;
     for (int k = 2; k <= P; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = t; i < P; i += SB) {
-                int ixj = i ^ j;
-                if (ixj > i) {
-                    unsigned long long x = skey[i], y = skey[ixj];
-                    bool up = (i & k) == 0;
-                    if ((x > y) == up) {
-                        skey[i] = y;
-                        skey[ixj] = x;
+            if (j >= SB) {
+                const int ej = j / SB;
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const int e2 = e ^ ej;
+                    if (e2 <= e || e2 >= E) continue;
+                    const bool up = ((e * SB + t) & k) == 0;
+                    const unsigned long long a = x[e], b = x[e2];
+                    if ((a > b) == up) {
+                        x[e] = b;
+                        x[e2] = a;
                     }
                 }
+            } else if (j >= 64) {
+                __syncthreads();
+#pragma unroll
+                for (int e = 0; e < E; ++e) xch[e * SB + t] = x[e];
+                __syncthreads();
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const int i = e * SB + t;
+                    const unsigned long long y = xch[i ^ j];
+                    const bool keep_min = ((i & j) == 0) == ((i & k) == 0);
+                    x[e] = keep_min ? (x[e] < y ? x[e] : y) : (x[e] > y ? x[e] : y);
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const int i = e * SB + t;
+                    const unsigned lo = (unsigned)__shfl_xor((int)(unsigned)x[e], j);
+                    const unsigned hi = (unsigned)__shfl_xor((int)(unsigned)(x[e] >> 32), j);
+                    const unsigned long long y = ((unsigned long long)hi << 32) | lo;
+                    const bool keep_min = ((i & j) == 0) == ((i & k) == 0);
+                    x[e] = keep_min ? (x[e] < y ? x[e] : y) : (x[e] > y ? x[e] : y);
+                }
             }
-            __syncthreads();
         }
     }
+    __syncthreads();  // xch may be reused at once
 }
 
 // orig (optional) gives each point's index in the caller's original input:
@@ -179,12 +208,20 @@ __global__ __launch_bounds__(SB) void k_build_small(const float4* __restrict__ p
         }
     } else {
     // tie-break rank of every point: its order by original index, or its position
+    constexpr int E = SMAX / SB;
+    unsigned long long x[E];
     if (orig) {
-        for (int i = t; i < P; i += SB)
-            skey[i] = i < m ? (((unsigned long long)orig[l0 + i] << 16) | (unsigned)i) : ~0ull;
-        __syncthreads();
-        block_bitonic<SB>(skey, P);
-        for (int i = t; i < m; i += SB) rnk[skey[i] & 0xffffu] = (unsigned short)i;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int i = e * SB + t;
+            x[e] = i < m ? (((unsigned long long)orig[l0 + i] << 16) | (unsigned)i) : ~0ull;
+        }
+        reg_bitonic<SB, E>(x, P, skey);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int i = e * SB + t;
+            if (i < m) rnk[x[e] & 0xffffu] = (unsigned short)i;
+        }
     } else {
         for (int i = t; i < m; i += SB) rnk[i] = (unsigned short)i;
     }
@@ -192,14 +229,20 @@ __global__ __launch_bounds__(SB) void k_build_small(const float4* __restrict__ p
     // three stable sorted index lists: key (coordinate, rank, position)
     for (int a = 0; a < 3; ++a) {
         const float* c = a == 0 ? px : (a == 1 ? py : pz);
-        for (int i = t; i < P; i += SB)
-            skey[i] = i < m ? (((unsigned long long)float_key(c[i]) << 32) | ((unsigned)rnk[i] << 16) | (unsigned)i)
-                            : ~0ull;
-        __syncthreads();
-        block_bitonic<SB>(skey, P);
-        for (int i = t; i < m; i += SB) lst[0][a][i] = (unsigned short)(skey[i] & 0xffffu);
-        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int i = e * SB + t;
+            x[e] = i < m ? (((unsigned long long)float_key(c[i]) << 32) | ((unsigned)rnk[i] << 16) | (unsigned)i)
+                         : ~0ull;
+        }
+        reg_bitonic<SB, E>(x, P, skey);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int i = e * SB + t;
+            if (i < m) lst[0][a][i] = (unsigned short)(x[e] & 0xffffu);
+        }
     }
+    __syncthreads();
     }
     for (int i = t; i < m; i += SB) sid[0][i] = 0;
     if (t == 0) {
