@@ -73,8 +73,9 @@ def main():
     ap.add_argument("--trees", type=int, default=64, help="trees in the multi-tree batching phase (0 skips it)")
     ap.add_argument("--robots", type=int, default=16, help="maps in the serving phase (0 skips it)")
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="simulated arrival window of the serving phase")
-    ap.add_argument("--serve-defer", type=int, default=20_000,
+    ap.add_argument("--serve-defer", type=int, default=2_000,
                     help="serving: subtrees above this many live points are rebuilt between requests")
+    ap.add_argument("--serve-slots", type=int, default=8, help="serving: requests on the device at once")
     ap.add_argument("--serve-chunk", type=int, default=100_000,
                     help="serving: deferred-rebuild points per maintenance step (-1: all at once)")
     ap.add_argument("--lanes", type=int, default=0, help="force lanes per query (0: autotuned)")
@@ -257,7 +258,8 @@ def main():
 
     progress("phase C done")
     # ================================================ phase D: serving many robots
-    serving = (serving_phase(ikd, scenes, args.robots, args.serve_seconds, args.serve_defer, args.serve_chunk)
+    serving = (serving_phase(ikd, scenes, args.robots, args.serve_seconds, args.serve_defer, args.serve_chunk,
+                             args.serve_slots)
                if args.robots > 0 else None)
 
     result = {
@@ -496,7 +498,7 @@ def batching_phase(ikd, torch, dev, scenes, n_trees, seed=1004):
     return out
 
 
-def serving_phase(ikd, scenes, n_robots, seconds, defer_points, chunk_points, seed=1005):
+def serving_phase(ikd, scenes, n_robots, seconds, defer_points, chunk_points, slots, seed=1005):
     """Many robots, each with its own device-resident map (its own mapping
     scene, 0.5-3M points), sending scans (10k-60k points) as Poisson arrivals at
     10 Hz; each request is kNN (k=5, max_dist 5 m) + Add_Points (0.5 m voxels),
@@ -516,7 +518,7 @@ def serving_phase(ikd, scenes, n_robots, seconds, defer_points, chunk_points, se
             trees.append(tr)
             first.append(f)
     scan_len = rng.integers(10_000, 60_001, size=(n_robots, 4096))
-    srv = serving.Server(slots=8, hbm_budget_bytes=64 << 30)
+    srv = serving.Server(slots=slots, hbm_budget_bytes=64 << 30)
     # rebuilds of subtrees above defer_points run between a map's requests,
     # chunk_points at a time
     ids = [srv.add_map(t, defer_rebuild_points=defer_points) for t in trees]
@@ -546,7 +548,7 @@ def serving_phase(ikd, scenes, n_robots, seconds, defer_points, chunk_points, se
     srv.close()
     for t in trees:
         t.close()
-    res.update({"robots": n_robots, "map_points_total": int(sizes.sum()), "slots": 8, "hbm_budget_gb": 64,
+    res.update({"robots": n_robots, "map_points_total": int(sizes.sum()), "slots": slots, "hbm_budget_gb": 64,
                 "rate_hz_per_robot": 10.0, "defer_rebuild_points": defer_points,
                 "maintenance_chunk_points": chunk_points})
     return res

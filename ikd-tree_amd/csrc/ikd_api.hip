@@ -571,6 +571,10 @@ int run_build(ikd_tree_t t, const float4* pts, const int* seg_of, int m, const i
 // in root order, each group in the reference's flatten order (own, left,
 // right).  live[i] is root i's live point count (its group's size).  Returns
 // the points in B_PTS2 (float4) and their root index in B_SEGOF.
+// Subtree nodes one workgroup enumerates: a level costs it about what a
+// launch costs, so it only pays while every level fits one pass of 1024 threads.
+constexpr long long ENUM_BLOCK_MAX = 1024;
+
 int enumerate_live(ikd_tree_t t, const std::vector<int>& roots, const std::vector<long long>& live,
                    long long total_nodes, long long total_live, float4** out_pts, int** out_seg) {
     hipStream_t st = t->st;
@@ -595,10 +599,15 @@ int enumerate_live(ikd_tree_t t, const std::vector<int>& roots, const std::vecto
     HIPCHK(hipMemcpyAsync(t->d_hdr->front3, init, sizeof(init), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, t->d_hdr, 2);
     int levels = t->max_depth + 2;
-    for (int L = 0; L < levels; ++L) {
-        int4* fin = (L & 1) ? f1 : f0;
-        int4* fout = (L & 1) ? f0 : f1;
-        launch_enum_preorder(t->pool, fin, L, fout, pts, segof, t->d_hdr, (int)cap, std::max(n, 1), st);
+    if (cap <= ENUM_BLOCK_MAX) {
+        // small subtrees: one workgroup walks all levels
+        launch_enum_preorder_block(t->pool, f0, f1, R, pts, segof, t->d_hdr, (int)cap, std::max(n, 1), levels, st);
+    } else {
+        for (int L = 0; L < levels; ++L) {
+            int4* fin = (L & 1) ? f1 : f0;
+            int4* fout = (L & 1) ? f0 : f1;
+            launch_enum_preorder(t->pool, fin, L, fout, pts, segof, t->d_hdr, (int)cap, std::max(n, 1), st);
+        }
     }
     CHK(sync_hdr(t));  // also keeps h and init alive until the copies ran
     if (t->h_hdr->rec_count != total_live || (t->h_hdr->overflow & 32)) {
@@ -922,7 +931,7 @@ int add_points_ds_chunk(ikd_tree_t t, const float* d_xyz, const uint32_t* d_ids,
     // lanes per voxel: teams split the walks of dense voxels (maps of raw
     // scans hold up to thousands of points per voxel); the team deques index
     // their LDS columns with a power-of-two mask
-    const int fold_lanes = t->fold_lanes > 0 ? t->fold_lanes : (t->pool_used >= (1 << 16) ? 8 : 1);
+    const int fold_lanes = t->fold_lanes > 0 ? t->fold_lanes : (t->pool_used >= 4096 ? 8 : 1);
     int fold_stack = std::min(t->max_depth + 2, MAX_DEPTH);
     if (fold_lanes > 1) {
         int p2 = 1;

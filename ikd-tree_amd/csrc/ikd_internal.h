@@ -110,6 +110,8 @@ void launch_clear_viol(Pool pool, const int* viol, int bound, DevHdr* hdr, hipSt
 // ---------------------------------------------------------------- enumerate
 void launch_enum_preorder(Pool pool, const int4* front, int lvl, int4* front_next, float4* out_pts, int* out_seg,
                           DevHdr* hdr, int bound, int out_cap, hipStream_t s);
+void launch_enum_preorder_block(Pool pool, int4* f0, int4* f1, int n_roots, float4* out_pts, int* out_seg,
+                                DevHdr* hdr, int bound, int out_cap, int max_levels, hipStream_t s);
 
 // ---------------------------------------------------------------- insert
 void launch_insert_descend(Pool pool, const float4* pts, int n, unsigned long long* keys, uint32_t* vals,

@@ -357,6 +357,74 @@ __global__ void k_enum_preorder(Pool pool, const int4* __restrict__ front, int l
     }
 }
 
+// The same walk by one workgroup over every level, for subtrees of a few
+// thousand nodes: the levels are separated by workgroup barriers instead of
+// launches, and the walk ends as soon as a level is empty.  f0 holds the
+// n_roots root entries; f0 and f1 alternate as frontiers of `bound` entries.
+__global__ __launch_bounds__(1024) void k_enum_preorder_block(Pool pool, int4* __restrict__ f0, int4* __restrict__ f1,
+                                                              int n_roots, float4* __restrict__ out_pts,
+                                                              int* __restrict__ out_seg, DevHdr* hdr, int bound,
+                                                              int out_cap, int max_levels) {
+    __shared__ int next_count, written;
+    if (threadIdx.x == 0) written = 0;
+    int nf = min(n_roots, bound);
+    int4* fin = f0;
+    int4* fout = f1;
+    int lvl = 0;
+    for (; lvl < max_levels && nf > 0; ++lvl) {
+        if (threadIdx.x == 0) next_count = 0;
+        __syncthreads();
+        int mine = 0;
+        for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+            const int4 e = fin[i];
+            const int u = e.x;
+            const uint32_t m = pool.meta[u];
+            if (m & M_TDEL) continue;
+            int next = e.y;
+            if (!(m & M_PDEL)) {
+                if (next < out_cap) {
+                    const float4 p = *reinterpret_cast<const float4*>(pool.hot[u].f);
+                    out_pts[next] = make_float4(p.x, p.y, p.z, bitsf(pool.ids[u]));
+                    out_seg[next] = e.z;
+                    ++mine;
+                } else {
+                    atomicOr(&hdr->overflow, 32);
+                }
+                ++next;
+            }
+            const uint32_t lk = load_link(pool.hot, u);
+            const int pair = (int)lk_pair(lk);
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                if (!(lk & lk_has(s)) || (lk & lk_td(s))) continue;
+                const int c = 2 * pair + s;
+                const int4 cc = pool.cnt[c];
+                const int pos = atomicAdd(&next_count, 1);
+                if (pos < bound) fout[pos] = make_int4(c, next, e.z, 0);
+                else atomicOr(&hdr->overflow, 32);
+                next += cc.x - cc.y;
+            }
+        }
+        if (mine) atomicAdd(&written, mine);
+        __syncthreads();
+        nf = min(next_count, bound);
+        int4* tmp = fin;
+        fin = fout;
+        fout = tmp;
+        __syncthreads();  // every thread has read next_count before it is reset
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(&hdr->rec_count, written);
+        if (nf > 0) atomicOr(&hdr->overflow, 32);  // deeper than the levels allowed
+    }
+}
+
+void launch_enum_preorder_block(Pool pool, int4* f0, int4* f1, int n_roots, float4* out_pts, int* out_seg,
+                                DevHdr* hdr, int bound, int out_cap, int max_levels, hipStream_t s) {
+    hipLaunchKernelGGL(k_enum_preorder_block, dim3(1), dim3(1024), 0, s, pool, f0, f1, n_roots, out_pts, out_seg, hdr,
+                       bound, out_cap, max_levels);
+}
+
 void launch_enum_preorder(Pool pool, const int4* front, int lvl, int4* front_next, float4* out_pts, int* out_seg,
                           DevHdr* hdr, int bound, int out_cap, hipStream_t s) {
     // a grid-stride loop: the frontier size is known only on the device

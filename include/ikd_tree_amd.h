@@ -173,7 +173,7 @@ int ikd_get_counters(ikd_tree_t t, ikd_counters_t* out);
 #define IKD_TUNE_DEFER_REBUILD 5
 /*  IKD_TUNE_FOLD_LANES  lanes that walk one voxel's box in Add_Points'
  *                       downsample: 1, 4, 8 or 16; 0 (default) takes 8 for
- *                       maps of >= 65536 nodes and 1 below (identical results). */
+ *                       maps of >= 4096 nodes and 1 below (identical results). */
 #define IKD_TUNE_FOLD_LANES 6
 int ikd_set_tuning(ikd_tree_t t, int key, int value);
 
