@@ -37,7 +37,7 @@ __global__ void k_reset_counters(DevHdr* h, int which) {
     if (which & 1) { h->dirty_count = 0; h->viol_count = 0; h->top_count = 0; }
     if (which & 2) h->rec_count = 0;
     if (which & 4) h->conflicts = 0;
-    if (which & 8) { h->split_index = INT_MAX; h->added = 0; }
+    if (which & 8) { h->split_index = INT_MAX; h->added = 0; h->ins_count = 0; }
     if (which & 16) h->group_count = 0;
 }
 
@@ -305,6 +305,12 @@ int read_root(ikd_tree_t t) {
     HIPCHK(hipMemcpyAsync(t->h_root, t->d_root, sizeof(RootInfo), hipMemcpyDeviceToHost, t->st));
     HIPCHK(wait_stream(t));
     return IKD_OK;
+}
+// Header and root summary in one host wait.
+int refresh(ikd_tree_t t) {
+    hipLaunchKernelGGL(k_root_info, dim3(1), dim3(1), 0, t->st, t->pool, t->d_root);
+    HIPCHK(hipMemcpyAsync(t->h_root, t->d_root, sizeof(RootInfo), hipMemcpyDeviceToHost, t->st));
+    return sync_hdr(t);
 }
 bool has_root(ikd_tree_t t) { return (t->h_root->child.z & (int)LK_HASL) != 0; }
 long long live_count(ikd_tree_t t) {
@@ -714,10 +720,11 @@ int rebuild_listed(ikd_tree_t t, const std::vector<int4>& hinfo, const std::vect
     return IKD_OK;
 }
 
+// Returns with the header and root summary fresh (refresh()).
 int rebuild_tops(ikd_tree_t t) {
     hipStream_t st = t->st;
     for (int iter = 0; iter < 8; ++iter) {
-        CHK(sync_hdr(t));
+        CHK(refresh(t));
         int ntop = t->h_hdr->top_count;
         if (ntop == 0) return IKD_OK;
         int* top;
@@ -744,15 +751,19 @@ int rebuild_tops(ikd_tree_t t) {
 
 // Rebuild the whole tree from its live points when the pool holds mostly
 // garbage or the tree got deeper than the traversal stacks allow.
-int compact_if_needed(ikd_tree_t t, bool force) {
-    CHK(read_root(t));
-    CHK(sync_hdr(t));
+// fresh: the header and root summary were refreshed after the last change.
+// Returns with both fresh.
+int compact_if_needed(ikd_tree_t t, bool force, bool fresh = false) {
+    if (!fresh) CHK(refresh(t));
     long long live = live_count(t);
     bool deep = t->max_depth + 2 > MAX_DEPTH;
     bool sparse = t->pool_used > 4 * live + (1ll << 22);
     if (!force && !deep && !sparse) return IKD_OK;
     t->ctr.compactions += 1;
-    if (!has_root(t)) return reset_pool(t);
+    if (!has_root(t)) {
+        CHK(reset_pool(t));
+        return refresh(t);
+    }
     float4* pts;
     int* segof;
     std::vector<int> roots = {ROOT};
@@ -763,7 +774,7 @@ int compact_if_needed(ikd_tree_t t, bool force) {
     CHK(ensure(t, B_PTS, live, &keep));
     HIPCHK(hipMemcpyAsync(keep, pts, live * sizeof(float4), hipMemcpyDeviceToDevice, t->st));
     CHK(reset_pool(t));
-    if (live == 0) return IKD_OK;
+    if (live == 0) return refresh(t);
     int* segof0;
     CHK(ensure(t, B_SEGOF, live, &segof0));
     launch_fill_i32(segof0, 0, live, t->st);
@@ -778,7 +789,7 @@ int compact_if_needed(ikd_tree_t t, bool force) {
     CHK(run_build(t, keep, segof0, (int)live, dseg, dsegd, 1, (int)live));
     hipLaunchKernelGGL(k_link_bits, dim3(1), dim3(1), 0, t->st, t->pool.hot, SENTINEL, LK_HASL, LK_TDL);
     t->rebuilds += 1;
-    return sync_hdr(t);
+    return refresh(t);
 }
 
 // ---------------------------------------------------------------- insert
@@ -836,7 +847,9 @@ int insert_points(ikd_tree_t t, const float4* pts, int m, const int* extra_seeds
     int* seeds;
     CHK(ensure(t, B_SEEDS, S, &seeds));
     hipLaunchKernelGGL(k_seeds_from_attach, dim3(nblk(S)), dim3(BS), 0, st, attach, t->d_hdr, S, seeds);
-    CHK(sync_hdr(t));
+    // no wait for the exact count: the insert took at most the 4m + 16 nodes
+    // reserved above, and the refit buffers only need an upper bound
+    t->pool_used = std::min(t->cap, t->pool_used + 4ll * m + 16);
     CHK(refit_begin(t));
     CHK(refit_mark(t, seeds, nullptr, S, 0));
     if (extra_bound > 0) CHK(refit_mark(t, extra_seeds, n_extra_dev, extra_bound, 0));
@@ -966,9 +979,8 @@ int add_points_ds_chunk(ikd_tree_t t, const float* d_xyz, const uint32_t* d_ids,
     launch_fill_i32(counts, 0, 2, st);
     CHK(exscan_i32(t, insfl, scan, nv));
     launch_compact_pts(insfl, scan, inspt, nv, t->d_hdr, cpts, counts + 1, st);
-    int hc = 0;
-    HIPCHK(hipMemcpyAsync(&hc, counts + 1, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(wait_stream(t));
+    // the fold counted the voxels that keep a point (the compacted count)
+    const int hc = t->h_hdr->ins_count;
     return insert_points(t, cpts, hc, dseeds, nullptr, ndel);
 }
 
@@ -1357,8 +1369,7 @@ int ikd_add_points_device(ikd_tree_t t, const float* d_xyz, const uint32_t* d_id
     int added = 0;
     int rc = add_points_dev(t, d_xyz, d_ids, (int)n, downsample_on, &added);
     if (out_added) *out_added = added;
-    if (rc != IKD_OK) return rc;
-    return read_root(t);
+    return rc;  // add_points_dev ends with the root summary fresh
 }
 
 int ikd_add_points(ikd_tree_t t, const float* xyz, const uint32_t* ids, int64_t n, int downsample_on,
@@ -1395,8 +1406,7 @@ int ikd_delete_points(ikd_tree_t t, const float* xyz, int64_t n) {
     }
     CHK(refit(t, seeds, (int)n, 0));
     CHK(rebuild_tops(t));
-    CHK(compact_if_needed(t, false));
-    return read_root(t);
+    return compact_if_needed(t, false, true);
 }
 
 int ikd_delete_point_boxes(ikd_tree_t t, const float* boxes, int64_t nb, int32_t* out_deleted) {
@@ -1410,7 +1420,6 @@ int ikd_delete_point_boxes(ikd_tree_t t, const float* boxes, int64_t nb, int32_t
     CHK(upload(t, B_IO4, boxes, 6 * nb, &db));
     CHK(delete_boxes_dev(t, db, (int)nb, 0));
     CHK(compact_if_needed(t, false));
-    CHK(read_root(t));
     if (out_deleted) *out_deleted = (int32_t)(before - live_count(t));
     return IKD_OK;
 }

@@ -30,6 +30,7 @@ struct DevHdr {
     int added;          // downsample: Add_Points return value of the batch
     int defer_count;    // deferred rebuilds listed (kept across calls)
     int front_over;     // a box-deletion frontier outgrew its buffer
+    int ins_count;      // downsample: voxels whose fold keeps a point to insert
 };
 constexpr int DEFER_CAP = 4096;  // deferred-rebuild list entries
 

@@ -235,6 +235,7 @@ __global__ void k_voxel_fold(Pool pool, const float4* __restrict__ pts, const ui
     }
     if (split != 0x7fffffff) atomicMin(&hdr->split_index, split);
     atomicAdd(&hdr->added, added);
+    if (have_new) atomicAdd(&hdr->ins_count, 1);
     del_flag[v] = (have_new && n0 > 0) ? 1 : 0;
     ins_flag[v] = have_new ? 1 : 0;
 #pragma unroll
@@ -509,6 +510,7 @@ __global__ __launch_bounds__(64) void k_voxel_fold_team(Pool pool, const float4*
     if (valid && tl == 0) {
         if (split != 0x7fffffff) atomicMin(&hdr->split_index, split);
         atomicAdd(&hdr->added, added);
+        if (have_new) atomicAdd(&hdr->ins_count, 1);
         del_flag[v] = (have_new && n0 > 0) ? 1 : 0;
         ins_flag[v] = have_new ? 1 : 0;
 #pragma unroll
