@@ -581,8 +581,10 @@ int run_build(ikd_tree_t t, const float4* pts, const int* seg_of, int m, const i
 // launch costs, so it only pays while every level fits one pass of 1024 threads.
 constexpr long long ENUM_BLOCK_MAX = 1024;
 
+// min_depth: the smallest depth among the roots (their subtrees end by the
+// tree's deepest level, so max_depth - min_depth + 1 levels cover them).
 int enumerate_live(ikd_tree_t t, const std::vector<int>& roots, const std::vector<long long>& live,
-                   long long total_nodes, long long total_live, float4** out_pts, int** out_seg) {
+                   long long total_nodes, long long total_live, int min_depth, float4** out_pts, int** out_seg) {
     hipStream_t st = t->st;
     int R = (int)roots.size();
     long long cap = std::max<long long>(total_nodes, 1);
@@ -604,7 +606,7 @@ int enumerate_live(ikd_tree_t t, const std::vector<int>& roots, const std::vecto
     int init[3] = {R, 0, 0};
     HIPCHK(hipMemcpyAsync(t->d_hdr->front3, init, sizeof(init), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, t->d_hdr, 2);
-    int levels = t->max_depth + 2;
+    const int levels = std::max(t->max_depth - std::max(min_depth, 0), 0) + 2;
     if (cap <= ENUM_BLOCK_MAX) {
         // small subtrees: one workgroup walks all levels
         launch_enum_preorder_block(t->pool, f0, f1, R, pts, segof, t->d_hdr, (int)cap, std::max(n, 1), levels, st);
@@ -677,6 +679,7 @@ int rebuild_listed(ikd_tree_t t, const std::vector<int4>& hinfo, const std::vect
     const int ntop = (int)hinfo.size();
     std::vector<int> roots;
     std::vector<long long> lives;
+    int min_depth = INT_MAX;
     std::vector<int4> segs;
     std::vector<int> segd;
     long long total_nodes = 0, total_live = 0;
@@ -696,6 +699,7 @@ int rebuild_listed(ikd_tree_t t, const std::vector<int4>& hinfo, const std::vect
         segd.push_back(depth);
         roots.push_back(u);
         lives.push_back(live);
+        min_depth = std::min(min_depth, depth);
         total_nodes += hbefore[i];
         total_live += live;
         t->ctr.rebuilt_points += live;
@@ -704,7 +708,7 @@ int rebuild_listed(ikd_tree_t t, const std::vector<int4>& hinfo, const std::vect
     if (!roots.empty()) {
         float4* pts;
         int* segof;
-        CHK(enumerate_live(t, roots, lives, total_nodes, total_live, &pts, &segof));
+        CHK(enumerate_live(t, roots, lives, total_nodes, total_live, min_depth, &pts, &segof));
         int4* dseg;
         int* dsegd;
         CHK(ensure(t, B_ISEG, (long long)segs.size(), &dseg));
@@ -768,7 +772,7 @@ int compact_if_needed(ikd_tree_t t, bool force, bool fresh = false) {
     int* segof;
     std::vector<int> roots = {ROOT};
     std::vector<long long> lives = {live};
-    CHK(enumerate_live(t, roots, lives, t->h_root->cnt.x, live, &pts, &segof));
+    CHK(enumerate_live(t, roots, lives, t->h_root->cnt.x, live, 0, &pts, &segof));
     // move the points out of the scratch slot the build reuses
     float4* keep;
     CHK(ensure(t, B_PTS, live, &keep));
@@ -1481,7 +1485,7 @@ int ikd_flatten(ikd_tree_t t, float* out_xyz, uint32_t* out_ids, int64_t capacit
     int* segof;
     std::vector<int> roots = {ROOT};
     std::vector<long long> lives = {live};
-    CHK(enumerate_live(t, roots, lives, t->h_root->cnt.x, live, &pts, &segof));
+    CHK(enumerate_live(t, roots, lives, t->h_root->cnt.x, live, 0, &pts, &segof));
     float* dx;
     uint32_t* di;
     CHK(ensure(t, B_IO0, 3 * live, &dx));
