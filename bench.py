@@ -73,7 +73,7 @@ def main():
     ap.add_argument("--trees", type=int, default=64, help="trees in the multi-tree batching phase (0 skips it)")
     ap.add_argument("--robots", type=int, default=16, help="maps in the serving phase (0 skips it)")
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="simulated arrival window of the serving phase")
-    ap.add_argument("--serve-defer", type=int, default=2_000,
+    ap.add_argument("--serve-defer", type=int, default=1_000,
                     help="serving: subtrees above this many live points are rebuilt between requests")
     ap.add_argument("--serve-slots", type=int, default=8, help="serving: requests on the device at once")
     ap.add_argument("--serve-chunk", type=int, default=100_000,

@@ -40,7 +40,7 @@ EXPORTS = (
     "ikd_nearest_search_device", "ikd_nearest_search_counters", "ikd_box_search", "ikd_radius_search",
     "ikd_add_points", "ikd_add_points_device", "ikd_delete_points", "ikd_delete_point_boxes", "ikd_size",
     "ikd_validnum", "ikd_tree_range", "ikd_root_alpha", "ikd_flatten", "ikd_synchronize", "ikd_stream",
-    "ikd_pool_stats", "ikd_get_counters", "ikd_set_tuning", "ikd_dump_nodes", "ikd_last_error",
+    "ikd_pool_stats", "ikd_get_counters", "ikd_acquire_removed_points", "ikd_set_tuning", "ikd_dump_nodes", "ikd_last_error",
     "ikd_batch_create", "ikd_batch_destroy", "ikd_batch_nearest_search_device", "ikd_batch_synchronize",
     "ikd_batch_stream", "ikd_batch_set_tuning", "ikd_batch_last_error", "ikd_batch_radius_search_device",
     "ikd_now_ns", "ikd_server_create", "ikd_server_add_map", "ikd_server_submit", "ikd_server_wait",
@@ -53,6 +53,7 @@ IKD_TUNE_LANES_PER_QUERY = 3
 IKD_TUNE_SMALL_BUILD = 4
 IKD_TUNE_DEFER_REBUILD = 5
 IKD_TUNE_FOLD_LANES = 6
+IKD_TUNE_RECORD_REMOVED = 7
 
 
 class Counters(ctypes.Structure):
@@ -115,6 +116,7 @@ def load_library(path: str | None = None):
         "ikd_stream": [vp, ctypes.POINTER(vp)],
         "ikd_pool_stats": [vp, vp, vp, vp, vp],
         "ikd_get_counters": [vp, vp],
+        "ikd_acquire_removed_points": [vp, vp, vp, i64, vp],
         "ikd_set_tuning": [vp, i32, i32],
         "ikd_dump_nodes": [vp, vp, i64, vp],
         "ikd_rebuild_deferred": [vp, i64, vp],
@@ -336,6 +338,21 @@ class KD_TREE:
                                              ctypes.byref(reb)))
         return {"used_nodes": used.value, "capacity_nodes": cap.value, "max_depth": depth.value,
                 "rebuilds": reb.value}
+
+    def acquire_removed_points(self):
+        """acquire_removed_points (ikd_Tree.cpp:558-571): the points rebuilds
+        removed since the last call (recorded while IKD_TUNE_RECORD_REMOVED is
+        1), as (xyz, ids); clears the record."""
+        n = ctypes.c_int64(0)
+        rc = self._lib.ikd_acquire_removed_points(self._h, None, None, 0, ctypes.byref(n))
+        if rc not in (IKD_OK, IKD_ERR_CAPACITY):
+            self._check(rc)
+        cnt = n.value
+        xyz = np.zeros((max(cnt, 1), 3), np.float32)
+        ids = np.zeros(max(cnt, 1), np.uint32)
+        if cnt > 0:
+            self._check(self._lib.ikd_acquire_removed_points(self._h, _ptr(xyz), _ptr(ids), cnt, ctypes.byref(n)))
+        return xyz[:cnt], ids[:cnt]
 
     def counters(self):
         c = Counters()

@@ -68,6 +68,7 @@ __global__ void k_init_pool(Pool pool, DevHdr* hdr) {
     DevHdr z = {};
     z.pair_used = 2;
     z.split_index = INT_MAX;
+    z.removed_count = hdr->removed_count;  // the removed-point record outlives the nodes
     *hdr = z;
 }
 
@@ -161,6 +162,9 @@ struct ikd_tree_s {
     int knn_lanes = -1;   // lanes per query: -1 picks from the batch size; 0 persistent; else 1, 2, 4, 8, 16
     int small_build = 1;  // build subtrees of <= SMALL_MAX points with one workgroup each
     int fold_lanes = 0;   // lanes per voxel in the downsample fold: 0 picks by map size; else 1, 4, 8, 16
+    int record_removed = 0;     // 1: rebuilds record the deleted points they drop (acquire_removed_points)
+    float4* removed = nullptr;  // that record (x, y, z, id bits), hdr->removed_count entries
+    long long removed_cap = 0;
     hipEvent_t ev = nullptr;  // orders batch launches against this tree's stream
     ikd_counters_t ctr{};     // event counts since creation (ikd_get_counters)
     hipMemPool_t mp = nullptr;  // stream-ordered allocations (null: hipMalloc / hipFree)
@@ -581,10 +585,37 @@ int run_build(ikd_tree_t t, const float4* pts, const int* seg_of, int m, const i
 // launch costs, so it only pays while every level fits one pass of 1024 threads.
 constexpr long long ENUM_BLOCK_MAX = 1024;
 
+// Room for `extra` more records beyond the current count (fresh in h_hdr):
+// the record survives the growth, unlike scratch.
+int reserve_removed(ikd_tree_t t, long long extra) {
+    const long long have = t->h_hdr->removed_count;
+    const long long need = have + extra;
+    if (need <= t->removed_cap) return IKD_OK;
+    if (need >= (1ll << 31)) {
+        t->err = "removed-point record would exceed 2^31 entries";
+        return IKD_ERR_NOMEM;
+    }
+    const long long ncap = std::max(need + need / 2, 2 * t->removed_cap);
+    float4* np = nullptr;
+    if (dev_alloc(t, (void**)&np, (size_t)ncap * sizeof(float4)) != hipSuccess) {
+        (void)hipGetLastError();
+        t->err = "device allocation of the removed-point record failed";
+        return IKD_ERR_NOMEM;
+    }
+    if (have > 0) HIPCHK(hipMemcpyAsync(np, t->removed, have * sizeof(float4), hipMemcpyDeviceToDevice, t->st));
+    dev_free(t, t->removed);
+    t->removed = np;
+    t->removed_cap = ncap;
+    return IKD_OK;
+}
+
 // min_depth: the smallest depth among the roots (their subtrees end by the
 // tree's deepest level, so max_depth - min_depth + 1 levels cover them).
+// record: a rebuild's flatten (DELETE_POINTS_REC, cpp:638); plain flattens
+// (NOT_RECORD) pass false.
 int enumerate_live(ikd_tree_t t, const std::vector<int>& roots, const std::vector<long long>& live,
-                   long long total_nodes, long long total_live, int min_depth, float4** out_pts, int** out_seg) {
+                   long long total_nodes, long long total_live, int min_depth, bool record, float4** out_pts,
+                   int** out_seg) {
     hipStream_t st = t->st;
     int R = (int)roots.size();
     long long cap = std::max<long long>(total_nodes, 1);
@@ -607,14 +638,24 @@ int enumerate_live(ikd_tree_t t, const std::vector<int>& roots, const std::vecto
     HIPCHK(hipMemcpyAsync(t->d_hdr->front3, init, sizeof(init), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, t->d_hdr, 2);
     const int levels = std::max(t->max_depth - std::max(min_depth, 0), 0) + 2;
+    float4* rec = nullptr;
+    int rec_cap = 0;
+    if (record && t->record_removed) {
+        // every node of the subtrees may be a deleted one
+        CHK(reserve_removed(t, cap));
+        rec = t->removed;
+        rec_cap = (int)t->removed_cap;
+    }
     if (cap <= ENUM_BLOCK_MAX) {
         // small subtrees: one workgroup walks all levels
-        launch_enum_preorder_block(t->pool, f0, f1, R, pts, segof, t->d_hdr, (int)cap, std::max(n, 1), levels, st);
+        launch_enum_preorder_block(t->pool, f0, f1, R, pts, segof, t->d_hdr, (int)cap, std::max(n, 1), levels, rec,
+                                   rec_cap, st);
     } else {
         for (int L = 0; L < levels; ++L) {
             int4* fin = (L & 1) ? f1 : f0;
             int4* fout = (L & 1) ? f0 : f1;
-            launch_enum_preorder(t->pool, fin, L, fout, pts, segof, t->d_hdr, (int)cap, std::max(n, 1), st);
+            launch_enum_preorder(t->pool, fin, L, fout, pts, segof, t->d_hdr, (int)cap, std::max(n, 1), rec, rec_cap,
+                                 st);
         }
     }
     CHK(sync_hdr(t));  // also keeps h and init alive until the copies ran
@@ -708,7 +749,7 @@ int rebuild_listed(ikd_tree_t t, const std::vector<int4>& hinfo, const std::vect
     if (!roots.empty()) {
         float4* pts;
         int* segof;
-        CHK(enumerate_live(t, roots, lives, total_nodes, total_live, min_depth, &pts, &segof));
+        CHK(enumerate_live(t, roots, lives, total_nodes, total_live, min_depth, true, &pts, &segof));
         int4* dseg;
         int* dsegd;
         CHK(ensure(t, B_ISEG, (long long)segs.size(), &dseg));
@@ -772,7 +813,7 @@ int compact_if_needed(ikd_tree_t t, bool force, bool fresh = false) {
     int* segof;
     std::vector<int> roots = {ROOT};
     std::vector<long long> lives = {live};
-    CHK(enumerate_live(t, roots, lives, t->h_root->cnt.x, live, 0, &pts, &segof));
+    CHK(enumerate_live(t, roots, lives, t->h_root->cnt.x, live, 0, true, &pts, &segof));
     // move the points out of the scratch slot the build reuses
     float4* keep;
     CHK(ensure(t, B_PTS, live, &keep));
@@ -1197,6 +1238,8 @@ int ikd_create(float delete_param, float balance_param, float box_length, ikd_tr
         ikd_destroy(t);
         return IKD_ERR_HIP;
     }
+    // a zero header first: the pool's initialisation keeps only the record count
+    launch_fill_i32(reinterpret_cast<int*>(t->d_hdr), 0, sizeof(DevHdr) / sizeof(int), t->st);
     int rc = reset_pool(t);
     if (rc == IKD_OK) rc = sync_hdr(t);
     if (rc == IKD_OK) rc = read_root(t);
@@ -1213,6 +1256,7 @@ int ikd_destroy(ikd_tree_t t) {
     if (t->st) {
         if (t->cap > 0) free_pool(t, &t->pool);
         for (auto& b : t->b) dev_free(t, b.p);
+        dev_free(t, t->removed);
         hipStreamSynchronize(t->st);
     }
     if (t->d_hdr) hipFree(t->d_hdr);
@@ -1485,7 +1529,7 @@ int ikd_flatten(ikd_tree_t t, float* out_xyz, uint32_t* out_ids, int64_t capacit
     int* segof;
     std::vector<int> roots = {ROOT};
     std::vector<long long> lives = {live};
-    CHK(enumerate_live(t, roots, lives, t->h_root->cnt.x, live, 0, &pts, &segof));
+    CHK(enumerate_live(t, roots, lives, t->h_root->cnt.x, live, 0, false, &pts, &segof));
     float* dx;
     uint32_t* di;
     CHK(ensure(t, B_IO0, 3 * live, &dx));
@@ -1664,6 +1708,9 @@ int ikd_set_tuning(ikd_tree_t t, int key, int value) {
             if (value < -1 || value > 16 || (value > 0 && (value & (value - 1)))) return IKD_ERR_ARG;
             t->knn_lanes = value;
             return IKD_OK;
+        case IKD_TUNE_RECORD_REMOVED:
+            t->record_removed = value ? 1 : 0;
+            return IKD_OK;
         case IKD_TUNE_FOLD_LANES:
             if (!(value == 0 || value == 1 || value == 4 || value == 8 || value == 16)) return IKD_ERR_ARG;
             t->fold_lanes = value;
@@ -1681,6 +1728,26 @@ int ikd_pool_stats(ikd_tree_t t, int64_t* used, int64_t* capacity, int32_t* max_
     if (max_depth) *max_depth = t->max_depth;
     if (rebuilds) *rebuilds = t->rebuilds;
     return IKD_OK;
+}
+
+int ikd_acquire_removed_points(ikd_tree_t t, float* out_xyz, uint32_t* out_ids, int64_t capacity, int64_t* out_n) {
+    CHECK_T();
+    if (!out_n || capacity < 0 || (capacity > 0 && !out_xyz)) return IKD_ERR_ARG;
+    CHK(sync_hdr(t));
+    const long long n = t->h_hdr->removed_count;
+    *out_n = n;
+    if (n > capacity) return IKD_ERR_CAPACITY;
+    if (n > 0) {
+        float* dx;
+        uint32_t* di;
+        CHK(ensure(t, B_IO0, 3 * n, &dx));
+        CHK(ensure(t, B_IO1, n, &di));
+        hipLaunchKernelGGL(k_unpack_points, dim3(nblk(n)), dim3(BS), 0, t->st, t->removed, (int)n, dx, di);
+        HIPCHK(hipMemcpyAsync(out_xyz, dx, 3 * n * sizeof(float), hipMemcpyDeviceToHost, t->st));
+        if (out_ids) HIPCHK(hipMemcpyAsync(out_ids, di, n * sizeof(uint32_t), hipMemcpyDeviceToHost, t->st));
+    }
+    launch_fill_i32(&t->d_hdr->removed_count, 0, 1, t->st);
+    return sync_hdr(t);
 }
 
 int ikd_get_counters(ikd_tree_t t, ikd_counters_t* out) {

@@ -31,7 +31,9 @@ struct DevHdr {
     int defer_count;    // deferred rebuilds listed (kept across calls)
     int front_over;     // a box-deletion frontier outgrew its buffer
     int ins_count;      // downsample: voxels whose fold keeps a point to insert
+    int removed_count;  // points rebuilds removed since the last acquire (kept across calls)
 };
+static_assert(sizeof(DevHdr) % sizeof(int) == 0, "the header is cleared as ints");
 constexpr int DEFER_CAP = 4096;  // deferred-rebuild list entries
 
 struct Pool {
@@ -109,10 +111,13 @@ void launch_find_top(Pool pool, const int* viol, int bound, DevHdr* hdr, int* to
 void launch_clear_viol(Pool pool, const int* viol, int bound, DevHdr* hdr, hipStream_t s);
 
 // ---------------------------------------------------------------- enumerate
+// rec (nullable): points of deleted nodes that the downsample did not delete
+// are appended at hdr->removed_count (rec_cap entries).
 void launch_enum_preorder(Pool pool, const int4* front, int lvl, int4* front_next, float4* out_pts, int* out_seg,
-                          DevHdr* hdr, int bound, int out_cap, hipStream_t s);
+                          DevHdr* hdr, int bound, int out_cap, float4* rec, int rec_cap, hipStream_t s);
 void launch_enum_preorder_block(Pool pool, int4* f0, int4* f1, int n_roots, float4* out_pts, int* out_seg,
-                                DevHdr* hdr, int bound, int out_cap, int max_levels, hipStream_t s);
+                                DevHdr* hdr, int bound, int out_cap, int max_levels, float4* rec, int rec_cap,
+                                hipStream_t s);
 
 // ---------------------------------------------------------------- insert
 void launch_insert_descend(Pool pool, const float4* pts, int n, unsigned long long* keys, uint32_t* vals,

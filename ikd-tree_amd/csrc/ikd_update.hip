@@ -314,57 +314,115 @@ void launch_clear_viol(Pool pool, const int* viol, int bound, DevHdr* hdr, hipSt
 // ======================================================= enumerate (flatten)
 // Live points of several subtrees in the reference's flatten order (own
 // point, left subtree, right subtree; ikd_Tree.cpp:1326-1333), one depth
-// level per launch.  A frontier entry (node, base, segment) knows where the
-// first live point of its subtree goes: its own point takes base when live,
-// the left subtree starts after it and the right subtree after the left's
-// live count (TreeSize - invalid_point_num of the left child).  Tree-deleted
-// subtrees hold no live points and are not entered.  Three rotating frontier
-// counters let a level reset the counter of the level after next.
+// level per launch.  A frontier entry (node, base, segment, flags) knows where
+// the first live point of its subtree goes: its own point takes base when
+// live, the left subtree starts after it and the right subtree after the
+// left's live count (TreeSize - invalid_point_num of the left child).
+// Tree-deleted subtrees hold no live points: without a record they are not
+// entered; with one (rec) they are walked too, flags bit 0 marking a deleted
+// subtree and bit 1 a downsample-deleted one, and every deleted point the
+// downsample did not delete is recorded, as flatten's DELETE_POINTS_REC mode
+// does during Rebuild (cpp:1338-1341, 638).  Three rotating frontier counters
+// let a level reset the counter of the level after next.
+struct EnumStep {
+    int pushes;       // children pushed
+    int4 child[2];
+};
+
+__device__ __forceinline__ void enum_record(const Pool& pool, int u, DevHdr* hdr, float4* rec, int rec_cap) {
+    const int pos = atomicAdd(&hdr->removed_count, 1);
+    if (pos < rec_cap) {
+        const float4 p = *reinterpret_cast<const float4*>(pool.hot[u].f);
+        rec[pos] = make_float4(p.x, p.y, p.z, bitsf(pool.ids[u]));
+    } else {
+        atomicOr(&hdr->overflow, 32);
+    }
+}
+
+// One frontier entry: writes its own live point (returns 1) or records it,
+// and lists the children to push.
+__device__ __forceinline__ int enum_visit(const Pool& pool, int4 e, float4* __restrict__ out_pts,
+                                          int* __restrict__ out_seg, DevHdr* hdr, int out_cap, float4* rec,
+                                          int rec_cap, EnumStep& st) {
+    st.pushes = 0;
+    const int u = e.x;
+    const uint32_t m = pool.meta[u];
+    const bool dead = (e.w & 1) || (m & M_TDEL);
+    const bool dsub = (e.w & 2) || (m & M_TDS);
+    if (dead && !rec) return 0;
+    const uint32_t lk = load_link(pool.hot, u);
+    const int pair = (int)lk_pair(lk);
+    if (dead) {
+        // a deleted subtree: every node is deleted; walk it for the record
+        if (!(m & M_PDS) && !(e.w & 2)) enum_record(pool, u, hdr, rec, rec_cap);
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+            if (lk & lk_has(s)) st.child[st.pushes++] = make_int4(2 * pair + s, 0, e.z, 1 | (dsub ? 2 : 0));
+        return 0;
+    }
+    int next = e.y, wrote = 0;
+    if (!(m & M_PDEL)) {
+        if (next < out_cap) {
+            const float4 p = *reinterpret_cast<const float4*>(pool.hot[u].f);
+            out_pts[next] = make_float4(p.x, p.y, p.z, bitsf(pool.ids[u]));
+            out_seg[next] = e.z;
+            wrote = 1;
+        } else {
+            atomicOr(&hdr->overflow, 32);
+        }
+        ++next;
+    } else if (rec && !(m & M_PDS)) {
+        enum_record(pool, u, hdr, rec, rec_cap);
+    }
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        if (!(lk & lk_has(s))) continue;
+        const int c = 2 * pair + s;
+        if (lk & lk_td(s)) {
+            if (rec) st.child[st.pushes++] = make_int4(c, 0, e.z, 1);
+            continue;
+        }
+        const int4 cc = pool.cnt[c];
+        st.child[st.pushes++] = make_int4(c, next, e.z, 0);
+        next += cc.x - cc.y;
+    }
+    return wrote;
+}
+
 __global__ void k_enum_preorder(Pool pool, const int4* __restrict__ front, int lvl, int4* __restrict__ front_next,
                                 float4* __restrict__ out_pts, int* __restrict__ out_seg, DevHdr* hdr, int bound,
-                                int out_cap) {
+                                int out_cap, float4* __restrict__ rec, int rec_cap) {
     if (blockIdx.x == 0 && threadIdx.x == 0) hdr->front3[(lvl + 2) % 3] = 0;
     const int nf = min(hdr->front3[lvl % 3], bound);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nf; i += gridDim.x * blockDim.x) {
-        const int4 e = front[i];
-        const int u = e.x;
-        const uint32_t m = pool.meta[u];
-        if (m & M_TDEL) continue;
-        int next = e.y;
-        if (!(m & M_PDEL)) {
-            if (next < out_cap) {
-                const float4 p = *reinterpret_cast<const float4*>(pool.hot[u].f);
-                out_pts[next] = make_float4(p.x, p.y, p.z, bitsf(pool.ids[u]));
-                out_seg[next] = e.z;
-                atomicAdd(&hdr->rec_count, 1);
-            } else {
-                atomicOr(&hdr->overflow, 32);
-            }
-            ++next;
-        }
-        const uint32_t lk = load_link(pool.hot, u);
-        const int pair = (int)lk_pair(lk);
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            if (!(lk & lk_has(s)) || (lk & lk_td(s))) continue;
-            const int c = 2 * pair + s;
-            const int4 cc = pool.cnt[c];
+        EnumStep st;
+        if (enum_visit(pool, front[i], out_pts, out_seg, hdr, out_cap, rec, rec_cap, st))
+            atomicAdd(&hdr->rec_count, 1);
+        for (int c = 0; c < st.pushes; ++c) {
             const int pos = atomicAdd(&hdr->front3[(lvl + 1) % 3], 1);
-            if (pos < bound) front_next[pos] = make_int4(c, next, e.z, 0);
+            if (pos < bound) front_next[pos] = st.child[c];
             else atomicOr(&hdr->overflow, 32);
-            next += cc.x - cc.y;
         }
     }
 }
 
+void launch_enum_preorder(Pool pool, const int4* front, int lvl, int4* front_next, float4* out_pts, int* out_seg,
+                          DevHdr* hdr, int bound, int out_cap, float4* rec, int rec_cap, hipStream_t s) {
+    // a grid-stride loop: the frontier size is known only on the device
+    const int blocks = std::min(blocks_for(bound > 0 ? bound : 1, 256), 2048);
+    hipLaunchKernelGGL(k_enum_preorder, dim3(blocks), dim3(256), 0, s, pool, front, lvl, front_next, out_pts, out_seg,
+                       hdr, bound, out_cap, rec, rec_cap);
+}
+
 // The same walk by one workgroup over every level, for subtrees of a few
-// thousand nodes: the levels are separated by workgroup barriers instead of
+// hundred nodes: the levels are separated by workgroup barriers instead of
 // launches, and the walk ends as soon as a level is empty.  f0 holds the
 // n_roots root entries; f0 and f1 alternate as frontiers of `bound` entries.
 __global__ __launch_bounds__(1024) void k_enum_preorder_block(Pool pool, int4* __restrict__ f0, int4* __restrict__ f1,
                                                               int n_roots, float4* __restrict__ out_pts,
                                                               int* __restrict__ out_seg, DevHdr* hdr, int bound,
-                                                              int out_cap, int max_levels) {
+                                                              int out_cap, int max_levels, float4* __restrict__ rec,
+                                                              int rec_cap) {
     __shared__ int next_count, written;
     if (threadIdx.x == 0) written = 0;
     int nf = min(n_roots, bound);
@@ -376,33 +434,12 @@ __global__ __launch_bounds__(1024) void k_enum_preorder_block(Pool pool, int4* _
         __syncthreads();
         int mine = 0;
         for (int i = threadIdx.x; i < nf; i += blockDim.x) {
-            const int4 e = fin[i];
-            const int u = e.x;
-            const uint32_t m = pool.meta[u];
-            if (m & M_TDEL) continue;
-            int next = e.y;
-            if (!(m & M_PDEL)) {
-                if (next < out_cap) {
-                    const float4 p = *reinterpret_cast<const float4*>(pool.hot[u].f);
-                    out_pts[next] = make_float4(p.x, p.y, p.z, bitsf(pool.ids[u]));
-                    out_seg[next] = e.z;
-                    ++mine;
-                } else {
-                    atomicOr(&hdr->overflow, 32);
-                }
-                ++next;
-            }
-            const uint32_t lk = load_link(pool.hot, u);
-            const int pair = (int)lk_pair(lk);
-#pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                if (!(lk & lk_has(s)) || (lk & lk_td(s))) continue;
-                const int c = 2 * pair + s;
-                const int4 cc = pool.cnt[c];
+            EnumStep st;
+            mine += enum_visit(pool, fin[i], out_pts, out_seg, hdr, out_cap, rec, rec_cap, st);
+            for (int c = 0; c < st.pushes; ++c) {
                 const int pos = atomicAdd(&next_count, 1);
-                if (pos < bound) fout[pos] = make_int4(c, next, e.z, 0);
+                if (pos < bound) fout[pos] = st.child[c];
                 else atomicOr(&hdr->overflow, 32);
-                next += cc.x - cc.y;
             }
         }
         if (mine) atomicAdd(&written, mine);
@@ -420,17 +457,10 @@ __global__ __launch_bounds__(1024) void k_enum_preorder_block(Pool pool, int4* _
 }
 
 void launch_enum_preorder_block(Pool pool, int4* f0, int4* f1, int n_roots, float4* out_pts, int* out_seg,
-                                DevHdr* hdr, int bound, int out_cap, int max_levels, hipStream_t s) {
+                                DevHdr* hdr, int bound, int out_cap, int max_levels, float4* rec, int rec_cap,
+                                hipStream_t s) {
     hipLaunchKernelGGL(k_enum_preorder_block, dim3(1), dim3(1024), 0, s, pool, f0, f1, n_roots, out_pts, out_seg, hdr,
-                       bound, out_cap, max_levels);
-}
-
-void launch_enum_preorder(Pool pool, const int4* front, int lvl, int4* front_next, float4* out_pts, int* out_seg,
-                          DevHdr* hdr, int bound, int out_cap, hipStream_t s) {
-    // a grid-stride loop: the frontier size is known only on the device
-    const int blocks = std::min(blocks_for(bound > 0 ? bound : 1, 256), 2048);
-    hipLaunchKernelGGL(k_enum_preorder, dim3(blocks), dim3(256), 0, s, pool, front, lvl, front_next, out_pts, out_seg,
-                       hdr, bound, out_cap);
+                       bound, out_cap, max_levels, rec, rec_cap);
 }
 
 // ================================================================== insert

@@ -48,6 +48,7 @@ public:
 
     KD_TREE(float delete_param = 0.5, float balance_param = 0.6, float box_length = 0.2) {
         check(ikd_create(delete_param, balance_param, box_length, &h_), "KD_TREE");
+        check(ikd_set_tuning(h_, IKD_TUNE_RECORD_REMOVED, 1), "KD_TREE");
     }
     ~KD_TREE() {
         if (h_) ikd_destroy(h_);
@@ -204,6 +205,19 @@ public:
         std::vector<uint32_t> ids(n);
         check(ikd_flatten(h_, xyz.data(), ids.data(), n, &n), "flatten");
         for (int64_t i = 0; i < n; ++i) Storage.push_back(payload_[ids[i]]);
+    }
+
+    // acquire_removed_points, ikd_Tree.cpp:558-571 (the facade records from
+    // construction on, as the reference does)
+    void acquire_removed_points(PointVector& removed_points) {
+        int64_t n = 0;
+        int rc = ikd_acquire_removed_points(h_, nullptr, nullptr, 0, &n);
+        if (rc != IKD_OK && rc != IKD_ERR_CAPACITY) check(rc, "acquire_removed_points");
+        if (n == 0) return;
+        std::vector<float> xyz(3 * n);
+        std::vector<uint32_t> ids(n);
+        check(ikd_acquire_removed_points(h_, xyz.data(), ids.data(), n, &n), "acquire_removed_points");
+        for (int64_t i = 0; i < n; ++i) removed_points.push_back(payload_[ids[i]]);
     }
 
     ikd_tree_t handle() const { return h_; }

@@ -145,6 +145,14 @@ int ikd_stream(ikd_tree_t t, void** out_stream);
  * capacity, deepest node, rebuilds run so far. */
 int ikd_pool_stats(ikd_tree_t t, int64_t* used_nodes, int64_t* capacity_nodes, int32_t* max_depth,
                    int64_t* rebuilds);
+/* acquire_removed_points, ikd_Tree.cpp:558-571: the points rebuilds removed
+ * since the last call (recorded while IKD_TUNE_RECORD_REMOVED is 1), then the
+ * record is cleared.  *out_n gets the count; when it exceeds capacity the call
+ * returns IKD_ERR_CAPACITY and keeps the record.  Which deleted points a
+ * rebuild has dropped so far depends on when rebuilds ran, which differs from
+ * the reference (rebuilds run after each call here, inside the per-point
+ * loops there). */
+int ikd_acquire_removed_points(ikd_tree_t t, float* out_xyz, uint32_t* out_ids, int64_t capacity, int64_t* out_n);
 /* Diagnostics (no counterpart in the reference): the tree's event counts. */
 int ikd_get_counters(ikd_tree_t t, ikd_counters_t* out);
 
@@ -175,6 +183,12 @@ int ikd_get_counters(ikd_tree_t t, ikd_counters_t* out);
  *                       downsample: 1, 4, 8 or 16; 0 (default) takes 8 for
  *                       maps of >= 4096 nodes and 1 below (identical results). */
 #define IKD_TUNE_FOLD_LANES 6
+/*  IKD_TUNE_RECORD_REMOVED 1: rebuilds record every deleted point they drop
+ *                       that the downsample did not delete, for
+ *                       ikd_acquire_removed_points (flatten's DELETE_POINTS_REC
+ *                       mode, ikd_Tree.cpp:638,1338-1341); 0 (default): no
+ *                       record, and rebuilds skip deleted subtrees. */
+#define IKD_TUNE_RECORD_REMOVED 7
 int ikd_set_tuning(ikd_tree_t t, int key, int value);
 
 /* Run the deferred rebuilds that are still due, in the order they were

@@ -72,6 +72,8 @@ def _load():
     lib.ora_flatten.argtypes = [vp, vp, vp, i64]
     lib.ora_dump.restype = i64
     lib.ora_dump.argtypes = [vp, vp, i64]
+    lib.ora_acquire_removed.restype = i64
+    lib.ora_acquire_removed.argtypes = [vp, vp, vp, i64]
     lib.ora_dump_row_bytes.restype = i32
     assert lib.ora_dump_row_bytes() == DUMP_DTYPE.itemsize
     _lib = lib
@@ -179,6 +181,16 @@ class OracleTree:
 
     def flatten(self):
         return self._collect(self._lib.ora_flatten)
+
+    def acquire_removed_points(self):
+        """acquire_removed_points (cpp:558-571): points rebuilds removed since
+        the last call, deleted but not by the downsample; clears the record."""
+        n = int(self._lib.ora_acquire_removed(self._h, None, None, 0))
+        xyz = np.zeros((max(n, 1), 3), np.float32)
+        ids = np.zeros(max(n, 1), np.uint32)
+        got = int(self._lib.ora_acquire_removed(self._h, _ptr(xyz), _ptr(ids), n))
+        assert got == n
+        return xyz[:n], ids[:n]
 
     def dump(self) -> np.ndarray:
         n = self._lib.ora_dump(self._h, None, 0)

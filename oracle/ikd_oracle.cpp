@@ -360,14 +360,19 @@ public:
                b.vmin[2] <= p.z && b.vmax[2] > p.z;
     }
 
-    // flatten, cpp:1325-1352 (pre-order; the removed-point records are not
-    // restated).
-    void flatten(Node* n, std::vector<Pt>& out) {
+    // Points_deleted (cpp:558-571): points removed from the tree by rebuilds.
+    std::vector<Pt> removed;
+
+    // flatten, cpp:1325-1352: live points in pre-order; with rec (the
+    // DELETE_POINTS_REC mode, cpp:1338-1341) every deleted point that was not
+    // removed by the downsample is recorded after the node's children.
+    void flatten(Node* n, std::vector<Pt>& out, std::vector<Pt>* rec = nullptr) {
         if (!n) return;
         push_down(n);
         if (!n->point_deleted) out.push_back(n->point);
-        flatten(n->left, out);
-        flatten(n->right, out);
+        flatten(n->left, out, rec);
+        flatten(n->right, out, rec);
+        if (rec && n->point_deleted && !n->point_ds_deleted) rec->push_back(n->point);
     }
 
     // Search_by_range, cpp:1015-1044.
@@ -415,7 +420,7 @@ public:
     void rebuild(Node** slot) {
         Node* father = (*slot)->father;
         std::vector<Pt> st;
-        flatten(*slot, st);
+        flatten(*slot, st, &removed);
         destroy(*slot);
         *slot = nullptr;
         if (!st.empty()) build_tree(slot, 0, (int)st.size() - 1, st);
@@ -734,5 +739,22 @@ int64_t ora_dump(void* h, DumpRow* rows, int64_t cap) {
 }
 
 int ora_dump_row_bytes() { return (int)sizeof(DumpRow); }
+
+// acquire_removed_points, cpp:558-571: the points rebuilds removed since the
+// last call (deleted, not by the downsample), then the record is cleared.
+// Returns the count; copies only when cap holds them all (and only then clears).
+int64_t ora_acquire_removed(void* h, float* out_xyz, uint32_t* out_ids, int64_t cap) {
+    Tree* t = (Tree*)h;
+    int64_t n = (int64_t)t->removed.size();
+    if (n > cap) return n;
+    for (int64_t i = 0; i < n; ++i) {
+        out_xyz[3 * i] = t->removed[i].x;
+        out_xyz[3 * i + 1] = t->removed[i].y;
+        out_xyz[3 * i + 2] = t->removed[i].z;
+        if (out_ids) out_ids[i] = t->removed[i].id;
+    }
+    t->removed.clear();
+    return n;
+}
 
 }  // extern "C"

@@ -328,3 +328,41 @@ def test_rebuild_collapses_deleted_nodes():
     keep = np.ones(3000, bool)
     keep[sel] = False
     assert_same_multiset(t.flatten()[0], pts[keep], "live set after rebuilds")
+
+
+def test_acquire_removed_points_records_rebuild_drops():
+    """acquire_removed_points (cpp:558-571): a box delete of most points rebuilds
+    the root, which drops and records exactly the points in the box; points the
+    downsample deleted are never recorded; recorded + deleted still in the tree
+    = deleted by the caller; the record is cleared by the call."""
+    rng = np.random.default_rng(93)
+    pts = uniform_cloud(rng, 8000, 0, 20)
+    ids = np.arange(8000, dtype=np.uint32)
+    o = OracleTree(0.5, 0.6, 0.5)
+    o.Build(pts, ids)
+    box = np.array([[-1, -1, -1, 13, 21, 21]], np.float32)
+    n = o.Delete_Point_Boxes(box)
+    x, i = o.acquire_removed_points()
+    inside = in_box(pts, box[0])
+    assert n == len(i) == int(inside.sum())
+    assert sorted(i.tolist()) == sorted(ids[inside].tolist())
+    assert np.array_equal(x[np.argsort(i)], pts[np.sort(i)])
+    assert len(o.acquire_removed_points()[1]) == 0
+    # downsample deletions: voxel centres replace the old points of their voxels
+    live, live_ids = o.flatten()
+    cc = (np.floor(live[:200] / 0.5) * 0.5 + 0.25).astype(np.float32)
+    o.Add_Points(cc, True, np.arange(10000, 10200, dtype=np.uint32))
+    deleted = 0
+    recorded = set()
+    for it in range(4):
+        live, live_ids = o.flatten()
+        sel = rng.choice(live.shape[0], live.shape[0] // 3, replace=False)
+        v0 = o.validnum()
+        o.Delete_Points(live[sel])
+        deleted += v0 - o.validnum()
+        _, ri = o.acquire_removed_points()
+        assert not (set(ri.tolist()) & recorded)
+        recorded |= set(ri.tolist())
+        assert recorded <= set(range(8000)) | set(range(10000, 10200))
+    ds_size = o.size() - o.validnum()  # deleted nodes still in the tree, downsample ones included
+    assert len(recorded) <= deleted <= len(recorded) + ds_size
