@@ -196,9 +196,11 @@ __global__ void k_refit_up(Pool pool, Params prm, const int* __restrict__ dirty,
             refit_node(pool, prm, u, viol, defer, hdr);
             if (u == ROOT) break;
             const int p = pool.par[u];
-            __threadfence();
+            // release this node's results to the lane that finishes the
+            // parent; that lane acquires them after the last decrement
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             if (atomicSub(&pend[p], 1) != 1) break;  // a sibling is still on its way
-            __threadfence();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             u = p;
         }
     }
