@@ -88,8 +88,8 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (maps shard by tree/robot; "
-              "see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (maps would shard by tree / robot, "
+              "one process per GPU). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch
