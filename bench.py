@@ -33,6 +33,9 @@ against one call per tree and (k, max_dist) group.
 
 Phase D.  --robots maps of 0.5-3M points served by the request loop with
 Poisson scan arrivals at 10 Hz per robot: p50/p99 of completion - arrival.
+Subtrees above --serve-defer live points are rebuilt between a map's
+requests, --serve-chunk points at a time; each finished request carries the
+map's event counts (rebuilds, rebuilt points, allocations, host waits).
 
 The CPU baseline is the oracle (CPU restatement of the reference) on a bounded
 sample of the same queries and map.
