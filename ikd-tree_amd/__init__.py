@@ -38,7 +38,8 @@ EXPORTS = (
     "ikd_create", "ikd_destroy", "ikd_set_delete_criterion_param", "ikd_set_balance_criterion_param",
     "ikd_set_downsample_param", "ikd_initialize", "ikd_build", "ikd_nearest_search",
     "ikd_nearest_search_device", "ikd_nearest_search_counters", "ikd_box_search", "ikd_radius_search",
-    "ikd_add_points", "ikd_add_points_device", "ikd_delete_points", "ikd_delete_point_boxes", "ikd_size",
+    "ikd_add_points", "ikd_add_points_device", "ikd_delete_points", "ikd_delete_point_boxes",
+    "ikd_add_point_boxes", "ikd_size",
     "ikd_validnum", "ikd_tree_range", "ikd_root_alpha", "ikd_flatten", "ikd_synchronize", "ikd_stream",
     "ikd_pool_stats", "ikd_get_counters", "ikd_acquire_removed_points", "ikd_set_tuning", "ikd_dump_nodes", "ikd_last_error",
     "ikd_batch_create", "ikd_batch_destroy", "ikd_batch_nearest_search_device", "ikd_batch_synchronize",
@@ -107,6 +108,7 @@ def load_library(path: str | None = None):
         "ikd_add_points_device": [vp, vp, vp, i64, i32, vp],
         "ikd_delete_points": [vp, vp, i64],
         "ikd_delete_point_boxes": [vp, vp, i64, vp],
+        "ikd_add_point_boxes": [vp, vp, i64],
         "ikd_size": [vp, vp],
         "ikd_validnum": [vp, vp],
         "ikd_tree_range": [vp, vp],
@@ -297,6 +299,12 @@ class KD_TREE:
         out = ctypes.c_int32(0)
         self._check(self._lib.ikd_delete_point_boxes(self._h, _ptr(b), b.shape[0], ctypes.byref(out)))
         return int(out.value)
+
+    def Add_Point_Boxes(self, boxes):
+        """Add_Point_Boxes (ikd_Tree.cpp:491-511): restore the deleted points the
+        boxes reach that the downsample did not delete (no return value)."""
+        b = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
+        self._check(self._lib.ikd_add_point_boxes(self._h, _ptr(b), b.shape[0]))
 
     # ------------------------------------------------------------ accessors
     def size(self) -> int:

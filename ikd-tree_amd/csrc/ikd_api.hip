@@ -140,6 +140,18 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// Pinned staging for small host-to-device copies of host values (segment
+// lists, counters, tree tables): hipMemcpyAsync from pageable memory may read
+// the source after the call returns, so values that live on the host stack or
+// in a local vector are first copied here.  A bump allocator; a region is
+// reused only after the event recorded behind the last staged copy completed.
+struct Stager {
+    char* buf = nullptr;
+    size_t cap = 0, used = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+};
+
 }  // namespace
 
 struct ikd_tree_s {
@@ -168,6 +180,12 @@ struct ikd_tree_s {
     hipEvent_t ev = nullptr;  // orders batch launches against this tree's stream
     ikd_counters_t ctr{};     // event counts since creation (ikd_get_counters)
     hipMemPool_t mp = nullptr;  // stream-ordered allocations (null: hipMalloc / hipFree)
+    Stager stg;                 // pinned staging of small host-to-device copies
+    // One call at a time per tree: the C ABI locks this for the whole call, so
+    // reads from many host threads (FAST-LIO's OpenMP loop of Nearest_Search)
+    // serialize onto the tree's stream instead of sharing its scratch slots.
+    // Recursive: public entry points call one another.
+    std::recursive_mutex mu;
     // scratch
     DevBuf b[80];
 };
@@ -184,6 +202,8 @@ struct ikd_batch_s {
     int* h_flag = nullptr;  // pinned mirror of the overflow flag
     ikd_counters_t ctr{};   // only scratch_grows and host_waits move
     hipMemPool_t mp = nullptr;
+    Stager stg;
+    std::mutex mu;
     DevBuf b[8];
 };
 
@@ -195,7 +215,8 @@ enum Buf {
     B_SEEDS, B_DIRTY, B_VIOL, B_TOP, B_INFO, B_FRONT0, B_FRONT1, B_EKEY0, B_EKEY1, B_RECS, B_GFLAG, B_GSCAN, B_ATTACH,
     B_PFLAG, B_PSCAN, B_ISEG, B_ISEGD, B_IO0, B_IO1, B_IO2, B_IO3, B_IO4, B_VSTART, B_DELBOX, B_DELFL, B_INSPT,
     B_INSFL, B_CBOX, B_CPTS, B_MISC, B_MORTK, B_MORTV, B_MORTK2, B_MORTV2, B_CNT64, B_OFF64, B_ADDX, B_ADDI, B_PNEXT, B_SPAIRS, B_SPSCAN,
-    B_DEFER, B_DINFO, B_DBEFORE, B_DKEEP, B_PTS3, B_PEND, B_START, B_INV0, B_COUNT
+    B_DEFER, B_DINFO, B_DBEFORE, B_DKEEP, B_PTS3, B_PEND, B_START, B_INV0, B_LASTP, B_PSEED, B_DBELOW, B_MAXDEL,
+    B_COUNT
 };
 static_assert(B_COUNT <= 80, "scratch table");
 
@@ -290,6 +311,59 @@ int ensure(Owner* t, int slot, long long count, T** out) {
 hipError_t wait_stream(ikd_tree_t t) {
     t->ctr.host_waits += 1;
     return hipStreamSynchronize(t->st);
+}
+
+// Copies `bytes` of host memory to device `dst` on the owner's stream through
+// the owner's pinned staging buffer, so `src` may go away as soon as this
+// returns.
+template <class Owner>
+int stage_copy(Owner* t, void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return IKD_OK;
+    Stager& g = t->stg;
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (g.used + need > g.cap) {
+        // every staged region is about to be reused: wait for the last copy
+        if (g.pending) HIPCHK(hipEventSynchronize(g.ev));
+        g.pending = false;
+        g.used = 0;
+        if (need > g.cap) {
+            if (g.buf) HIPCHK(hipHostFree(g.buf));
+            g.buf = nullptr;
+            g.cap = 0;
+            const size_t nc = std::max<size_t>(need, 64 << 10);
+            HIPCHK(hipHostMalloc((void**)&g.buf, nc, hipHostMallocDefault));
+            g.cap = nc;
+        }
+    }
+    if (!g.ev) HIPCHK(hipEventCreateWithFlags(&g.ev, hipEventDisableTiming));
+    char* p = g.buf + g.used;
+    std::memcpy(p, src, bytes);
+    g.used += need;
+    HIPCHK(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, t->st));
+    HIPCHK(hipEventRecord(g.ev, t->st));
+    g.pending = true;
+    return IKD_OK;
+}
+
+template <class Owner>
+void stage_release(Owner* t) {
+    Stager& g = t->stg;
+    if (g.pending) hipEventSynchronize(g.ev);
+    if (g.ev) hipEventDestroy(g.ev);
+    if (g.buf) hipHostFree(g.buf);
+    g = Stager{};
+}
+
+// A count the host read back from the device (and sizes launches or host
+// loops by) must lie in [lo, hi]; anything else means a kernel misbehaved,
+// and the call stops with IKD_ERR_INTERNAL instead of launching with it.
+template <class Owner>
+int check_count(Owner* t, const char* what, long long v, long long lo, long long hi) {
+    if (v >= lo && v <= hi) return IKD_OK;
+    char msg[200];
+    snprintf(msg, sizeof msg, "device count out of range: %s = %lld, expected [%lld, %lld]", what, v, lo, hi);
+    t->err = msg;
+    return IKD_ERR_INTERNAL;
 }
 
 int sync_hdr(ikd_tree_t t) {
@@ -501,9 +575,7 @@ int run_build(ikd_tree_t t, const float4* pts, const int* seg_of, int m, const i
     }
     HIPCHK(hipMemcpyAsync(seg[0], d_seg, S * sizeof(int4), hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(segd[0], d_seg_depth, S * sizeof(int), hipMemcpyDeviceToDevice, st));
-    HIPCHK(hipMemcpyAsync(&t->d_hdr->seg_count[0], &S, sizeof(int), hipMemcpyHostToDevice, st));
-    // a pageable host source: make sure it is consumed before S goes away
-    HIPCHK(wait_stream(t));
+    CHK(stage_copy(t, &t->d_hdr->seg_count[0], &S, sizeof(int)));
 
     int levels = bits_for(maxlen);  // floor(log2(maxlen)) + 1
     // A segment of x points leaves children of at most x/2: from level Ls on
@@ -633,9 +705,9 @@ int enumerate_live(ikd_tree_t t, const std::vector<int>& roots, const std::vecto
         h[i] = make_int4(roots[i], (int)base, i, 0);
         base += live[i];
     }
-    HIPCHK(hipMemcpyAsync(f0, h.data(), R * sizeof(int4), hipMemcpyHostToDevice, st));
+    CHK(stage_copy(t, f0, h.data(), R * sizeof(int4)));
     int init[3] = {R, 0, 0};
-    HIPCHK(hipMemcpyAsync(t->d_hdr->front3, init, sizeof(init), hipMemcpyHostToDevice, st));
+    CHK(stage_copy(t, t->d_hdr->front3, init, sizeof(init)));
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, t->d_hdr, 2);
     const int levels = std::max(t->max_depth - std::max(min_depth, 0), 0) + 2;
     float4* rec = nullptr;
@@ -658,7 +730,7 @@ int enumerate_live(ikd_tree_t t, const std::vector<int>& roots, const std::vecto
                                  st);
         }
     }
-    CHK(sync_hdr(t));  // also keeps h and init alive until the copies ran
+    CHK(sync_hdr(t));
     if (t->h_hdr->rec_count != total_live || (t->h_hdr->overflow & 32)) {
         char msg[160];
         snprintf(msg, sizeof msg, "enumerate found %d live points, counts say %lld", t->h_hdr->rec_count, total_live);
@@ -728,6 +800,11 @@ int rebuild_listed(ikd_tree_t t, const std::vector<int4>& hinfo, const std::vect
     bool root_rebuilt = false;
     for (int i = 0; i < ntop; ++i) {
         int u = hinfo[i].x, live = hinfo[i].y, depth = hinfo[i].z, parent = hinfo[i].w;
+        CHK(check_count(t, "rebuild: subtree root", u, ROOT, t->cap - 1));
+        CHK(check_count(t, "rebuild: parent", parent, SENTINEL, t->cap - 1));
+        CHK(check_count(t, "rebuild: subtree nodes", hbefore[i], 1, t->pool_used));
+        CHK(check_count(t, "rebuild: live points", live, 0, hbefore[i]));
+        CHK(check_count(t, "rebuild: depth", depth, 0, 4 * MAX_DEPTH));
         if (u == ROOT) root_rebuilt = true;
         if (live <= 0) {
             // Rebuild of an all-deleted subtree leaves an empty slot (BuildTree of
@@ -754,13 +831,14 @@ int rebuild_listed(ikd_tree_t t, const std::vector<int4>& hinfo, const std::vect
         int* dsegd;
         CHK(ensure(t, B_ISEG, (long long)segs.size(), &dseg));
         CHK(ensure(t, B_ISEGD, (long long)segs.size(), &dsegd));
-        HIPCHK(hipMemcpyAsync(dseg, segs.data(), segs.size() * sizeof(int4), hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(dsegd, segd.data(), segd.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        CHK(stage_copy(t, dseg, segs.data(), segs.size() * sizeof(int4)));
+        CHK(stage_copy(t, dsegd, segd.data(), segd.size() * sizeof(int)));
         if (root_rebuilt) {
             int zero = 0;
-            HIPCHK(hipMemcpyAsync(&t->d_hdr->max_depth, &zero, sizeof(int), hipMemcpyHostToDevice, st));
+            CHK(stage_copy(t, &t->d_hdr->max_depth, &zero, sizeof(int)));
         }
         CHK(run_build(t, pts, segof, (int)total_live, dseg, dsegd, (int)segs.size(), maxlen));
+        if (root_rebuilt) launch_root_alpha_fresh(t->pool, t->d_hdr, st);
     }
     return IKD_OK;
 }
@@ -771,6 +849,7 @@ int rebuild_tops(ikd_tree_t t) {
     for (int iter = 0; iter < 8; ++iter) {
         CHK(refresh(t));
         int ntop = t->h_hdr->top_count;
+        CHK(check_count(t, "rebuild: subtrees", ntop, 0, t->pool_used));
         if (ntop == 0) return IKD_OK;
         int* top;
         int4* info;
@@ -829,10 +908,11 @@ int compact_if_needed(ikd_tree_t t, bool force, bool fresh = false) {
     int* dsegd;
     CHK(ensure(t, B_ISEG, 1, &dseg));
     CHK(ensure(t, B_ISEGD, 1, &dsegd));
-    HIPCHK(hipMemcpyAsync(dseg, &s0, sizeof(s0), hipMemcpyHostToDevice, t->st));
-    HIPCHK(hipMemcpyAsync(dsegd, &d0, sizeof(d0), hipMemcpyHostToDevice, t->st));
+    CHK(stage_copy(t, dseg, &s0, sizeof(s0)));
+    CHK(stage_copy(t, dsegd, &d0, sizeof(d0)));
     CHK(run_build(t, keep, segof0, (int)live, dseg, dsegd, 1, (int)live));
     hipLaunchKernelGGL(k_link_bits, dim3(1), dim3(1), 0, t->st, t->pool.hot, SENTINEL, LK_HASL, LK_TDL);
+    launch_root_alpha_fresh(t->pool, t->d_hdr, t->st);
     t->rebuilds += 1;
     return refresh(t);
 }
@@ -851,6 +931,8 @@ int insert_points(ikd_tree_t t, const float4* pts, int m, const int* extra_seeds
         return rebuild_tops(t);
     }
     hipStream_t st = t->st;
+    // a root made by this insert is a new node (its statistics are set below)
+    const bool new_root = !has_root(t);
     CHK(reserve_nodes(t, 4ll * m + 16));
     unsigned long long *ka, *kb;
     uint32_t *va, *vb;
@@ -887,8 +969,11 @@ int insert_points(ikd_tree_t t, const float4* pts, int m, const int* extra_seeds
     HIPCHK(hipMemcpyAsync(&maxlen, misc, sizeof(int), hipMemcpyDeviceToHost, st));
     CHK(sync_hdr(t));
     int S = t->h_hdr->group_count;
+    CHK(check_count(t, "insert: attach groups", S, 1, m));
+    CHK(check_count(t, "insert: largest group", maxlen, 1, m));
     CHK(run_build(t, spts, segof, m, seg, segd, S, maxlen));
     launch_insert_link(t->pool, attach, seg, t->d_hdr, S, st);
+    if (new_root) launch_root_alpha_fresh(t->pool, t->d_hdr, st);
     int* seeds;
     CHK(ensure(t, B_SEEDS, S, &seeds));
     hipLaunchKernelGGL(k_seeds_from_attach, dim3(nblk(S)), dim3(BS), 0, st, attach, t->d_hdr, S, seeds);
@@ -907,7 +992,8 @@ int delete_boxes_dev(ikd_tree_t t, const float* d_boxes, int nb, int ds) {
     if (nb <= 0) return IKD_OK;
     hipStream_t st = t->st;
     unsigned int* recs;
-    long long cap = std::max<long long>(t->pool_used, 16) + 4ll * nb;
+    // per box: the nodes it cuts, the points it deletes and the subtrees it covers
+    long long cap = 2 * std::max<long long>(t->pool_used, 16) + 4ll * nb;
     long long fcap = std::max<long long>(std::max<long long>(t->pool_used, 1 << 16), nb);
     int2 *f0, *f1;
     CHK(ensure(t, B_RECS, cap, &recs));
@@ -930,6 +1016,7 @@ int delete_boxes_dev(ikd_tree_t t, const float* d_boxes, int nb, int ds) {
         L1 += 8;
     }
     int nrec = t->h_hdr->rec_count;
+    CHK(check_count(t, "box delete: records", nrec, 0, INT_MAX));
     if (nrec > cap || t->h_hdr->front_over) {
         // overlapping boxes recorded more than one entry per node, or their
         // frontiers together outgrew the buffer: the union is order-free, so
@@ -948,6 +1035,58 @@ int delete_boxes_dev(ikd_tree_t t, const float* d_boxes, int nb, int ds) {
     launch_box_delete_apply(t->pool, recs, t->d_hdr, nrec, ds, seeds, st);
     CHK(refit(t, seeds, nrec, 1));
     return rebuild_tops(t);
+}
+
+// ------------------------------------------------------- box re-insertion
+// Add_Point_Boxes (cpp:491-511): the boxes one after another, as the
+// reference's loop runs Add_by_range once per box (each box's walk reads the
+// ranges the previous box's Updates and rebuilds left).  Per box: the walk
+// over the cut nodes, the reset of the covered subtrees, then Update and
+// Criterion_Check of the cut nodes bottom-up and the rebuilds.
+int add_boxes_dev(ikd_tree_t t, const float* d_boxes, int nb) {
+    hipStream_t st = t->st;
+    for (int b = 0; b < nb; ++b) {
+        const float* box = d_boxes + 6ll * b;
+        // the walk records each node at most once, as cut or as covered
+        const long long cap = std::max<long long>(t->pool_used + 16, 1 << 16);
+        unsigned int* recs;
+        int2 *f0, *f1;
+        CHK(ensure(t, B_RECS, cap, &recs));
+        CHK(ensure(t, B_FRONT0, cap, &f0));
+        CHK(ensure(t, B_FRONT1, cap, &f1));
+        int L0 = 0, L1 = t->max_depth + 2;
+        while (true) {
+            launch_addbox_walk(t->pool, box, f0, f1, (int)cap, L0, L1, recs, t->d_hdr, (int)cap, st);
+            CHK(sync_hdr(t));
+            if (t->h_hdr->front_over || t->h_hdr->front3[L1 % 3] == 0) break;
+            if (L1 > 4 * MAX_DEPTH) {
+                t->err = "Add_Point_Boxes: the walk did not drain";
+                return IKD_ERR_INTERNAL;
+            }
+            L0 = L1;
+            L1 += 8;
+        }
+        const int nrec = t->h_hdr->rec_count;
+        if (t->h_hdr->front_over || nrec > cap) {
+            t->err = "Add_Point_Boxes: the walk outgrew its buffers";
+            return IKD_ERR_INTERNAL;
+        }
+        CHK(check_count(t, "box re-insertion: records", nrec, 0, cap));
+        if (nrec == 0) continue;
+        int* seeds;
+        CHK(ensure(t, B_SEEDS, nrec, &seeds));
+        const int levels = t->max_depth + 2;
+        launch_addbox_reset(t->pool, recs, nrec, seeds, f0, f1, (int)cap, levels, t->d_hdr, st);
+        HIPCHK(hipGetLastError());
+        CHK(sync_hdr(t));
+        if (t->h_hdr->front_over || t->h_hdr->front3[levels % 3] != 0) {
+            t->err = "Add_Point_Boxes: the reset of covered subtrees did not drain";
+            return IKD_ERR_INTERNAL;
+        }
+        CHK(refit(t, seeds, nrec, 0));
+        CHK(rebuild_tops(t));
+    }
+    return IKD_OK;
 }
 
 // ---------------------------------------------------- Add_Points pipeline
@@ -1015,6 +1154,9 @@ int add_points_ds_chunk(ikd_tree_t t, const float* d_xyz, const uint32_t* d_ids,
     }
     int nv = t->h_hdr->group_count;
     int ndel = t->h_hdr->rec_count;
+    CHK(check_count(t, "downsample: voxels", nv, 0, n));
+    CHK(check_count(t, "downsample: deleted points", ndel, 0, dcap));
+    CHK(check_count(t, "downsample: kept points", t->h_hdr->ins_count, 0, nv));
     // compact the points to insert (voxel order), then insert them and refit
     // the insert paths and the deleted points' paths in one pass
     float4* cpts;
@@ -1182,6 +1324,7 @@ int range_dev(ikd_tree_t t, int mode, const float* d_boxes, const float* d_cente
     HIPCHK(hipMemcpyAsync(h_offsets, off, (nq + 1) * sizeof(long long), hipMemcpyDeviceToHost, st));
     CHK(sync_hdr(t));
     long long total = h_offsets[nq];
+    CHK(check_count(t, "range search: results", total, 0, (long long)nq * std::max<long long>(t->pool_used, 1)));
     if (!h_xyz) return IKD_OK;
     if (capacity < total) {
         t->err = "output capacity below the result count";
@@ -1211,10 +1354,10 @@ int upload(ikd_tree_t t, int slot, const T* h, long long n, T** d) {
 }  // namespace
 
 // =================================================================== C ABI
-#define CHECK_T()                        \
-    do {                                 \
-        if (!t) return IKD_ERR_ARG;      \
-    } while (0)
+// Every entry point on a tree holds the tree's lock for the whole call.
+#define CHECK_T()                    \
+    if (!t) return IKD_ERR_ARG;      \
+    std::lock_guard<std::recursive_mutex> call_lock_(t->mu)
 
 extern "C" {
 
@@ -1252,13 +1395,18 @@ int ikd_create(float delete_param, float balance_param, float box_length, ikd_tr
 }
 
 int ikd_destroy(ikd_tree_t t) {
-    CHECK_T();
+    if (!t) return IKD_ERR_ARG;
+    {
+        // wait for a call still running on another thread
+        std::lock_guard<std::recursive_mutex> g(t->mu);
+    }
     if (t->st) {
         if (t->cap > 0) free_pool(t, &t->pool);
         for (auto& b : t->b) dev_free(t, b.p);
         dev_free(t, t->removed);
         hipStreamSynchronize(t->st);
     }
+    stage_release(t);
     if (t->d_hdr) hipFree(t->d_hdr);
     if (t->h_hdr) hipHostFree(t->h_hdr);
     if (t->d_root) hipFree(t->d_root);
@@ -1310,9 +1458,10 @@ int ikd_build(ikd_tree_t t, const float* xyz, const uint32_t* ids, int64_t n) {
         int d0 = 0;
         int4* dseg;
         int* dsegd;
-        CHK(upload(t, B_ISEG, &s0, 1, &dseg));
-        CHK(upload(t, B_ISEGD, &d0, 1, &dsegd));
-        HIPCHK(wait_stream(t));
+        CHK(ensure(t, B_ISEG, 1, &dseg));
+        CHK(ensure(t, B_ISEGD, 1, &dsegd));
+        CHK(stage_copy(t, dseg, &s0, sizeof(s0)));
+        CHK(stage_copy(t, dsegd, &d0, sizeof(d0)));
         CHK(run_build(t, pts, segof, (int)n, dseg, dsegd, 1, (int)n));
         hipLaunchKernelGGL(k_link_bits, dim3(1), dim3(1), 0, t->st, t->pool.hot, SENTINEL, LK_HASL, LK_TDL);
     }
@@ -1439,20 +1588,31 @@ int ikd_delete_points(ikd_tree_t t, const float* xyz, int64_t n) {
     if (n < 0 || n >= (1ll << 31) || (n > 0 && !xyz)) return IKD_ERR_ARG;
     if (n == 0) return IKD_OK;
     float* dx;
-    int *cand, *seeds;
+    int *cand, *seeds, *last_pass, *path_seed, *del_below, *max_del;
+    CHK(sync_hdr(t));
+    const long long nodes = std::max<long long>(t->pool_used, 16);
     CHK(upload(t, B_IO4, xyz, 3 * n, &dx));
     CHK(ensure(t, B_IO0, n, &cand));
-    CHK(ensure(t, B_SEEDS, n, &seeds));
+    CHK(ensure(t, B_SEEDS, 2 * n, &seeds));
+    CHK(ensure(t, B_PSEED, n, &path_seed));
+    CHK(ensure(t, B_LASTP, nodes, &last_pass));
+    CHK(ensure(t, B_DBELOW, nodes, &del_below));
+    CHK(ensure(t, B_MAXDEL, nodes, &max_del));
+    launch_fill_i32(last_pass, -1, nodes, t->st);
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, t->st, t->d_hdr, 4);
     launch_delete_points_find(t->pool, dx, (int)n, cand, t->d_hdr, t->st);
     CHK(sync_hdr(t));
     if (t->h_hdr->conflicts == 0) {
-        launch_delete_points_apply(t->pool, cand, (int)n, seeds, t->st);
+        launch_fill_i32(del_below, 0, nodes, t->st);
+        launch_fill_i32(max_del, -1, nodes, t->st);
+        launch_delete_points_apply(t->pool, dx, cand, del_below, max_del, last_pass, path_seed, (int)n, seeds,
+                                   t->d_hdr, t->st);
     } else {
         launch_delete_points_unclaim(t->pool, cand, (int)n, seeds, t->st);
-        launch_delete_points_serial(t->pool, dx, (int)n, seeds, t->st);
+        launch_delete_points_serial(t->pool, dx, (int)n, last_pass, path_seed, seeds, t->d_hdr, t->st);
     }
-    CHK(refit(t, seeds, (int)n, 0));
+    // the deleted nodes and every node a walk Updates (cpp:756-759)
+    CHK(refit(t, seeds, 2 * (int)n, 0));
     CHK(rebuild_tops(t));
     return compact_if_needed(t, false, true);
 }
@@ -1470,6 +1630,17 @@ int ikd_delete_point_boxes(ikd_tree_t t, const float* boxes, int64_t nb, int32_t
     CHK(compact_if_needed(t, false));
     if (out_deleted) *out_deleted = (int32_t)(before - live_count(t));
     return IKD_OK;
+}
+
+int ikd_add_point_boxes(ikd_tree_t t, const float* boxes, int64_t nb) {
+    CHECK_T();
+    if (nb < 0 || nb >= (1ll << 31) || (nb > 0 && !boxes)) return IKD_ERR_ARG;
+    if (nb == 0) return IKD_OK;
+    CHK(sync_hdr(t));
+    float* db;
+    CHK(upload(t, B_IO4, boxes, 6 * nb, &db));
+    CHK(add_boxes_dev(t, db, (int)nb));
+    return compact_if_needed(t, false);
 }
 
 int ikd_size(ikd_tree_t t, int64_t* out) {
@@ -1496,19 +1667,14 @@ int ikd_tree_range(ikd_tree_t t, float* out6) {
     return IKD_OK;
 }
 
-// Update's root statistics, cpp:1315-1321, evaluated on the current root.
+// root_alpha, cpp:147-165: the root's statistics as of its last Update
+// (cpp:1315-1321), kept in the device header by the refit and the builds.
 int ikd_root_alpha(ikd_tree_t t, float* alpha_bal, float* alpha_del) {
     CHECK_T();
     if (!alpha_bal || !alpha_del) return IKD_ERR_ARG;
-    CHK(read_root(t));
-    *alpha_bal = 0.0f;
-    *alpha_del = 0.0f;
-    if (!has_root(t) || t->h_root->cnt.x <= 3) return IKD_OK;
-    int size = t->h_root->cnt.x;
-    int son = t->h_root->child.x >= 0 ? t->h_root->child.x : t->h_root->child.y;
-    float bal = float(son) / (size - 1);
-    *alpha_del = float(t->h_root->cnt.y) / size;
-    *alpha_bal = (bal >= 0.5 - EPSS) ? bal : 1 - bal;
+    CHK(sync_hdr(t));
+    *alpha_bal = t->h_hdr->alpha_bal;
+    *alpha_del = t->h_hdr->alpha_del;
     return IKD_OK;
 }
 
@@ -1628,6 +1794,7 @@ int ikd_rebuild_deferred(ikd_tree_t t, int64_t max_points, int32_t* out_rebuilt)
     CHECK_T();
     if (out_rebuilt) *out_rebuilt = 0;
     CHK(sync_hdr(t));
+    CHK(check_count(t, "deferred rebuilds", t->h_hdr->defer_count, 0, INT_MAX));
     int n = std::min(t->h_hdr->defer_count, DEFER_CAP);
     if (n <= 0) return IKD_OK;
     hipStream_t st = t->st;
@@ -1663,10 +1830,10 @@ int ikd_rebuild_deferred(ikd_tree_t t, int64_t max_points, int32_t* out_rebuilt)
     launch_defer_clear(t->pool, defer, n, st);
     int cnt = (int)rest.size();
     if (cnt > 0) {
-        HIPCHK(hipMemcpyAsync(defer, rest.data(), cnt * sizeof(int), hipMemcpyHostToDevice, st));
+        CHK(stage_copy(t, defer, rest.data(), cnt * sizeof(int)));
         hipLaunchKernelGGL(k_link_defer_flags, dim3(nblk(cnt)), dim3(BS), 0, st, t->pool, defer, cnt);
     }
-    HIPCHK(hipMemcpyAsync(&t->d_hdr->defer_count, &cnt, sizeof(int), hipMemcpyHostToDevice, st));
+    CHK(stage_copy(t, &t->d_hdr->defer_count, &cnt, sizeof(int)));
     HIPCHK(wait_stream(t));
     if (take.empty()) return IKD_OK;
     CHK(rebuild_listed(t, take, take_before));
@@ -1676,7 +1843,7 @@ int ikd_rebuild_deferred(ikd_tree_t t, int64_t max_points, int32_t* out_rebuilt)
     for (const int4& e : take) seeds.push_back(e.y > 0 ? e.x : e.w);
     int* dseeds;
     CHK(ensure(t, B_DKEEP, (long long)seeds.size(), &dseeds));
-    HIPCHK(hipMemcpyAsync(dseeds, seeds.data(), seeds.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    CHK(stage_copy(t, dseeds, seeds.data(), seeds.size() * sizeof(int)));
     CHK(sync_hdr(t));
     CHK(refit_begin(t));
     CHK(refit_mark(t, dseeds, nullptr, (int)seeds.size(), 0));
@@ -1779,6 +1946,7 @@ static int batch_begin(ikd_batch_t b, const ikd_tree_t* trees, int ntrees, int* 
             b->err = "batch: null tree handle";
             return IKD_ERR_ARG;
         }
+        std::lock_guard<std::recursive_mutex> g(tr->mu);
         if (tr->max_depth + 2 > MAX_DEPTH) {
             int rc = compact_if_needed(tr, true);
             if (rc != IKD_OK) {
@@ -1797,7 +1965,10 @@ static int batch_begin(ikd_batch_t b, const ikd_tree_t* trees, int ntrees, int* 
 static int batch_end(ikd_batch_t b, const ikd_tree_t* trees, int ntrees) {
     auto* t = b;
     HIPCHK(hipEventRecord(b->ev, b->st));
-    for (int i = 0; i < ntrees; ++i) HIPCHK(hipStreamWaitEvent(trees[i]->st, b->ev, 0));
+    for (int i = 0; i < ntrees; ++i) {
+        std::lock_guard<std::recursive_mutex> g(trees[i]->mu);
+        HIPCHK(hipStreamWaitEvent(trees[i]->st, b->ev, 0));
+    }
     return IKD_OK;
 }
 
@@ -1838,6 +2009,7 @@ int ikd_batch_destroy(ikd_batch_t b) {
         for (auto& d : b->b) dev_free(b, d.p);
         hipStreamSynchronize(b->st);
     }
+    stage_release(b);
     if (b->h_flag) hipHostFree(b->h_flag);
     if (b->ev) hipEventDestroy(b->ev);
     if (b->st) hipStreamDestroy(b->st);
@@ -1847,6 +2019,7 @@ int ikd_batch_destroy(ikd_batch_t b) {
 
 int ikd_batch_set_tuning(ikd_batch_t b, int key, int value) {
     if (!b) return IKD_ERR_ARG;
+    std::lock_guard<std::mutex> batch_lock_(b->mu);
     switch (key) {
         case IKD_TUNE_KNN_STACK:
             if (value < 0 || value > MAX_DEPTH) return IKD_ERR_ARG;
@@ -1866,6 +2039,7 @@ int ikd_batch_nearest_search_device(ikd_batch_t b, const ikd_tree_t* trees, int 
                                     const double* d_max_dist, double max_dist, float* d_out_xyz, float* d_out_d2,
                                     uint32_t* d_out_ids, int32_t* d_out_counts) {
     if (!b) return IKD_ERR_ARG;
+    std::lock_guard<std::mutex> batch_lock_(b->mu);
     if (!trees || ntrees <= 0 || nq < 0 || nq >= (1ll << 31)) {
         b->err = "batch: bad tree list or query count";
         return IKD_ERR_ARG;
@@ -1894,7 +2068,7 @@ int ikd_batch_nearest_search_device(ikd_batch_t b, const ikd_tree_t* trees, int 
     CHK(ensure(b, BB_TREES, ntrees, &d_refs));
     CHK(ensure(b, BB_NEXT, 8, &next));
     CHK(ensure(b, BB_FLAG, 1, &flag));
-    HIPCHK(hipMemcpyAsync(d_refs, refs.data(), ntrees * sizeof(TreeRef), hipMemcpyHostToDevice, b->st));
+    CHK(stage_copy(b, d_refs, refs.data(), ntrees * sizeof(TreeRef)));
     KnnArgs a{};
     a.trees = d_refs;
     a.qtree = d_query_tree;
@@ -1931,6 +2105,7 @@ int ikd_batch_radius_search_device(ikd_batch_t b, const ikd_tree_t* trees, int n
                                    const float* d_centers, const float* d_radii, int64_t nq, int64_t* d_out_offsets,
                                    float* d_out_xyz, uint32_t* d_out_ids, int64_t capacity, int64_t* out_total) {
     if (!b) return IKD_ERR_ARG;
+    std::lock_guard<std::mutex> batch_lock_(b->mu);
     if (!trees || ntrees <= 0 || nq < 0 || nq >= (1ll << 31) || !out_total) {
         b->err = "batch: bad tree list, query count or total pointer";
         return IKD_ERR_ARG;
@@ -1953,7 +2128,7 @@ int ikd_batch_radius_search_device(ikd_batch_t b, const ikd_tree_t* trees, int n
     CHK(ensure(b, BB_POOLS, ntrees, &d_pools));
     CHK(ensure(b, BB_CNT, nq + 1, &cnt));
     CHK(ensure(b, BB_FLAG, 1, &flag));
-    HIPCHK(hipMemcpyAsync(d_pools, pools.data(), ntrees * sizeof(Pool), hipMemcpyHostToDevice, b->st));
+    CHK(stage_copy(b, d_pools, pools.data(), ntrees * sizeof(Pool)));
     launch_fill_i32(reinterpret_cast<int*>(cnt + nq), 0, 2, b->st);
     RangeArgs a{};
     a.pools = d_pools;
@@ -1990,6 +2165,7 @@ int ikd_batch_radius_search_device(ikd_batch_t b, const ikd_tree_t* trees, int n
 
 int ikd_batch_synchronize(ikd_batch_t b) {
     if (!b) return IKD_ERR_ARG;
+    std::lock_guard<std::mutex> batch_lock_(b->mu);
     auto* t = b;
     int* flag;
     CHK(ensure(b, BB_FLAG, 1, &flag));

@@ -55,6 +55,10 @@ constexpr uint32_t M_VIOL = 1u << 9;   // violates Criterion_Check after refit
 constexpr uint32_t M_TOP = 1u << 10;   // topmost violator: subtree is rebuilt
 constexpr uint32_t M_CLAIM = 1u << 11; // claimed by a Delete_Points lane
 constexpr uint32_t M_DEFER = 1u << 12; // violator whose rebuild was deferred (listed once)
+// Delete_Points leaves the node it deletes without Update (cpp:717-721); the
+// refit then keeps its range as of its last Update, when its point was live:
+constexpr uint32_t M_FROZEN = 1u << 13;  // range from the children and the own point as if live, no Criterion_Check
+constexpr uint32_t M_KEEPBOX = 1u << 14; // not Updated at all in this call: the stored range stays
 
 constexpr int SENTINEL = 0;
 constexpr int ROOT = 2;

@@ -32,6 +32,8 @@ struct DevHdr {
     int front_over;     // a box-deletion frontier outgrew its buffer
     int ins_count;      // downsample: voxels whose fold keeps a point to insert
     int removed_count;  // points rebuilds removed since the last acquire (kept across calls)
+    float alpha_bal;    // the root's alpha_bal / alpha_del as of its last Update (cpp:1315-1321)
+    float alpha_del;
 };
 static_assert(sizeof(DevHdr) % sizeof(int) == 0, "the header is cleared as ints");
 constexpr int DEFER_CAP = 4096;  // deferred-rebuild list entries
@@ -109,6 +111,8 @@ void launch_defer_check(Pool pool, Params prm, const int* defer, int n, int4* in
 void launch_defer_clear(Pool pool, const int* defer, int n, hipStream_t s);
 void launch_find_top(Pool pool, const int* viol, int bound, DevHdr* hdr, int* top, hipStream_t s);
 void launch_clear_viol(Pool pool, const int* viol, int bound, DevHdr* hdr, hipStream_t s);
+// root statistics of a root the build just created (0 when it has <= 3 nodes)
+void launch_root_alpha_fresh(Pool pool, DevHdr* hdr, hipStream_t s);
 
 // ---------------------------------------------------------------- enumerate
 // rec (nullable): points of deleted nodes that the downsample did not delete
@@ -134,12 +138,27 @@ void launch_insert_link(Pool pool, const int* attach, const int4* seg, DevHdr* h
 // levels [L0, L1) of the breadth-first box deletion (L0 == 0 seeds the frontier)
 void launch_box_delete_bfs(Pool pool, const float* boxes, int nb, int2* f0, int2* f1, int fcap, int L0, int L1,
                            unsigned int* recs, DevHdr* hdr, int cap, hipStream_t s);
+// Add_by_range for one box: levels [L0, L1) of the walk (L0 == 0 starts it);
+// then the reset of the covered subtrees (`levels` levels) and the refit
+// seeds (-1 or a cut node) in seeds[0, nrec).
+void launch_addbox_walk(Pool pool, const float* box, int2* f0, int2* f1, int fcap, int L0, int L1,
+                        unsigned int* recs, DevHdr* hdr, int cap, hipStream_t s);
+void launch_addbox_reset(Pool pool, const unsigned int* recs, int nrec, int* seeds, int2* f0, int2* f1, int fcap,
+                         int levels, DevHdr* hdr, hipStream_t s);
 void launch_box_delete_apply(Pool pool, const unsigned int* recs, DevHdr* hdr, int bound, int ds, int* seeds,
                              hipStream_t s);
+// Delete_Points: find claims each point's node (conflicts counted); apply
+// (conflict-free batches) works out which nodes the reference's walks Update
+// (per-node del_below = 0, max_del = -1, last_pass = -1 before the call;
+// per-point path_seed) and labels the deleted nodes; seeds get 2n entries
+// (deleted node, deepest Updated node) per point.  serial: the ordered
+// fallback, same outputs.
 void launch_delete_points_find(Pool pool, const float* xyz, int n, int* cand, DevHdr* hdr, hipStream_t s);
-void launch_delete_points_apply(Pool pool, const int* cand, int n, int* seeds, hipStream_t s);
+void launch_delete_points_apply(Pool pool, const float* xyz, const int* cand, int* del_below, int* max_del,
+                                int* last_pass, int* path_seed, int n, int* seeds, DevHdr* hdr, hipStream_t s);
 void launch_delete_points_unclaim(Pool pool, const int* cand, int n, int* seeds, hipStream_t s);
-void launch_delete_points_serial(Pool pool, const float* xyz, int n, int* seeds, hipStream_t s);
+void launch_delete_points_serial(Pool pool, const float* xyz, int n, int* last_pass, int* path_seed, int* seeds,
+                                 DevHdr* hdr, hipStream_t s);
 
 // ---------------------------------------------------------------- downsample
 void launch_voxel_keys2(const float* xyz, const uint32_t* ids, uint32_t id_base, int n, float ds,

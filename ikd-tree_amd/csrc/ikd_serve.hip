@@ -71,14 +71,20 @@ struct ikd_server_s {
     bool stop = false;
     std::string err;
 
-    int grow(SlotBufs& b, long long n, int k, ikd_request_t* r) {
+    // Slot buffers grow with stream-ordered allocations on the request's
+    // stream (hipFree would wait for every stream of the device).  The slot's
+    // previous request ended with a host wait on its own stream, so nothing
+    // still reads a buffer freed here.
+    template <class T>
+    static bool regrow(T** p, long long count, size_t elem, hipStream_t st) {
+        if (*p) hipFreeAsync(*p, st);
+        *p = nullptr;
+        return hipMallocAsync((void**)p, (size_t)count * elem, st) == hipSuccess;
+    }
+    int grow(SlotBufs& b, long long n, int k, ikd_request_t* r, hipStream_t st) {
         if (n > b.cap_pts) {
-            if (b.xyz) hipFree(b.xyz);
-            if (b.out_cnt) hipFree(b.out_cnt);
-            b.xyz = nullptr;
-            b.out_cnt = nullptr;
             long long c = std::max(n, b.cap_pts + b.cap_pts / 2);
-            if (hipMalloc(&b.xyz, c * 12) != hipSuccess || hipMalloc(&b.out_cnt, c * 4) != hipSuccess) {
+            if (!regrow(&b.xyz, c, 12, st) || !regrow(&b.out_cnt, c, 4, st)) {
                 b.cap_pts = 0;
                 r->status = IKD_ERR_NOMEM;
                 return IKD_ERR_NOMEM;
@@ -87,14 +93,8 @@ struct ikd_server_s {
         }
         long long res = n * std::max(k, 1);
         if (res > b.cap_res) {
-            if (b.out_xyz) hipFree(b.out_xyz);
-            if (b.out_d2) hipFree(b.out_d2);
-            if (b.out_ids) hipFree(b.out_ids);
-            b.out_xyz = b.out_d2 = nullptr;
-            b.out_ids = nullptr;
             long long c = std::max(res, b.cap_res + b.cap_res / 2);
-            if (hipMalloc(&b.out_xyz, c * 12) != hipSuccess || hipMalloc(&b.out_d2, c * 4) != hipSuccess ||
-                hipMalloc(&b.out_ids, c * 4) != hipSuccess) {
+            if (!regrow(&b.out_xyz, c, 12, st) || !regrow(&b.out_d2, c, 4, st) || !regrow(&b.out_ids, c, 4, st)) {
                 b.cap_res = 0;
                 r->status = IKD_ERR_NOMEM;
                 return IKD_ERR_NOMEM;
@@ -104,16 +104,16 @@ struct ikd_server_s {
         return IKD_OK;
     }
 
-    // the request's work, on its map's stream
-    int run(SlotBufs& b, ikd_request_t* r) {
-        ikd_tree_t t = maps[r->map];
+    // the request's work, on its map's stream (t: the request's map, read
+    // under the server lock at admission)
+    int run(SlotBufs& b, ikd_request_t* r, ikd_tree_t t) {
         void* sv = nullptr;
         int rc = ikd_stream(t, &sv);
         if (rc != IKD_OK) return rc;
         hipStream_t st = (hipStream_t)sv;
         const long long n = r->n;
         if (n > 0) {
-            rc = grow(b, n, r->k, r);
+            rc = grow(b, n, r->k, r, st);
             if (rc != IKD_OK) return rc;
             if (hipMemcpyAsync(b.xyz, r->xyz, n * 12, hipMemcpyHostToDevice, st) != hipSuccess) return IKD_ERR_HIP;
         }
@@ -172,15 +172,17 @@ struct ikd_server_s {
             busy[r->map] = 1;
             in_flight_bytes += need;
             ++in_flight;
+            // the map list may grow (and move) while the lock is released
+            const int map = r->map;
+            ikd_tree_t tree = maps[map];
             lk.unlock();
             // the caller may release the request as soon as it is handed back:
             // nothing below reads it after that
-            const int map = r->map;
             r->t_start_ns = now_ns();
             ikd_counters_t c0{}, c1{};
-            ikd_get_counters(maps[map], &c0);
-            int rc = run(b, r);
-            ikd_get_counters(maps[map], &c1);
+            ikd_get_counters(tree, &c0);
+            int rc = run(b, r, tree);
+            ikd_get_counters(tree, &c1);
             r->t_done_ns = now_ns();
             r->delta.rebuilds = c1.rebuilds - c0.rebuilds;
             r->delta.rebuilt_points = c1.rebuilt_points - c0.rebuilt_points;
@@ -204,7 +206,7 @@ struct ikd_server_s {
                 if (waiting) break;
                 lk.unlock();
                 int32_t nreb = 0;
-                int mrc = ikd_rebuild_deferred(maps[map], maintain_points, &nreb);
+                int mrc = ikd_rebuild_deferred(tree, maintain_points, &nreb);
                 lk.lock();
                 maintenance_runs += nreb;
                 if (mrc != IKD_OK || nreb == 0 || maintain_points < 0) break;
@@ -304,6 +306,8 @@ int ikd_server_destroy(ikd_server_t s) {
         s->cv_work.notify_all();
     }
     for (auto& w : s->workers) w.join();
+    // the workers' last requests ended with a wait on their streams
+    hipDeviceSynchronize();
     for (auto& b : s->bufs) {
         if (b.xyz) hipFree(b.xyz);
         if (b.out_xyz) hipFree(b.out_xyz);

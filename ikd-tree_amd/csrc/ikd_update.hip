@@ -105,7 +105,11 @@ __device__ void refit_node(Pool pool, Params prm, int u, int* viol, int* defer, 
     }
     bool tdel = all_td && pdel;
     bool tds = all_tds && pds;
-    bool everything = tdel || (!any_td && !pdel);
+    // a node Delete_Points deleted and no later point of the call passed
+    // through: its range is the one of its last Update, own point live
+    const bool frozen = (m & M_FROZEN) != 0;
+    const bool rdel = frozen ? false : pdel;
+    bool everything = (frozen ? false : tdel) || (!any_td && !rdel);
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
@@ -116,15 +120,22 @@ __device__ void refit_node(Pool pool, Params prm, int u, int* viol, int* defer, 
             lo[2] = smin(lo[2], b[2]); hi[2] = smax(hi[2], b[5]);
         }
     }
-    if (everything || !pdel) {
+    if (everything || !rdel) {
         lo[0] = smin(lo[0], h.f[0]); hi[0] = smax(hi[0], h.f[0]);
         lo[1] = smin(lo[1], h.f[1]); hi[1] = smax(hi[1], h.f[1]);
         lo[2] = smin(lo[2], h.f[2]); hi[2] = smax(hi[2], h.f[2]);
     }
     int depth = pool.cnt[u].w;
     pool.cnt[u] = make_int4(size, inv, dd, depth);
-    uint32_t nm = (m & ~(M_TDEL | M_TDS | M_DIRTY | M_VIOL | M_TOP)) | (tdel ? M_TDEL : 0u) | (tds ? M_TDS : 0u);
-    if (size > MIN_UNBALANCED_SIZE) {
+    if (u == ROOT && size > 3 && !frozen) {
+        // Update's root statistics, "for paper data" (cpp:1315-1321)
+        const float bal = (float)(has[0] ? csize[0] : csize[1]) / (float)(size - 1);
+        hdr->alpha_del = (float)inv / (float)size;
+        hdr->alpha_bal = ((double)bal >= 0.5 - EPSS) ? bal : 1.0f - bal;
+    }
+    uint32_t nm = (m & ~(M_TDEL | M_TDS | M_DIRTY | M_VIOL | M_TOP | M_FROZEN | M_KEEPBOX)) | (tdel ? M_TDEL : 0u) |
+                  (tds ? M_TDS : 0u);
+    if (size > MIN_UNBALANCED_SIZE && !frozen) {
         int son = has[0] ? csize[0] : csize[1];
         float del_eval = (float)inv / (float)size;
         float bal_eval = (float)son / (float)(size - 1);
@@ -159,7 +170,7 @@ __device__ void refit_node(Pool pool, Params prm, int u, int* viol, int* defer, 
     pool.meta[u] = nm;
     int p = pool.par[u];
     float box[6] = {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]};
-    write_child_box(pool.hot, p, u & 1, box);
+    if (!(m & M_KEEPBOX)) write_child_box(pool.hot, p, u & 1, box);
     uint32_t tdbit = lk_td(u & 1);
     if (tdel) atomicOr(link_ptr(pool.hot, p), tdbit);
     else atomicAnd(link_ptr(pool.hot, p), ~tdbit);
@@ -296,6 +307,31 @@ __global__ void k_clear_viol(Pool pool, const int* __restrict__ viol, DevHdr* hd
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= hdr->viol_count) return;
     atomicAnd(&pool.meta[viol[2 * i]], ~M_VIOL);
+}
+
+// A root the build just made is a new node: BuildTree's Update of it sets the
+// statistics when it has more than 3 nodes (cpp:1315); a smaller new root
+// has none (0 here, the reference leaves them unset).
+__global__ void k_root_alpha_fresh(Pool pool, DevHdr* hdr) {
+    float ab = 0.0f, ad = 0.0f;
+    const uint32_t lk0 = fbits(pool.hot[SENTINEL].f[3]);
+    if (lk0 & LK_HASL) {
+        const int4 c = pool.cnt[ROOT];
+        if (c.x > 3) {
+            const uint32_t lk = fbits(pool.hot[ROOT].f[3]);
+            const int pair = (int)lk_pair(lk);
+            const int son = (lk & LK_HASL) ? pool.cnt[2 * pair].x : pool.cnt[2 * pair + 1].x;
+            const float bal = (float)son / (float)(c.x - 1);
+            ad = (float)c.y / (float)c.x;
+            ab = ((double)bal >= 0.5 - EPSS) ? bal : 1.0f - bal;
+        }
+    }
+    hdr->alpha_bal = ab;
+    hdr->alpha_del = ad;
+}
+
+void launch_root_alpha_fresh(Pool pool, DevHdr* hdr, hipStream_t s) {
+    hipLaunchKernelGGL(k_root_alpha_fresh, dim3(1), dim3(1), 0, s, pool, hdr);
 }
 
 void launch_mark_dirty(Pool pool, const int* seeds, const int* nseeds_dev, int bound, int stop_at_tdel,
@@ -584,6 +620,12 @@ __global__ void k_insert_link(Pool pool, const int* __restrict__ attach, const i
     int key = attach[g];
     int X = key >> 1, side = key & 1;
     int slot = seg[g].z;
+    if (seg[g].y == seg[g].x) {
+        // a single point becomes a leaf split on its parent's axis + 1
+        // (Add_by_point, cpp:823; the sentinel's axis is 0)
+        const uint32_t axis = ((pool.meta[X] & M_AXIS) + 1) % 3;
+        pool.meta[slot] = (pool.meta[slot] & ~M_AXIS) | axis;
+    }
     uint32_t bits = lk_has(side) | ((uint32_t)(slot >> 1) << LK_PAIR_SHIFT);
     atomicOr(link_ptr(pool.hot, X), bits);
     atomicAnd(link_ptr(pool.hot, X), ~lk_td(side));
@@ -623,6 +665,7 @@ void launch_insert_link(Pool pool, const int* attach, const int4* seg, DevHdr* h
 // inside the half-open box (the subtree root takes the lazy label), POINT
 // for a single live node point inside it.
 constexpr uint32_t REC_COVER = 0x80000000u;
+constexpr uint32_t REC_PARTIAL = 0x40000000u;  // a node the box cuts: Updated, labels unchanged
 
 // Pass 1 runs breadth-first over all boxes at once: a frontier of (node, box)
 // pairs per tree level, one lane per pair, so a box that cuts through much of
@@ -659,8 +702,11 @@ __global__ void k_box_delete_level(Pool pool, const float* __restrict__ boxes, c
         float f[16];
         load_record(pool.hot[u].f, f);
         const uint32_t lk = fbits(f[3]);
+        // a node the box cuts is Updated on the way out (cpp:703-704)
+        int pos = atomicAdd(&hdr->rec_count, 1);
+        if (pos < cap) recs[pos] = REC_PARTIAL | (uint32_t)u;
         if (!(lk & LK_PDEL) && box_contains(box, f[0], f[1], f[2])) {
-            int pos = atomicAdd(&hdr->rec_count, 1);
+            pos = atomicAdd(&hdr->rec_count, 1);
             if (pos < cap) recs[pos] = (uint32_t)u;
         }
         const int pair = (int)lk_pair(lk);
@@ -671,10 +717,10 @@ __global__ void k_box_delete_level(Pool pool, const float* __restrict__ boxes, c
             if (box_disjoint(box, cb)) continue;
             const int c = 2 * pair + s;
             if (box_covers(box, cb)) {
-                int pos = atomicAdd(&hdr->rec_count, 1);
+                pos = atomicAdd(&hdr->rec_count, 1);
                 if (pos < cap) recs[pos] = REC_COVER | (uint32_t)c;
             } else {
-                int pos = atomicAdd(&hdr->front3[(lvl + 1) % 3], 1);
+                pos = atomicAdd(&hdr->front3[(lvl + 1) % 3], 1);
                 if (pos < fcap) front_next[pos] = make_int2(c, e.y);
                 else hdr->front_over = 1;
             }
@@ -704,8 +750,10 @@ __global__ void k_box_delete_apply(Pool pool, const unsigned int* __restrict__ r
     if (i >= bound) return;
     if (i >= hdr->rec_count) return;
     uint32_t r = recs[i];
-    int u = (int)(r & ~REC_COVER);
-    if (r & REC_COVER) {
+    int u = (int)(r & ~(REC_COVER | REC_PARTIAL));
+    if (r & REC_PARTIAL) {
+        seeds[i] = u;
+    } else if (r & REC_COVER) {
         uint32_t bits = M_PDEL | M_TDEL | (ds ? (M_PDS | M_TDS) : 0u);
         atomicOr(&pool.meta[u], bits);
         atomicOr(link_ptr(pool.hot, u), LK_PDEL);
@@ -727,6 +775,193 @@ void launch_box_delete_apply(Pool pool, const unsigned int* recs, DevHdr* hdr, i
     if (bound <= 0) return;
     hipLaunchKernelGGL(k_box_delete_apply, dim3(blocks_for(bound, 256)), dim3(256), 0, s, pool, recs, hdr, bound, ds,
                        seeds);
+}
+
+// ====================================================== box re-insertion
+// Add_by_range (cpp:762-815) for one box.  Breadth-first from the root over
+// the nodes whose stored range the box cuts, entering deleted subtrees too
+// (the reference does not stop at tree_deleted here).  A cut node first
+// hands its deletion label to its children (Push_Down, cpp:1109-1181, which
+// the device applies only along the paths it walks), gets its own point back
+// if the box holds it and the downsample did not delete it (cpp:784-786),
+// and becomes a refit seed (it is Updated on the way out, cpp:809-812).  A
+// child whose range the box covers is recorded for the reset below; a child
+// the box cuts goes to the next level.  front3[lvl % 3] holds the level's
+// frontier size, rec_count the records.
+__global__ void k_addbox_init(Pool pool, const float* __restrict__ box, int2* __restrict__ front,
+                              unsigned int* __restrict__ recs, DevHdr* hdr) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    hdr->front3[0] = 0;
+    hdr->front3[1] = 0;
+    hdr->front3[2] = 0;
+    hdr->rec_count = 0;
+    hdr->front_over = 0;
+    const uint32_t lk0 = load_link(pool.hot, SENTINEL);
+    if (!(lk0 & LK_HASL)) return;
+    const float* rb = pool.hot[SENTINEL].f + 4;
+    if (box_disjoint(box, rb)) return;
+    if (box_covers(box, rb)) {
+        recs[0] = REC_COVER | (uint32_t)ROOT;
+        hdr->rec_count = 1;
+        return;
+    }
+    front[0] = make_int2(ROOT, 0);
+    hdr->front3[0] = 1;
+}
+
+__global__ void k_addbox_level(Pool pool, const float* __restrict__ box, const int2* __restrict__ front, int lvl,
+                               int2* __restrict__ front_next, unsigned int* __restrict__ recs, DevHdr* hdr, int fcap,
+                               int cap) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid == 0) hdr->front3[(lvl + 2) % 3] = 0;
+    const int count = min(hdr->front3[lvl % 3], fcap);
+    for (int i = tid; i < count; i += gridDim.x * blockDim.x) {
+        const int u = front[i].x;
+        const uint32_t m = pool.meta[u];
+        const uint32_t lk = load_link(pool.hot, u);
+        const int pair = (int)lk_pair(lk);
+        if (m & M_TDEL) {
+            // Push_Down of a deleted node: the children take its labels
+            const bool ds = (m & M_TDS) != 0;
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                if (!(lk & lk_has(s))) continue;
+                push_deleted(pool, 2 * pair + s, ds);
+                atomicOr(link_ptr(pool.hot, u), lk_td(s));
+            }
+        }
+        const float* f = pool.hot[u].f;
+        if (box_contains(box, f[0], f[1], f[2])) {
+            // point_deleted := point_downsample_deleted
+            if (m & M_PDS) {
+                atomicOr(&pool.meta[u], M_PDEL);
+                atomicOr(link_ptr(pool.hot, u), LK_PDEL);
+            } else {
+                atomicAnd(&pool.meta[u], ~M_PDEL);
+                atomicAnd(link_ptr(pool.hot, u), ~LK_PDEL);
+            }
+        }
+        int pos = atomicAdd(&hdr->rec_count, 1);
+        if (pos < cap) recs[pos] = REC_PARTIAL | (uint32_t)u;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            if (!(lk & lk_has(s))) continue;
+            const float* cb = f + 4 + 6 * s;
+            if (box_disjoint(box, cb)) continue;
+            const int c = 2 * pair + s;
+            if (box_covers(box, cb)) {
+                pos = atomicAdd(&hdr->rec_count, 1);
+                if (pos < cap) recs[pos] = REC_COVER | (uint32_t)c;
+            } else {
+                pos = atomicAdd(&hdr->front3[(lvl + 1) % 3], 1);
+                if (pos < fcap) front_next[pos] = make_int2(c, 0);
+                else hdr->front_over = 1;
+            }
+        }
+    }
+}
+
+// Records -> refit seeds (cut nodes) and the first level of the reset
+// (covered subtree roots, bit 0 set).  front3 must be zero.
+__global__ void k_addbox_split(const unsigned int* __restrict__ recs, DevHdr* hdr, int bound, int* __restrict__ seeds,
+                               int2* __restrict__ rfront) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= bound || i >= hdr->rec_count) return;
+    const uint32_t r = recs[i];
+    const int u = (int)(r & ~(REC_COVER | REC_PARTIAL));
+    if (r & REC_COVER) {
+        seeds[i] = -1;
+        const int pos = atomicAdd(&hdr->front3[0], 1);
+        rfront[pos] = make_int2(u, 1);  // at most one record per node: pos < bound
+    } else {
+        seeds[i] = u;
+    }
+}
+
+// Add_by_range's covered case (cpp:777-783) and what Push_Down then hands
+// down the subtree (cpp:1115-1131): every node goes back to its downsample
+// state, deleted iff the downsample deleted it or a subtree it lies in, with
+// invalid_point_num := down_del_num.  The reference hands the labels down
+// lazily as later calls walk the subtree; the result each walk sees is the
+// same as rewriting the whole subtree now, level by level.  Ranges stay as
+// they are (nothing below the covered node is Updated).  Entry (node, bits):
+// bit 0 the covered root itself, bit 1 the parent's new
+// tree_downsample_deleted, bit 2 the parent's new tree_deleted.
+__global__ void k_addbox_reset(Pool pool, const int2* __restrict__ front, int lvl, int2* __restrict__ front_next,
+                               DevHdr* hdr, int fcap) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid == 0) hdr->front3[(lvl + 2) % 3] = 0;
+    const int count = min(hdr->front3[lvl % 3], fcap);
+    for (int i = tid; i < count; i += gridDim.x * blockDim.x) {
+        const int2 e = front[i];
+        const int v = e.x;
+        const uint32_t m = pool.meta[v];
+        const int4 c = pool.cnt[v];
+        bool tds, pds, td, pd;
+        int dd = c.z, inv;
+        if (e.y & 1) {
+            // tree_deleted := tree_downsample_deleted, point_deleted :=
+            // point_downsample_deleted, invalid := down_del (cpp:778-782)
+            tds = (m & M_TDS) != 0;
+            pds = (m & M_PDS) != 0;
+            td = tds;
+            pd = pds;
+            inv = dd;
+            const int p = pool.par[v];
+            if (td) atomicOr(link_ptr(pool.hot, p), lk_td(v & 1));
+            else atomicAnd(link_ptr(pool.hot, p), ~lk_td(v & 1));
+        } else {
+            // Push_Down from the reset parent (cpp:1115-1123)
+            const bool ptds = (e.y & 2) != 0, ptd = (e.y & 4) != 0;
+            tds = (m & M_TDS) || ptds;
+            pds = (m & M_PDS) || ptds;
+            td = ptd || tds;
+            pd = td || pds;
+            if (ptds) dd = c.x;
+            inv = ptd ? c.x : dd;
+        }
+        pool.meta[v] = (m & ~(M_PDEL | M_TDEL | M_PDS | M_TDS)) | (pd ? M_PDEL : 0u) | (td ? M_TDEL : 0u) |
+                       (pds ? M_PDS : 0u) | (tds ? M_TDS : 0u);
+        pool.cnt[v] = make_int4(c.x, inv, dd, c.w);
+        const uint32_t lk = load_link(pool.hot, v);
+        const int pair = (int)lk_pair(lk);
+        uint32_t nl = lk & ~(LK_PDEL | LK_TDL | LK_TDR);
+        if (pd) nl |= LK_PDEL;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            if (!(lk & lk_has(s))) continue;
+            const int ch = 2 * pair + s;
+            const bool ch_td = td || (pool.meta[ch] & M_TDS) || tds;
+            if (ch_td) nl |= lk_td(s);
+            const int pos = atomicAdd(&hdr->front3[(lvl + 1) % 3], 1);
+            if (pos < fcap) front_next[pos] = make_int2(ch, (tds ? 2 : 0) | (td ? 4 : 0));
+            else hdr->front_over = 1;
+        }
+        // only this lane writes v's own link word in this launch
+        *link_ptr(pool.hot, v) = nl;
+    }
+}
+
+void launch_addbox_walk(Pool pool, const float* box, int2* f0, int2* f1, int fcap, int L0, int L1,
+                        unsigned int* recs, DevHdr* hdr, int cap, hipStream_t s) {
+    if (L0 == 0) hipLaunchKernelGGL(k_addbox_init, dim3(1), dim3(64), 0, s, pool, box, f0, recs, hdr);
+    for (int L = L0; L < L1; ++L) {
+        const int2* fin = (L & 1) ? f1 : f0;
+        int2* fout = (L & 1) ? f0 : f1;
+        hipLaunchKernelGGL(k_addbox_level, dim3(512), dim3(256), 0, s, pool, box, fin, L, fout, recs, hdr, fcap, cap);
+    }
+}
+
+void launch_addbox_reset(Pool pool, const unsigned int* recs, int nrec, int* seeds, int2* f0, int2* f1, int fcap,
+                         int levels, DevHdr* hdr, hipStream_t s) {
+    if (nrec <= 0) return;
+    launch_fill_i32(hdr->front3, 0, 3, s);
+    hipLaunchKernelGGL(k_addbox_split, dim3(blocks_for(nrec, 256)), dim3(256), 0, s, recs, hdr, nrec, seeds, f0);
+    for (int L = 0; L < levels; ++L) {
+        const int2* fin = (L & 1) ? f1 : f0;
+        int2* fout = (L & 1) ? f0 : f1;
+        hipLaunchKernelGGL(k_addbox_reset, dim3(512), dim3(256), 0, s, pool, fin, L, fout, hdr, fcap);
+    }
 }
 
 // ========================================================== point deletion
@@ -764,34 +999,126 @@ __global__ void k_delete_points_find(Pool pool, const float* __restrict__ xyz, i
     }
 }
 
-// mode 0: apply the (conflict-free) candidates; mode 1: only drop the claims.
-__global__ void k_delete_points_apply(Pool pool, const int* __restrict__ cand, int n, int* __restrict__ seeds,
-                                      int mode) {
+// Which nodes the reference's walks Update.  Point i's walk (cpp:712-760)
+// Updates every node it leaves for a child slot; it does not Update the node
+// it deletes (cpp:717-721) and stops at a node that is tree_deleted by then,
+// which includes nodes the call's earlier points emptied.  With the
+// conflict-free candidates known, node w is emptied by this call at
+// max_del[w] (the last batch index deleting inside its subtree) iff its
+// pre-call invalid count plus the call's deletions below it reaches TreeSize.
+__global__ void k_delete_points_times(Pool pool, const int* __restrict__ cand, int n, int* __restrict__ del_below,
+                                      int* __restrict__ max_del, DevHdr* hdr) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int u = cand[i];
-    seeds[i] = -1;
+    for (int step = 0; u >= ROOT; ++step) {
+        if (step > WALK_LIMIT) {
+            atomicOr(&hdr->overflow, 16);
+            return;
+        }
+        atomicAdd(&del_below[u], 1);
+        atomicMax(&max_del[u], i);
+        u = pool.par[u];
+    }
+}
+
+// last_pass[w]: the highest batch index whose walk Updates w (-1: none);
+// path_seed[i]: the deepest node point i's walk Updates (-1: none).
+__device__ void walk_updates(Pool pool, float x, float y, float z, int i, int found, const int* del_below,
+                             const int* max_del, int* last_pass, int* path_seed) {
+    int seed = -1;
+    uint32_t lk0 = load_link(pool.hot, SENTINEL);
+    int u = (lk0 & LK_HASL) ? ROOT : -1;
+    for (int step = 0; u >= 0 && step <= WALK_LIMIT; ++step) {
+        if (u == found) break;
+        const uint32_t m = pool.meta[u];
+        if (m & M_TDEL) break;
+        if (del_below) {
+            const int4 c = pool.cnt[u];
+            if (c.y + del_below[u] == c.x && max_del[u] < i) break;  // emptied earlier in the call
+        }
+        atomicMax(&last_pass[u], i);
+        seed = u;
+        const float* f = pool.hot[u].f;
+        const uint32_t lk = fbits(f[3]);
+        const int axis = (int)(m & M_AXIS);
+        const float pc = axis == 0 ? x : (axis == 1 ? y : z);
+        const int side = pc < f[axis] ? 0 : 1;
+        u = (lk & lk_has(side)) ? 2 * (int)lk_pair(lk) + side : -1;
+    }
+    path_seed[i] = seed;
+}
+
+__global__ void k_delete_points_walk(Pool pool, const float* __restrict__ xyz, const int* __restrict__ cand, int n,
+                                     const int* __restrict__ del_below, const int* __restrict__ max_del,
+                                     int* __restrict__ last_pass, int* __restrict__ path_seed) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    walk_updates(pool, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], i, cand[i], del_below, max_del, last_pass,
+                 path_seed);
+}
+
+// The labels a deleted node u (batch index i) takes for the refit: frozen
+// range unless a later point's walk Updates it; the stored range kept as is
+// if no walk of the call Updates it at all.
+__device__ __forceinline__ uint32_t freeze_bits(const int* last_pass, int u, int i) {
+    const int lp = last_pass[u];
+    if (lp > i) return 0u;
+    return M_FROZEN | (lp < 0 ? M_KEEPBOX : 0u);
+}
+
+// mode 0: apply the (conflict-free) candidates; mode 1: only drop the claims.
+// seeds[2i] = deleted node, seeds[2i+1] = deepest Updated node of the walk.
+__global__ void k_delete_points_apply(Pool pool, const int* __restrict__ cand, const int* __restrict__ last_pass,
+                                      const int* __restrict__ path_seed, int n, int* __restrict__ seeds, int mode) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int u = cand[i];
+    seeds[2 * i] = -1;
+    seeds[2 * i + 1] = mode == 0 ? path_seed[i] : -1;
     if (u < 0) return;
     atomicAnd(&pool.meta[u], ~M_CLAIM);
     if (mode == 0) {
-        atomicOr(&pool.meta[u], M_PDEL);
+        atomicOr(&pool.meta[u], M_PDEL | freeze_bits(last_pass, u, i));
         atomicOr(link_ptr(pool.hot, u), LK_PDEL);
-        seeds[i] = u;
+        seeds[2 * i] = u;
     }
 }
 
 // Ordered fallback when two points of one batch reach the same node: one
 // lane walks the batch in order, as the reference's loop does (cpp:514-531).
-__global__ void k_delete_points_serial(Pool pool, const float* __restrict__ xyz, int n, int* __restrict__ seeds) {
+// A deletion raises invalid_point_num up the path and labels every node it
+// empties tree_deleted, so later walks stop there; the refit recomputes
+// both from the children afterwards.
+__global__ void k_delete_points_serial(Pool pool, const float* __restrict__ xyz, int n, int* __restrict__ last_pass,
+                                       int* __restrict__ path_seed, int* __restrict__ seeds, DevHdr* hdr) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     for (int i = 0; i < n; ++i) {
-        int u = find_point(pool, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
-        seeds[i] = u;
-        if (u >= 0) {
-            pool.meta[u] |= M_PDEL;
-            *link_ptr(pool.hot, u) |= LK_PDEL;
-            __threadfence();
+        const float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        const int u = find_point(pool, x, y, z);
+        walk_updates(pool, x, y, z, i, u, nullptr, nullptr, last_pass, path_seed);
+        seeds[2 * i] = u;
+        seeds[2 * i + 1] = path_seed[i];
+        if (u < 0) continue;
+        pool.meta[u] |= M_PDEL;
+        *link_ptr(pool.hot, u) |= LK_PDEL;
+        int a = u;
+        for (int step = 0; a >= ROOT; ++step) {
+            if (step > WALK_LIMIT) {
+                hdr->overflow |= 16;
+                return;
+            }
+            int4 c = pool.cnt[a];
+            c.y += 1;
+            pool.cnt[a] = c;
+            if (c.y == c.x) pool.meta[a] |= M_TDEL;
+            a = pool.par[a];
         }
+        __threadfence();
+    }
+    for (int i = 0; i < n; ++i) {
+        const int u = seeds[2 * i];
+        if (u >= 0) pool.meta[u] |= freeze_bits(last_pass, u, i);
     }
 }
 
@@ -799,17 +1126,26 @@ void launch_delete_points_find(Pool pool, const float* xyz, int n, int* cand, De
     if (n <= 0) return;
     hipLaunchKernelGGL(k_delete_points_find, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, xyz, n, cand, hdr);
 }
-void launch_delete_points_apply(Pool pool, const int* cand, int n, int* seeds, hipStream_t s) {
+void launch_delete_points_apply(Pool pool, const float* xyz, const int* cand, int* del_below, int* max_del,
+                                int* last_pass, int* path_seed, int n, int* seeds, DevHdr* hdr, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_delete_points_apply, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, cand, n, seeds, 0);
+    hipLaunchKernelGGL(k_delete_points_times, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, cand, n, del_below,
+                       max_del, hdr);
+    hipLaunchKernelGGL(k_delete_points_walk, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, xyz, cand, n, del_below,
+                       max_del, last_pass, path_seed);
+    hipLaunchKernelGGL(k_delete_points_apply, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, cand, last_pass,
+                       path_seed, n, seeds, 0);
 }
 void launch_delete_points_unclaim(Pool pool, const int* cand, int n, int* seeds, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_delete_points_apply, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, cand, n, seeds, 1);
+    hipLaunchKernelGGL(k_delete_points_apply, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, cand, nullptr, nullptr,
+                       n, seeds, 1);
 }
-void launch_delete_points_serial(Pool pool, const float* xyz, int n, int* seeds, hipStream_t s) {
+void launch_delete_points_serial(Pool pool, const float* xyz, int n, int* last_pass, int* path_seed, int* seeds,
+                                 DevHdr* hdr, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_delete_points_serial, dim3(1), dim3(64), 0, s, pool, xyz, n, seeds);
+    hipLaunchKernelGGL(k_delete_points_serial, dim3(1), dim3(64), 0, s, pool, xyz, n, last_pass, path_seed, seeds,
+                       hdr);
 }
 
 // ================================================================ misc

@@ -6,27 +6,52 @@
 // output conventions (ikd_Tree.h:226-248): outputs go into caller-owned
 // vectors that are cleared first, Nearest_Search returns points in ascending
 // squared distance, Box/Radius_Search return the live points found.  The
-// differences a caller can observe:
+// public members Root_Node, PCL_Storage and max_queue_size, the Ptr alias and
+// the delete_point_storage_set modes are there too (ikd_Tree.h:39, 62-63,
+// 245-248).  The differences a caller can observe:
 //   - errors throw std::runtime_error where the reference is undefined
 //     (k <= 0, k > 64);
 //   - Add_Points works on an empty tree (the reference dereferences a null
 //     root there, ikd_Tree.cpp:447);
-//   - flatten() takes only the output vector (the reference's takes a node
-//     pointer and a record mode, ikd_Tree.cpp:1325), and returns every live
-//     point;
-//   - PointVector uses std::allocator instead of Eigen::aligned_allocator.
-// Payload types (e.g. pcl::PointXYZI) round-trip unchanged: every point the
-// tree ever received is kept host-side under its payload id and results are
-// looked up by id, so fields beyond x, y, z come back as they went in.
+//   - Root_Node is an opaque handle: null while the tree has no nodes (as the
+//     reference's is before Build), else the one node flatten() accepts,
+//     standing for the whole tree; flatten() supports the NOT_RECORD mode;
+//   - the library's rebuilds do not use PCL_Storage as scratch (the
+//     reference's Rebuild leaves the last rebuilt subtree's points in it);
+//   - PointVector uses std::allocator unless Eigen is included first (then
+//     Eigen::aligned_allocator, as the reference, so PCL point vectors and
+//     PointVector assign to each other).
+// Payload types (e.g. pcl::PointXYZI) round-trip unchanged: the facade keeps
+// every point the tree may still return host-side under its payload id and
+// looks results up by id, so fields beyond x, y, z come back as they went in.
+//
+// Threading: one writer per tree, as in the reference; reads (Nearest_Search,
+// Box_Search, Radius_Search) may come from many threads at once, e.g. an
+// OpenMP loop of Nearest_Search calls: the library serializes calls on one
+// tree onto its stream.
 #pragma once
 
 #include <cmath>
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "ikd_tree_amd.h"
+
+#if defined(EIGEN_WORLD_VERSION) && !defined(IKD_AMD_STD_ALLOCATOR)
+template <class T>
+using ikd_point_allocator = Eigen::aligned_allocator<T>;
+#else
+template <class T>
+using ikd_point_allocator = std::allocator<T>;
+#endif
+
+#ifndef IKD_AMD_NO_STORAGE_ENUM
+// flatten's record modes, ikd_Tree.h:39
+enum delete_point_storage_set { NOT_RECORD, DELETE_POINTS_REC, MULTI_THREAD_REC };
+#endif
 
 #ifndef IKD_AMD_NO_BASIC_TYPES
 // ikdTree_PointType and BoxPointType, ikd_Tree.h:22-35
@@ -44,9 +69,17 @@ struct BoxPointType {
 template <typename PointType>
 class KD_TREE {
 public:
-    using PointVector = std::vector<PointType>;
+    using PointVector = std::vector<PointType, ikd_point_allocator<PointType>>;
+    using Ptr = std::shared_ptr<KD_TREE<PointType>>;  // ikd_Tree.h:63
 
-    KD_TREE(float delete_param = 0.5, float balance_param = 0.6, float box_length = 0.2) {
+    // Opaque stand-in for the reference's node type (ikd_Tree.h:64-86): the
+    // tree's nodes live in device memory, so the only node a caller can hold
+    // is Root_Node, which stands for the whole tree.
+    struct KD_TREE_NODE {
+        const KD_TREE* tree;
+    };
+
+    KD_TREE(float delete_param = 0.5, float balance_param = 0.6, float box_length = 0.2) : root_handle_{this} {
         check(ikd_create(delete_param, balance_param, box_length, &h_), "KD_TREE");
         check(ikd_set_tuning(h_, IKD_TUNE_RECORD_REMOVED, 1), "KD_TREE");
     }
@@ -87,11 +120,15 @@ public:
 
     // Build(PointVector), ikd_Tree.cpp:352-364
     void Build(PointVector point_cloud) {
+        // The removed-point record outlives Build (the reference keeps
+        // Points_deleted): turn its ids into records before the id table goes.
+        take_removed(pending_removed_);
         payload_ = std::move(point_cloud);
         std::vector<float> xyz = pack(payload_);
         std::vector<uint32_t> ids(payload_.size());
         for (size_t i = 0; i < ids.size(); ++i) ids[i] = (uint32_t)i;
         check(ikd_build(h_, xyz.data(), ids.data(), (int64_t)payload_.size()), "Build");
+        sync_root();
     }
 
     // Nearest_Search, ikd_Tree.cpp:366-397 (one query; see Nearest_Search_Batch)
@@ -171,6 +208,7 @@ public:
         int32_t added = 0;
         check(ikd_add_points(h_, xyz.data(), ids.data(), (int64_t)PointToAdd.size(), downsample_on ? 1 : 0, &added),
               "Add_Points");
+        sync_root();
         return added;
     }
 
@@ -179,25 +217,70 @@ public:
         if (PointToDel.empty()) return;
         std::vector<float> xyz = pack(PointToDel);
         check(ikd_delete_points(h_, xyz.data(), (int64_t)PointToDel.size()), "Delete_Points");
+        sync_root();
     }
 
     // Delete_Point_Boxes, ikd_Tree.cpp:535-556
     int Delete_Point_Boxes(std::vector<BoxPointType>& BoxPoints) {
         if (BoxPoints.empty()) return 0;
-        std::vector<float> b(6 * BoxPoints.size());
-        for (size_t i = 0; i < BoxPoints.size(); ++i)
-            for (int a = 0; a < 3; ++a) {
-                b[6 * i + a] = BoxPoints[i].vertex_min[a];
-                b[6 * i + 3 + a] = BoxPoints[i].vertex_max[a];
-            }
+        std::vector<float> b = pack_boxes(BoxPoints);
         int32_t n = 0;
         check(ikd_delete_point_boxes(h_, b.data(), (int64_t)BoxPoints.size(), &n), "Delete_Point_Boxes");
+        sync_root();
         return n;
     }
 
-    // flatten(Root_Node, Storage, NOT_RECORD), ikd_Tree.cpp:1325-1352
+    // Add_Point_Boxes, ikd_Tree.cpp:491-511
+    void Add_Point_Boxes(std::vector<BoxPointType>& BoxPoints) {
+        if (BoxPoints.empty()) return;
+        std::vector<float> b = pack_boxes(BoxPoints);
+        check(ikd_add_point_boxes(h_, b.data(), (int64_t)BoxPoints.size()), "Add_Point_Boxes");
+        sync_root();
+    }
+
+    // flatten(root, Storage, storage_type), ikd_Tree.cpp:1325-1352: appends the
+    // live points under root (Root_Node: the whole tree) in pre-order.
+    void flatten(KD_TREE_NODE* root, PointVector& Storage, delete_point_storage_set storage_type) {
+        if (root == nullptr) return;
+        if (root != &root_handle_) throw std::runtime_error("flatten: only this tree's Root_Node can be flattened");
+        if (storage_type != NOT_RECORD)
+            throw std::runtime_error("flatten: only the NOT_RECORD mode is supported");
+        append_live(Storage);
+    }
+
+    // flatten(Root_Node, Storage, NOT_RECORD) into a cleared vector
     void flatten(PointVector& Storage) {
         Storage.clear();
+        append_live(Storage);
+    }
+
+    // acquire_removed_points, ikd_Tree.cpp:558-571 (the facade records from
+    // construction on, as the reference does)
+    void acquire_removed_points(PointVector& removed_points) {
+        for (const PointType& p : pending_removed_) removed_points.push_back(p);
+        PointVector().swap(pending_removed_);
+        take_removed(removed_points);
+    }
+
+    ikd_tree_t handle() const { return h_; }
+
+    PointVector PCL_Storage;          // ikd_Tree.h:246 (public, unused by the library)
+    KD_TREE_NODE* Root_Node = nullptr;  // ikd_Tree.h:247 (null while the tree has no nodes)
+    int max_queue_size = 0;           // ikd_Tree.h:248
+
+private:
+    ikd_tree_t h_ = nullptr;
+    KD_TREE_NODE root_handle_;
+    PointVector payload_;          // payload id -> point record
+    PointVector pending_removed_;  // removed points recorded before the last Build
+
+    void sync_root() {
+        int64_t n = 0;
+        check(ikd_size(h_, &n), "size");
+        Root_Node = n > 0 ? &root_handle_ : nullptr;
+    }
+
+    void append_live(PointVector& Storage) {
         int64_t n = 0;
         check(ikd_flatten(h_, nullptr, nullptr, 0, &n), "flatten");
         if (n == 0) return;
@@ -207,9 +290,9 @@ public:
         for (int64_t i = 0; i < n; ++i) Storage.push_back(payload_[ids[i]]);
     }
 
-    // acquire_removed_points, ikd_Tree.cpp:558-571 (the facade records from
-    // construction on, as the reference does)
-    void acquire_removed_points(PointVector& removed_points) {
+    // moves the library's removed-point record (ids) into out as records
+    template <class Vec>
+    void take_removed(Vec& out) {
         int64_t n = 0;
         int rc = ikd_acquire_removed_points(h_, nullptr, nullptr, 0, &n);
         if (rc != IKD_OK && rc != IKD_ERR_CAPACITY) check(rc, "acquire_removed_points");
@@ -217,14 +300,18 @@ public:
         std::vector<float> xyz(3 * n);
         std::vector<uint32_t> ids(n);
         check(ikd_acquire_removed_points(h_, xyz.data(), ids.data(), n, &n), "acquire_removed_points");
-        for (int64_t i = 0; i < n; ++i) removed_points.push_back(payload_[ids[i]]);
+        for (int64_t i = 0; i < n; ++i) out.push_back(payload_[ids[i]]);
     }
 
-    ikd_tree_t handle() const { return h_; }
-
-private:
-    ikd_tree_t h_ = nullptr;
-    PointVector payload_;  // payload id -> point record
+    static std::vector<float> pack_boxes(const std::vector<BoxPointType>& boxes) {
+        std::vector<float> b(6 * boxes.size());
+        for (size_t i = 0; i < boxes.size(); ++i)
+            for (int a = 0; a < 3; ++a) {
+                b[6 * i + a] = boxes[i].vertex_min[a];
+                b[6 * i + 3 + a] = boxes[i].vertex_max[a];
+            }
+        return b;
+    }
 
     void check(int rc, const char* what = "ikd") {
         if (rc != IKD_OK) throw std::runtime_error(std::string(what) + ": " + ikd_last_error(h_));

@@ -120,6 +120,15 @@ int ikd_delete_points(ikd_tree_t t, const float* xyz, int64_t n);
  * the number of live points removed. */
 int ikd_delete_point_boxes(ikd_tree_t t, const float* boxes, int64_t nb, int32_t* out_deleted);
 
+/* Add_Point_Boxes(BoxPoints), ikd_Tree.cpp:491-511 (Add_by_range, cpp:762-815):
+ * gives back the deleted points the boxes reach that the downsample did not
+ * delete, box by box.  As in the reference, the walk decides by the nodes'
+ * stored ranges, which leave deleted subtrees out: a box that covers a node's
+ * range restores that whole subtree (points outside the box included), and
+ * deleted points inside a box stay deleted where an ancestor's range misses
+ * them. */
+int ikd_add_point_boxes(ikd_tree_t t, const float* boxes, int64_t nb);
+
 /* size(), ikd_Tree.cpp:78-96: nodes in the tree including lazily deleted ones */
 int ikd_size(ikd_tree_t t, int64_t* out);
 
@@ -129,7 +138,9 @@ int ikd_validnum(ikd_tree_t t, int64_t* out);
 /* tree_range(), ikd_Tree.cpp:98-126: root AABB (min xyz, max xyz); zeros when empty */
 int ikd_tree_range(ikd_tree_t t, float* out6);
 
-/* root_alpha(alpha_bal, alpha_del), ikd_Tree.cpp:147-165 */
+/* root_alpha(alpha_bal, alpha_del), ikd_Tree.cpp:147-165: the values the root's
+ * last Update stored (cpp:1315-1321); 0 after Build, whose BuildTree runs
+ * before the root is the tree's root (the reference leaves them unset). */
 int ikd_root_alpha(ikd_tree_t t, float* alpha_bal, float* alpha_del);
 
 /* flatten(Root_Node, Storage, NOT_RECORD), ikd_Tree.cpp:1325-1352: all live

@@ -62,6 +62,7 @@ def _load():
     lib.ora_delete_points.argtypes = [vp, vp, i64]
     lib.ora_delete_point_boxes.restype = i32
     lib.ora_delete_point_boxes.argtypes = [vp, vp, i64]
+    lib.ora_add_point_boxes.argtypes = [vp, vp, i64]
     lib.ora_size.restype = i64
     lib.ora_size.argtypes = [vp]
     lib.ora_validnum.restype = i64
@@ -161,6 +162,10 @@ class OracleTree:
     def Delete_Point_Boxes(self, boxes) -> int:
         b = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
         return int(self._lib.ora_delete_point_boxes(self._h, _ptr(b), b.shape[0]))
+
+    def Add_Point_Boxes(self, boxes) -> None:
+        b = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
+        self._lib.ora_add_point_boxes(self._h, _ptr(b), b.shape[0])
 
     def size(self) -> int:
         return int(self._lib.ora_size(self._h))

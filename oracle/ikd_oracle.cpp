@@ -13,7 +13,8 @@
 //   - the bounded heap and the nearest-first recursive kNN Search,
 //   - Search_by_range / Search_by_radius / flatten,
 //   - Add_Points with voxel downsample, Add_by_point, Delete_by_point,
-//     Delete_by_range, Criterion_Check and Rebuild.
+//     Delete_by_range, Add_by_range (Add_Point_Boxes), Criterion_Check and
+//     Rebuild.
 // Deliberate differences (none of them changes a live set or a query result):
 //   - every Rebuild runs synchronously, also for subtrees of >= 1500 nodes
 //     that the reference hands to its background thread (cpp:627-633); the
@@ -167,6 +168,9 @@ public:
     float downsample_size = 0.2f;
     Node* root = nullptr;
     Node sentinel;  // STATIC_ROOT_NODE, cpp:358-363
+    // every pending label has been pushed down since the last change (a
+    // multi-threaded kNN may then search without writing the tree)
+    bool labels_pushed = false;
 
     ~Tree() { destroy(root); }
 
@@ -470,6 +474,33 @@ public:
         finish(slot, allow_rebuild);
     }
 
+    // Add_by_range, cpp:762-815: enters deleted subtrees too (no tree_deleted
+    // check).  A subtree whose range the box covers gets back every point the
+    // downsample did not delete, through the labels Push_Down hands on
+    // (invalid := down_del, deletion := downsample deletion); a partly
+    // covered node gets back its own point if it lies in the box.  The
+    // decisions use the nodes' stored ranges, which leave out deleted
+    // subtrees, so a covered node may bring back points outside the box and
+    // points inside it stay deleted where an ancestor's range misses them.
+    void add_by_range(Node** slot, const Box& b, bool allow_rebuild) {
+        Node* n = *slot;
+        if (n == nullptr) return;
+        push_down(n);
+        if (disjoint(b, n)) return;
+        if (covers(b, n)) {
+            n->tree_deleted = n->tree_ds_deleted;
+            n->point_deleted = n->point_ds_deleted;
+            n->push_left = true;
+            n->push_right = true;
+            n->invalid = n->down_del;
+            return;
+        }
+        if (inside(b, n->point)) n->point_deleted = n->point_ds_deleted;
+        add_by_range(&n->left, b, allow_rebuild);
+        add_by_range(&n->right, b, allow_rebuild);
+        finish(slot, allow_rebuild);
+    }
+
     // Delete_by_range, cpp:647-710.
     int delete_by_range(Node** slot, const Box& b, bool allow_rebuild, bool is_ds) {
         Node* n = *slot;
@@ -552,6 +583,12 @@ public:
         return cnt;
     }
 
+    // Add_Point_Boxes, cpp:491-511.
+    void add_boxes(const std::vector<Box>& boxes) {
+        for (const Box& b : boxes) add_by_range(&root, b, true);
+        if (root) { root->father = &sentinel; sentinel.left = root; }
+    }
+
     // size / validnum / tree_range, cpp:78-145 (no rebuild in flight here).
     int size() const { return root ? root->tree_size : 0; }
     int validnum() const { return root ? root->tree_size - root->invalid : 0; }
@@ -615,6 +652,7 @@ void ora_set_params(void* h, float delete_param, float balance_param, float box_
 }
 
 void ora_build(void* h, const float* xyz, const uint32_t* ids, int64_t n) {
+    ((Tree*)h)->labels_pushed = false;
     ((Tree*)h)->build(make_points(xyz, ids, n));
 }
 
@@ -626,9 +664,10 @@ int ora_knn(void* h, const float* q, int64_t nq, int k, double max_dist, float* 
             uint32_t* out_ids, int32_t* out_counts, int64_t* visits, int64_t* child_boxes, int threads) {
     Tree* t = (Tree*)h;
     if (k <= 0) return -1;
-    if (threads > 1) {
+    if (threads > 1 && !t->labels_pushed) {
         std::vector<Pt> tmp;
-        t->flatten(t->root, tmp);  // pushes every pending label down once
+        t->flatten(t->root, tmp);  // pushes every pending label down once per change
+        t->labels_pushed = true;
     }
 #pragma omp parallel for schedule(dynamic, 256) num_threads(threads > 1 ? threads : 1)
     for (int64_t i = 0; i < nq; ++i) {
@@ -685,18 +724,30 @@ int64_t ora_radius_search(void* h, const float* q3, float radius, float* out_xyz
 }
 
 int ora_add_points(void* h, const float* xyz, const uint32_t* ids, int64_t n, int downsample_on) {
+    ((Tree*)h)->labels_pushed = false;
     return ((Tree*)h)->add_points(make_points(xyz, ids, n), downsample_on != 0);
 }
 
 void ora_delete_points(void* h, const float* xyz, int64_t n) {
+    ((Tree*)h)->labels_pushed = false;
     ((Tree*)h)->delete_points(make_points(xyz, nullptr, n));
 }
 
 int ora_delete_point_boxes(void* h, const float* boxes6, int64_t nb) {
+    ((Tree*)h)->labels_pushed = false;
     std::vector<Box> v((size_t)nb);
     for (int64_t i = 0; i < nb; ++i)
         for (int a = 0; a < 3; ++a) { v[i].vmin[a] = boxes6[6 * i + a]; v[i].vmax[a] = boxes6[6 * i + 3 + a]; }
     return ((Tree*)h)->delete_boxes(v);
+}
+
+// Add_Point_Boxes, cpp:491-511.
+void ora_add_point_boxes(void* h, const float* boxes6, int64_t nb) {
+    ((Tree*)h)->labels_pushed = false;
+    std::vector<Box> v((size_t)nb);
+    for (int64_t i = 0; i < nb; ++i)
+        for (int a = 0; a < 3; ++a) { v[i].vmin[a] = boxes6[6 * i + a]; v[i].vmax[a] = boxes6[6 * i + 3 + a]; }
+    ((Tree*)h)->add_boxes(v);
 }
 
 int64_t ora_size(void* h) { return ((Tree*)h)->size(); }

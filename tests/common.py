@@ -156,3 +156,40 @@ def knn_equivalent(d_a, id_a, c_a, d_b, id_b, c_b, strict_ties: bool = False):
         if ia != ib:
             bad.append(int(i))
     return bad
+
+
+def check_dump_invariants(d: np.ndarray, tight: bool = True) -> None:
+    """Structural invariants of a pre-order node dump (oracle or device layout):
+    TreeSize and invalid_point_num are the sums over the subtree (Update,
+    ikd-Tree/ikd_Tree.cpp:1183-1200), tree_deleted holds exactly when no live
+    point is left below, and the stored AABB bounds the live points (tight:
+    equals their bounding box; otherwise contains it, since Delete_by_point
+    leaves the deleted node's range as it was, cpp:717-721)."""
+    import sys
+    sys.setrecursionlimit(max(10000, sys.getrecursionlimit()))
+
+    def rec(i):
+        r = d[i]
+        size, inv = 1, int(r["flags"] & 1)
+        pts_live = [] if (r["flags"] & 1) else [(r["x"], r["y"], r["z"])]
+        for c in (int(r["left"]), int(r["right"])):
+            if c >= 0:
+                s, iv, pl = rec(c)
+                size += s
+                inv += iv
+                pts_live += pl
+        assert size == r["tree_size"], f"row {i}: TreeSize"
+        assert inv == r["invalid"], f"row {i}: invalid_point_num"
+        td = bool(r["flags"] & 2)
+        assert td == (len(pts_live) == 0), f"row {i}: tree_deleted <=> no live point"
+        if pts_live and tight is not None:
+            a = np.array(pts_live, np.float32)
+            bb = np.concatenate([a.min(axis=0), a.max(axis=0)])
+            if tight:
+                assert np.array_equal(bb, r["range"]), f"row {i}: AABB is the tight live bbox"
+            else:
+                assert np.all(r["range"][:3] <= bb[:3]) and np.all(r["range"][3:] >= bb[3:]), f"row {i}: AABB"
+        return size, inv, pts_live
+
+    if d.shape[0]:
+        rec(0)

@@ -246,6 +246,7 @@ hipError_t hipMemPoolDestroy(hipMemPool_t mp) {
 hipError_t hipMemPoolSetAttribute(hipMemPool_t, hipMemPoolAttr, void*) { return hipSuccess; }
 hipError_t hipMallocFromPoolAsync(void** p, size_t n, hipMemPool_t, hipStream_t) { return hipMalloc(p, n); }
 hipError_t hipFreeAsync(void* p, hipStream_t) { return hipFree(p); }
+hipError_t hipMallocAsync(void** p, size_t n, hipStream_t) { return hipMalloc(p, n); }
 hipError_t hipGetDevice(int* dev) {
     *dev = 0;
     return hipSuccess;
@@ -282,6 +283,7 @@ hipError_t hipEventDestroy(hipEvent_t e) {
     return hipSuccess;
 }
 hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
 hipError_t hipGetLastError() { return hipSuccess; }
 hipError_t hipDeviceSynchronize() { return hipSuccess; }
 const char* hipGetErrorString(hipError_t) { return "emulated HIP error"; }

@@ -139,6 +139,7 @@ hipError_t hipMemPoolDestroy(hipMemPool_t mp);
 hipError_t hipMemPoolSetAttribute(hipMemPool_t mp, hipMemPoolAttr attr, void* value);
 hipError_t hipMallocFromPoolAsync(void** p, size_t n, hipMemPool_t mp, hipStream_t st);
 hipError_t hipFreeAsync(void* p, hipStream_t st);
+hipError_t hipMallocAsync(void** p, size_t n, hipStream_t st);
 hipError_t hipGetDevice(int* dev);
 hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind k);
 hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t st);
@@ -150,6 +151,7 @@ hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned flags);
 hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned flags);
 hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s);
+hipError_t hipEventSynchronize(hipEvent_t e);
 hipError_t hipGetLastError();
 hipError_t hipDeviceSynchronize();
 const char* hipGetErrorString(hipError_t e);

@@ -31,3 +31,48 @@ def test_server_requests_may_be_released_on_return(emu_lib, tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300, env=env)
     assert r.returncode == 0, r.stderr[-3000:]
     assert r.stdout.startswith("ok")
+
+
+def _build_emu_exe(src_name, exe):
+    subprocess.run(["g++", "-std=c++20", "-O1", "-g", "-fsanitize=address", "-I", os.path.join(HERE, "..", "include"),
+                    os.path.join(HERE, "cpp", src_name), "-o", str(exe), "-L", EMU, "-likd_emu",
+                    f"-Wl,-rpath,{EMU}", "-lpthread"], check=True)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    env.pop("LD_PRELOAD", None)
+    return env
+
+
+def test_concurrent_nearest_search_matches_single_thread(emu_lib, tmp_path):
+    """8 host threads calling Nearest_Search on one tree at once (FAST-LIO's
+    OpenMP loop) get the single-threaded results, with no memory errors."""
+    exe = tmp_path / "concurrent_search"
+    env = _build_emu_exe("concurrent_search.cpp", exe)
+    r = subprocess.run([str(exe), "3000", "120", "8"], capture_output=True, text=True, timeout=600, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-3000:]
+    assert r.stdout.startswith("ok")
+
+
+def test_reference_demo_call_sequences(emu_lib, tmp_path):
+    """The three reference demos' call sequences, typed against the facade,
+    compile and give the oracle's results (emulation build, ASan)."""
+    from demo_harness import check_outputs, write_inputs
+    exe = tmp_path / "demo_calls"
+    env = _build_emu_exe("demo_calls.cpp", exe)
+    ind, outd = tmp_path / "in", tmp_path / "out"
+    ind.mkdir()
+    outd.mkdir()
+    inputs = write_inputs(ind, n=3000)
+    r = subprocess.run([str(exe), str(ind), str(outd)], capture_output=True, text=True, timeout=900, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-3000:]
+    check_outputs(inputs, outd, r.stdout)
+
+
+def test_acquire_removed_points_survives_a_second_build(emu_lib, tmp_path):
+    """Points a rebuild removed before a second Build come back from
+    acquire_removed_points as the records they were (the id table of the
+    first cloud is gone by then), with no out-of-bounds read."""
+    exe = tmp_path / "removed_after_build"
+    env = _build_emu_exe("removed_after_build.cpp", exe)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=600, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-3000:]
+    assert r.stdout.startswith("ok")

@@ -8,7 +8,7 @@ median index) fails it.
 import numpy as np
 import pytest
 
-from common import (INF, assert_same_multiset, brute_knn, brute_radius, d2_all, grid_boxes,
+from common import (INF, assert_same_multiset, brute_knn, brute_radius, check_dump_invariants, d2_all, grid_boxes,
                     grid_cloud, in_box, knn_equivalent, random_boxes, uniform_cloud)
 from oracle import OracleTree
 
@@ -268,35 +268,7 @@ def test_build_structure_matches_median_definition(n):
 # --------------------------------------------------------------- invariants
 def _check_invariants(t: OracleTree, tight: bool = True):
     d = t.dump()
-    n = d.shape[0]
-    live_pts = {}
-
-    def rec(i):
-        r = d[i]
-        size, inv = 1, int(r["flags"] & 1)
-        pts_live = [] if (r["flags"] & 1) else [(r["x"], r["y"], r["z"])]
-        for c in (int(r["left"]), int(r["right"])):
-            if c >= 0:
-                s, iv, pl = rec(c)
-                size += s
-                inv += iv
-                pts_live += pl
-        assert size == r["tree_size"], "TreeSize"
-        assert inv == r["invalid"], "invalid_point_num"
-        td = bool(r["flags"] & 2)
-        assert td == (len(pts_live) == 0), "tree_deleted <=> no live point"
-        if pts_live and tight is not None:
-            a = np.array(pts_live, np.float32)
-            bb = np.concatenate([a.min(axis=0), a.max(axis=0)])
-            if tight:
-                assert np.array_equal(bb, r["range"]), "AABB is the tight live bbox"
-            else:
-                assert np.all(r["range"][:3] <= bb[:3]) and np.all(r["range"][3:] >= bb[3:])
-        live_pts[i] = pts_live
-        return size, inv, pts_live
-
-    if n:
-        rec(0)
+    check_dump_invariants(d, tight)
     return d
 
 
@@ -366,3 +338,34 @@ def test_acquire_removed_points_records_rebuild_drops():
         assert recorded <= set(range(8000)) | set(range(10000, 10200))
     ds_size = o.size() - o.validnum()  # deleted nodes still in the tree, downsample ones included
     assert len(recorded) <= deleted <= len(recorded) + ds_size
+
+
+# ------------------------------------------------------------ near ties
+NEAR_TIE = np.array([[3e-6, 0, 0], [-7e-6, 0, 0], [0, 8e-6, 0]], np.float32)  # P1, P2, P3
+
+
+def test_knn_near_tie_comparator():
+    """PointType_CMP (ikd_Tree.h:99-102) orders distances closer than 1e-10 by
+    x.  Around the origin d2(P1) = 9e-12 < d2(P2) = 4.9e-11 < d2(P3) = 6.4e-11,
+    all within 1e-10, while x orders them P2 < P3 < P1.  BuildTree puts P3
+    (median x) at the root, P2 left and P1 right.  Search (cpp:868-1013) for
+    k = 2 takes P3, then the nearer child P1; the heap's top is then P1 (the
+    largest x, not the largest distance), so the left child's box distance
+    4.9e-11 is not below top.dist = 9e-12 and P2 is never reached.  The
+    result is {P3, P1}, popped top-first into the front: P3 then P1."""
+    t = OracleTree()
+    t.Build(NEAR_TIE)
+    d = t.dump()
+    assert [int(r["id"]) for r in d] == [2, 1, 0]  # pre-order: P3, P2, P1
+    xyz, d2, ids = t.Nearest_Search([0, 0, 0], 2)
+    assert ids.tolist() == [2, 0]
+    q = np.zeros(3, np.float32)
+    assert d2.view(np.uint32).tolist() == [d2_all(NEAR_TIE[2:3], q)[0].view(np.uint32),
+                                           d2_all(NEAR_TIE[0:1], q)[0].view(np.uint32)]
+    # the exact 2 nearest are P1 and P2; P2 and P3 lie within the comparator's
+    # 1e-10 window of each other, which knn_equivalent treats as a tie
+    bd, bi, bc = brute_knn(NEAR_TIE, np.arange(3, dtype=np.uint32), q[None], 2)
+    assert bi[0].tolist() == [0, 1]
+    _, od, oi, oc = t.knn(q[None], 2)
+    assert knn_equivalent(od, oi, oc, bd, bi, bc, strict_ties=True) == [0]
+    assert knn_equivalent(od, oi, oc, bd, bi, bc) == []
