@@ -71,6 +71,8 @@ def main():
     ap.add_argument("--max-dist", type=float, default=5.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-scans", type=int, default=2, help="scans of queries in the CPU sample")
+    ap.add_argument("--count-scans", type=int, default=16,
+                    help="scans whose reference-order traversal counts give the algorithmic bytes")
     ap.add_argument("--batch-scans", type=int, default=32, help="scans per launch for the batched form")
     ap.add_argument("--frames", type=int, default=200, help="scans in the mapping-pipeline phase")
     ap.add_argument("--trees", type=int, default=64, help="trees in the multi-tree batching phase (0 skips it)")
@@ -284,6 +286,8 @@ def main():
                          f"32x1800-ray scans; map = raw scans of every {SEED_STRIDE}th pose, queries = scans of the "
                          "poses in between); the plain form (one launch per scan, same scans, identical results) "
                          "is reported under 'plain'"),
+            "distinct_query_scans": len(set(qframes[j] for j in timed_scans)),
+            "timed_query_scans": len(timed_scans),
             "map_points": int(map_points),
             "scans_per_step": B,
             "queries_per_step": nq_total / args.steps,
@@ -319,8 +323,39 @@ def main():
 
     progress("phase D done")
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(seed_pts, [scans[j] for j in timed_scans][: args.cpu_scans], k,
-                                              args.max_dist)
+        cpu_idx = timed_scans[: args.cpu_scans]
+        gpu_out = tuple(np.concatenate([po[j][m].cpu().numpy() for j in cpu_idx]) for m in (1, 2, 3))
+        gpu_out = (gpu_out[0], gpu_out[1].view(np.uint32), gpu_out[2])
+        # reference-order V and E on a spread of the timed scans (every
+        # count_stride-th distinct scan), scaled to the timed set per query
+        distinct = sorted(set(qframes[j] for j in timed_scans))
+        stride = max(1, len(distinct) // args.count_scans)
+        count_frames = distinct[::stride][: args.count_scans]
+        count_scans = [by_frame[g] for g in count_frames]
+        cb = cpu_baseline(seed_pts, [scans[j] for j in cpu_idx], gpu_out, count_scans, k, args.max_dist)
+        result["cpu_baseline"] = cb
+        rw = cb["reference_walk"]
+        v_q = rw["visits"] / rw["queries"]
+        c_q = (rw["child_boxes"] - rw["visits"] + rw["queries"]) / rw["queries"]
+        bpq = 12 + 16 * found / nq_total + 48 * v_q + 24 * c_q
+        roof_r = result["roofline"]
+        roof_r["device_walk"] = {k_: roof_r.pop(k_) for k_ in ("achieved", "frac", "algorithmic_bytes_per_launch",
+                                                                  "bytes_per_query", "visits_per_query",
+                                                                  "unvisited_boxes_per_query")}
+        per_launch = bpq * nq_total / len(timed)
+        roof_r.update({
+            "achieved": per_launch / b_avg / 1e9,
+            "frac": per_launch / b_avg / 1e9 / HBM_PEAK_GBS,
+            "algorithmic_bytes_per_launch": per_launch,
+            "bytes_per_query": bpq,
+            "visits_per_query": v_q,
+            "unvisited_boxes_per_query": c_q,
+            "counts": (f"V and C from the oracle's reference-order walk on {rw['queries']} queries "
+                       f"({rw['scans']} of the {len(distinct)} distinct timed scans, evenly spaced); "
+                       "k_found from the device results of every timed query"),
+        })
+        result["value_check"] = {"identical_batched_plain": bool(identical),
+                                 "gpu_vs_oracle_mismatches": cb["mismatches"]}
     print(json.dumps(result))
 
 
@@ -583,13 +618,14 @@ def measured_traffic():
 
 
 def knn_kernel_name(k, lanes):
+    """The kernel symbol a launch shape runs (as rocprofv3 names it)."""
     if lanes == 1:
-        return f"k_knn<{knn_bucket(k)},32>"
+        return f"ikd::k_knn<{knn_bucket(k)}>"
     if lanes == 0:
-        return f"k_knn_persist<{knn_bucket(k)},32>"
+        return f"ikd::k_knn_persist<{knn_bucket(k)}>"
     if lanes < 0:
-        return f"k_knn*<{knn_bucket(k)}> (lanes per query from the batch size)"
-    return f"k_knn_team<{knn_bucket(k)},{lanes},32>"
+        return f"ikd::k_knn*<{knn_bucket(k)}> (lanes per query from the batch size)"
+    return f"ikd::k_knn_team<{knn_bucket(k)}, {lanes}>"
 
 
 def knn_bucket(k):
@@ -644,15 +680,38 @@ def pipeline_latency(ikd, torch, dev, scene, frames, k, max_dist):
     }
 
 
-def cpu_baseline(seed_pts, cpu_scans, k, max_dist, min_seconds=10.0):
+def cpu_baseline(seed_pts, cpu_scans, gpu_out, count_scans, k, max_dist, min_seconds=10.0):
     """The oracle (CPU restatement of the reference) on the host cores: Build of
     the same seed map (not timed), then kNN of a bounded sample of scans,
-    repeated until min_seconds of single-thread work."""
+    repeated until min_seconds of single-thread work.  The same leg checks the
+    GPU's results for that sample against the oracle's (squared distances and
+    ids; entries within the reference comparator's 1e-10 window of the k-th
+    distance may be exchanged, ikd_Tree.h:101) and counts, with the oracle's
+    reference-order walk (Search, ikd_Tree.cpp:868-1013), the nodes visited (V)
+    and child boxes evaluated (E) for count_scans, which the roofline's
+    algorithmic bytes use."""
     from oracle import OracleTree
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from common import knn_equivalent
 
     ora = OracleTree(0.5, 0.6, 0.5)
     ora.Build(seed_pts)
     q = np.concatenate(cpu_scans)
+    ncpu = min(os.cpu_count() or 1, 16)  # one GPU's share of the host
+    # parity of the timed GPU results on the sample
+    _, od, oi, oc = ora.knn(q, k, max_dist, threads=ncpu)
+    gd, gi, gc = gpu_out
+    loose = knn_equivalent(gd, gi, gc, od, oi, oc)
+    strict = knn_equivalent(gd, gi, gc, od, oi, oc, strict_ties=True)
+    # reference-order traversal counts
+    vis = box = nq = found = 0
+    for sc in count_scans:
+        _, _, _, c_, v_, b_ = ora.knn(sc, k, max_dist, counters=True, threads=ncpu)
+        vis += int(v_.sum())
+        box += int(b_.sum())
+        nq += sc.shape[0]
+        found += int(c_.sum())
 
     def timed(threads, seconds):
         n, t0 = 0, time.perf_counter()
@@ -664,7 +723,6 @@ def cpu_baseline(seed_pts, cpu_scans, k, max_dist, min_seconds=10.0):
                 return n / dt, n
 
     v1, n1 = timed(1, min_seconds)
-    ncpu = min(os.cpu_count() or 1, 16)  # one GPU's share of the host
     vn, _ = timed(ncpu, min_seconds / 2)
     return {
         "value": v1,
@@ -676,6 +734,11 @@ def cpu_baseline(seed_pts, cpu_scans, k, max_dist, min_seconds=10.0):
                   f"on 1 host thread",
         "all_cores_value": vn,
         "all_cores": ncpu,
+        "parity_queries": int(q.shape[0]),
+        "mismatches": len(loose),
+        "near_tie_differences": len(strict) - len(loose),
+        "reference_walk": {"queries": nq, "scans": len(count_scans), "visits": vis, "child_boxes": box,
+                           "found": found},
     }
 
 

@@ -76,3 +76,14 @@ def test_acquire_removed_points_survives_a_second_build(emu_lib, tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=600, env=env)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-3000:]
     assert r.stdout.startswith("ok")
+
+
+def test_server_stress_with_deferred_rebuilds(emu_lib, tmp_path):
+    """8 workers over 8 maps with deferred rebuilds and idle-map maintenance,
+    requests in flight at once and maps added meanwhile: every request
+    succeeds with no memory error (emulation build, ASan)."""
+    exe = tmp_path / "server_stress"
+    env = _build_emu_exe("server_stress.cpp", exe)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=900, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-3000:]
+    assert r.stdout.startswith("ok")

@@ -50,3 +50,38 @@ def test_cpp_facade_matches_oracle(tmp_path):
     assert int(line[1]) == n_del and int(line[3]) == n_add and int(line[5]) == o.validnum()
     flat = np.fromfile(tmp_path / "o_flat.f32", np.float32).reshape(-1, 4)
     assert_same_multiset(flat[:, :3], o.flatten()[0], "facade flatten after updates")
+
+
+def _build_exe(src, exe):
+    libdir = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "ikd-tree_amd")
+    inc = os.path.join(os.path.dirname(libdir), "include")
+    return subprocess.run(["g++", "-std=c++17", "-O2", "-I", inc, os.path.join(os.path.dirname(__file__), "cpp", src),
+                           "-L", libdir, "-likd_hip", f"-Wl,-rpath,{libdir}", "-lpthread", "-o", str(exe)],
+                          capture_output=True, text=True)
+
+
+def test_concurrent_nearest_search_on_one_tree(tmp_path):
+    """8 host threads call Nearest_Search on one tree at once (FAST-LIO's
+    OpenMP loop); every result equals the single-threaded one."""
+    exe = tmp_path / "concurrent_search"
+    r = _build_exe("concurrent_search.cpp", exe)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe), "200000", "4000", "8"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.startswith("ok")
+
+
+def test_reference_demo_call_sequences(tmp_path):
+    """The three reference demos' call sequences, typed against the facade,
+    give the oracle's results on the GPU."""
+    from demo_harness import check_outputs, write_inputs
+    exe = tmp_path / "demo_calls"
+    r = _build_exe("demo_calls.cpp", exe)
+    assert r.returncode == 0, r.stderr
+    ind, outd = tmp_path / "in", tmp_path / "out"
+    ind.mkdir()
+    outd.mkdir()
+    inputs = write_inputs(ind, n=20000)
+    out = subprocess.run([str(exe), str(ind), str(outd)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    check_outputs(inputs, outd, out.stdout)
