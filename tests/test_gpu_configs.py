@@ -9,7 +9,7 @@
 - C3, box/radius and trims on a 5M-point map (raw scans of the scene, seed
   1003): 1000 Box_Search (sides log-uniform 1-20 m) and 1000 Radius_Search
   (r 0.2-3 m) centred on live points, all checked against brute force over the
-  live set; then 30 frames of kNN + Add_Points with a trim (Delete_Point_Boxes
+  live set; then 20 frames of kNN + Add_Points with a trim (Delete_Point_Boxes
   of the six slabs outside a 200 m cube around the sensor) every 10 frames,
   driving rebuilds: trim counts and validnum against the oracle, the final
   live set against the oracle's.
@@ -160,11 +160,11 @@ def test_c3_box_radius_and_trims_on_5m_map(ikd, scenes):
         inside, band = brute_radius(pts[cand], cen[i], float(r[i]))
         got = set(gi[off[i]:off[i + 1]].tolist())
         assert set(cand[inside & ~band].tolist()) <= got <= set(cand[inside | band].tolist()), f"radius {i}"
-    # 30 frames of mapping with a 200 m-cube trim every 10 frames, beside the oracle
+    # 20 frames of mapping with a 200 m-cube trim every 10 frames, beside the oracle
     o = OracleTree(0.5, 0.6, 0.5)
     o.Build(pts, ids)
     nid = n
-    for fr in range(f, f + 30):
+    for fr in range(f, f + 20):
         s = sc.scan(fr)
         if (fr - f) % 10 == 0:
             _knn_check(g, o, s[::4], 5, 5.0, f"frame {fr}")
