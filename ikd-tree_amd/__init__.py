@@ -46,7 +46,7 @@ EXPORTS = (
     "ikd_batch_stream", "ikd_batch_set_tuning", "ikd_batch_last_error", "ikd_batch_radius_search_device",
     "ikd_now_ns", "ikd_server_create", "ikd_server_add_map", "ikd_server_submit", "ikd_server_wait",
     "ikd_server_pending", "ikd_server_destroy", "ikd_server_set_maintenance", "ikd_server_stats",
-    "ikd_rebuild_deferred",
+    "ikd_rebuild_deferred", "ikd_list_payload_ids", "ikd_relabel_payload_ids",
 )
 IKD_TUNE_QUERY_ORDER = 1
 IKD_TUNE_KNN_STACK = 2
@@ -122,6 +122,8 @@ def load_library(path: str | None = None):
         "ikd_set_tuning": [vp, i32, i32],
         "ikd_dump_nodes": [vp, vp, i64, vp],
         "ikd_rebuild_deferred": [vp, i64, vp],
+        "ikd_list_payload_ids": [vp, vp, i64, vp],
+        "ikd_relabel_payload_ids": [vp, vp, i64],
         "ikd_batch_create": [ctypes.POINTER(vp)],
         "ikd_batch_destroy": [vp],
         "ikd_batch_nearest_search_device": [vp, vp, i32, vp, vp, i64, vp, i32, vp, f64, vp, vp, vp, vp],
@@ -375,6 +377,19 @@ class KD_TREE:
         n = ctypes.c_int32(0)
         self._check(self._lib.ikd_rebuild_deferred(self._h, int(max_points), ctypes.byref(n)))
         return int(n.value)
+
+    def list_payload_ids(self):
+        """Ids the tree can still return (every node and the removed-point record), ascending."""
+        n = ctypes.c_int64(0)
+        self._check(self._lib.ikd_list_payload_ids(self._h, None, 0, ctypes.byref(n)))
+        out = np.zeros(max(n.value, 1), np.uint32)
+        self._check(self._lib.ikd_list_payload_ids(self._h, _ptr(out), n.value, ctypes.byref(n)))
+        return out[:n.value]
+
+    def relabel_payload_ids(self, old_ids):
+        """Id old_ids[i] becomes i (old_ids strictly ascending)."""
+        o = np.ascontiguousarray(old_ids, np.uint32)
+        self._check(self._lib.ikd_relabel_payload_ids(self._h, _ptr(o), o.shape[0]))
 
     def dump_nodes(self):
         """Pre-order rows (oracle DUMP_DTYPE layout) of the device tree."""

@@ -103,6 +103,26 @@ __global__ void k_node_info(Pool pool, const int* __restrict__ top, int n, int4*
 }
 
 
+// id := its index in the ascending list old (ids not in it are left alone:
+// nodes a rebuild left behind in the pool)
+__device__ __forceinline__ uint32_t relabel(uint32_t id, const uint32_t* old, int n) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (old[mid] < id) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < n && old[lo] == id) ? (uint32_t)lo : id;
+}
+__global__ void k_relabel_ids(uint32_t* __restrict__ ids, int used, const uint32_t* __restrict__ old, int n) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= ROOT && u < used) ids[u] = relabel(ids[u], old, n);
+}
+__global__ void k_relabel_records(float4* __restrict__ rec, int m, const uint32_t* __restrict__ old, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) rec[i].w = bitsf(relabel(fbits(rec[i].w), old, n));
+}
+
 __global__ void k_unpack_points(const float4* __restrict__ in, int n, float* __restrict__ xyz,
                                 uint32_t* __restrict__ ids) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1703,6 +1723,76 @@ int ikd_flatten(ikd_tree_t t, float* out_xyz, uint32_t* out_ids, int64_t capacit
     hipLaunchKernelGGL(k_unpack_points, dim3(nblk(live)), dim3(BS), 0, t->st, pts, (int)live, dx, di);
     HIPCHK(hipMemcpyAsync(out_xyz, dx, 3 * live * sizeof(float), hipMemcpyDeviceToHost, t->st));
     if (out_ids) HIPCHK(hipMemcpyAsync(out_ids, di, live * sizeof(uint32_t), hipMemcpyDeviceToHost, t->st));
+    return sync_hdr(t);
+}
+
+// Payload ids still reachable: every node of the tree (live or deleted: a
+// deleted point can come back through Add_Point_Boxes or leave through a
+// rebuild's removed-point record) and every point in that record.
+int ikd_list_payload_ids(ikd_tree_t t, uint32_t* out_ids, int64_t capacity, int64_t* out_n) {
+    CHECK_T();
+    if (!out_n || capacity < 0 || (capacity > 0 && !out_ids)) return IKD_ERR_ARG;
+    CHK(sync_hdr(t));
+    const long long used = t->pool_used;
+    std::vector<NodeHot> hot(used);
+    std::vector<uint32_t> ids(used);
+    HIPCHK(hipMemcpy(hot.data(), t->pool.hot, used * sizeof(NodeHot), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ids.data(), t->pool.ids, used * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> out;
+    if (fbits(hot[SENTINEL].f[3]) & LK_HASL) {
+        std::vector<int> stack = {ROOT};
+        while (!stack.empty()) {
+            const int u = stack.back();
+            stack.pop_back();
+            if (u < 0 || u >= used || (long long)out.size() > used) {
+                t->err = "list_payload_ids: node links out of range";
+                return IKD_ERR_INTERNAL;
+            }
+            out.push_back(ids[u]);
+            const uint32_t lk = fbits(hot[u].f[3]);
+            const int pair = (int)lk_pair(lk);
+            for (int s = 0; s < 2; ++s)
+                if (lk & lk_has(s)) stack.push_back(2 * pair + s);
+        }
+    }
+    const long long nrem = t->h_hdr->removed_count;
+    if (nrem > 0) {
+        std::vector<float4> rec(nrem);
+        HIPCHK(hipMemcpy(rec.data(), t->removed, nrem * sizeof(float4), hipMemcpyDeviceToHost));
+        for (const float4& r : rec) out.push_back(fbits(r.w));
+    }
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+    *out_n = (int64_t)out.size();
+    if (!out_ids) return IKD_OK;
+    if (capacity < (int64_t)out.size()) {
+        t->err = "list_payload_ids: capacity below the id count";
+        return IKD_ERR_CAPACITY;
+    }
+    std::memcpy(out_ids, out.data(), out.size() * sizeof(uint32_t));
+    return IKD_OK;
+}
+
+int ikd_relabel_payload_ids(ikd_tree_t t, const uint32_t* old_ids, int64_t n) {
+    CHECK_T();
+    if (n < 0 || n >= (1ll << 31) || (n > 0 && !old_ids)) return IKD_ERR_ARG;
+    for (int64_t i = 1; i < n; ++i)
+        if (old_ids[i] <= old_ids[i - 1]) {
+            t->err = "relabel_payload_ids: old ids must be strictly ascending";
+            return IKD_ERR_ARG;
+        }
+    CHK(sync_hdr(t));
+    if (n > 0) {
+        uint32_t* d_old;
+        CHK(upload(t, B_IO4, old_ids, n, &d_old));
+        hipLaunchKernelGGL(k_relabel_ids, dim3(nblk(t->pool_used)), dim3(BS), 0, t->st, t->pool.ids,
+                           (int)t->pool_used, d_old, (int)n);
+        if (t->h_hdr->removed_count > 0)
+            hipLaunchKernelGGL(k_relabel_records, dim3(nblk(t->h_hdr->removed_count)), dim3(BS), 0, t->st, t->removed,
+                               t->h_hdr->removed_count, d_old, (int)n);
+        HIPCHK(hipGetLastError());
+    }
+    t->next_id = (uint32_t)n;
     return sync_hdr(t);
 }
 

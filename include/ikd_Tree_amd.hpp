@@ -24,6 +24,10 @@
 // Payload types (e.g. pcl::PointXYZI) round-trip unchanged: the facade keeps
 // every point the tree may still return host-side under its payload id and
 // looks results up by id, so fields beyond x, y, z come back as they went in.
+// Records of points the tree no longer holds (dropped by the downsample,
+// removed by rebuilds and handed out) are reclaimed once the store has grown
+// to twice what the tree can still return, and ids are renumbered from 0 then,
+// so a long mapping run neither grows the store nor wraps the 32-bit ids.
 //
 // Threading: one writer per tree, as in the reference; reads (Nearest_Search,
 // Box_Search, Radius_Search) may come from many threads at once, e.g. an
@@ -31,6 +35,7 @@
 // tree onto its stream.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <memory>
@@ -128,6 +133,7 @@ public:
         std::vector<uint32_t> ids(payload_.size());
         for (size_t i = 0; i < ids.size(); ++i) ids[i] = (uint32_t)i;
         check(ikd_build(h_, xyz.data(), ids.data(), (int64_t)payload_.size()), "Build");
+        reclaim_at_ = std::max(2 * payload_.size(), min_reclaim_) + min_reclaim_;
         sync_root();
     }
 
@@ -209,6 +215,7 @@ public:
         check(ikd_add_points(h_, xyz.data(), ids.data(), (int64_t)PointToAdd.size(), downsample_on ? 1 : 0, &added),
               "Add_Points");
         sync_root();
+        reclaim_payload();
         return added;
     }
 
@@ -264,6 +271,14 @@ public:
 
     ikd_tree_t handle() const { return h_; }
 
+    // Point records the facade holds (for callers watching memory), and the
+    // store size below which reclaiming never runs (default 1M records).
+    size_t payload_records() const { return payload_.size(); }
+    void set_payload_reclaim_floor(size_t records) {
+        min_reclaim_ = records;
+        reclaim_at_ = std::max(2 * payload_.size(), min_reclaim_) + min_reclaim_;
+    }
+
     PointVector PCL_Storage;          // ikd_Tree.h:246 (public, unused by the library)
     KD_TREE_NODE* Root_Node = nullptr;  // ikd_Tree.h:247 (null while the tree has no nodes)
     int max_queue_size = 0;           // ikd_Tree.h:248
@@ -273,6 +288,24 @@ private:
     KD_TREE_NODE root_handle_;
     PointVector payload_;          // payload id -> point record
     PointVector pending_removed_;  // removed points recorded before the last Build
+    size_t min_reclaim_ = size_t(1) << 20;
+    size_t reclaim_at_ = size_t(2) << 20;  // payload_ size that triggers reclaim_payload()
+
+    // Keeps only the records of ids the tree can still return, renumbered
+    // 0..n-1 on both sides.
+    void reclaim_payload() {
+        if (payload_.size() < reclaim_at_) return;
+        int64_t n = 0;
+        check(ikd_list_payload_ids(h_, nullptr, 0, &n), "list_payload_ids");
+        std::vector<uint32_t> ids((size_t)n);
+        if (n) check(ikd_list_payload_ids(h_, ids.data(), n, &n), "list_payload_ids");
+        PointVector kept;
+        kept.reserve((size_t)n);
+        for (uint32_t id : ids) kept.push_back(payload_[id]);
+        check(ikd_relabel_payload_ids(h_, ids.data(), n), "relabel_payload_ids");
+        payload_.swap(kept);
+        reclaim_at_ = std::max(2 * payload_.size(), min_reclaim_) + min_reclaim_;
+    }
 
     void sync_root() {
         int64_t n = 0;

@@ -208,6 +208,18 @@ int ikd_set_tuning(ikd_tree_t t, int key, int value);
  * (ikd_Tree.cpp:200-350), run when the caller has time, e.g. between scans. */
 int ikd_rebuild_deferred(ikd_tree_t t, int64_t max_points, int32_t* out_rebuilt);
 
+/* Payload-id housekeeping (no counterpart in the reference; for callers that
+ * keep point records by payload id, as the C++ facade does).
+ * ikd_list_payload_ids: the ids the tree can still return, ascending and
+ * unique: every node, live or deleted, and every point in the removed-point
+ * record; *out_n gets the count, out_ids may be NULL to query it.
+ * ikd_relabel_payload_ids: gives the id old_ids[i] the new id i (old_ids
+ * strictly ascending, typically the list above) and continues automatic ids
+ * from n.  Together they let a caller drop the records of points the tree
+ * no longer holds and keep ids below 2^32 however many points pass through. */
+int ikd_list_payload_ids(ikd_tree_t t, uint32_t* out_ids, int64_t capacity, int64_t* out_n);
+int ikd_relabel_payload_ids(ikd_tree_t t, const uint32_t* old_ids, int64_t n);
+
 /* Pre-order dump of the tree (diagnostic, no counterpart in the reference):
  * per node x, y, z, id (4 x 4 bytes), axis, TreeSize, invalid_point_num, flags
  * (bit 0 point_deleted, 1 tree_deleted, 2/3 downsample labels), range[6]

@@ -87,3 +87,14 @@ def test_server_stress_with_deferred_rebuilds(emu_lib, tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=900, env=env)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-3000:]
     assert r.stdout.startswith("ok")
+
+
+def test_facade_reclaims_payload_records(emu_lib, tmp_path):
+    """A long downsampled run through the facade keeps its record store
+    bounded (records of dropped and removed points reclaimed, ids renumbered)
+    and every result keeps its own record (emulation build, ASan)."""
+    exe = tmp_path / "payload_reclaim"
+    env = _build_emu_exe("payload_reclaim.cpp", exe)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=900, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-3000:]
+    assert r.stdout.startswith("ok")
