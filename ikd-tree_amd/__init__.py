@@ -44,6 +44,7 @@ EXPORTS = (
     "ikd_pool_stats", "ikd_get_counters", "ikd_acquire_removed_points", "ikd_set_tuning", "ikd_dump_nodes", "ikd_last_error",
     "ikd_batch_create", "ikd_batch_destroy", "ikd_batch_nearest_search_device", "ikd_batch_synchronize",
     "ikd_batch_stream", "ikd_batch_set_tuning", "ikd_batch_last_error", "ikd_batch_radius_search_device",
+    "ikd_batch_nearest_search_noalloc",
     "ikd_now_ns", "ikd_server_create", "ikd_server_add_map", "ikd_server_submit", "ikd_server_wait",
     "ikd_server_pending", "ikd_server_destroy", "ikd_server_set_maintenance", "ikd_server_stats",
     "ikd_rebuild_deferred", "ikd_list_payload_ids", "ikd_relabel_payload_ids",
@@ -55,6 +56,8 @@ IKD_TUNE_SMALL_BUILD = 4
 IKD_TUNE_DEFER_REBUILD = 5
 IKD_TUNE_FOLD_LANES = 6
 IKD_TUNE_RECORD_REMOVED = 7
+IKD_NOALLOC_MAX_TREES = 128
+IKD_NOALLOC_MAX_QUERIES = 1 << 20
 
 
 class Counters(ctypes.Structure):
@@ -127,6 +130,7 @@ def load_library(path: str | None = None):
         "ikd_batch_create": [ctypes.POINTER(vp)],
         "ikd_batch_destroy": [vp],
         "ikd_batch_nearest_search_device": [vp, vp, i32, vp, vp, i64, vp, i32, vp, f64, vp, vp, vp, vp],
+        "ikd_batch_nearest_search_noalloc": [vp, vp, i32, vp, vp, i64, vp, i32, vp, f64, vp, vp, vp, vp],
         "ikd_batch_synchronize": [vp],
         "ikd_batch_radius_search_device": [vp, vp, i32, vp, vp, vp, i64, vp, vp, vp, i64, vp],
         "ikd_batch_stream": [vp, ctypes.POINTER(vp)],
@@ -451,6 +455,19 @@ class Batch:
         handles = (ctypes.c_void_p * len(trees))(*[t._h.value for t in trees])
         per_q = not isinstance(max_dist, (int, float))
         self._check(self._lib.ikd_batch_nearest_search_device(
+            self._h, handles, len(trees), _tensor_ptr(qtree), _tensor_ptr(q), q.shape[0],
+            None if k is None else _tensor_ptr(k), int(k_max), _tensor_ptr(max_dist) if per_q else None,
+            float("inf") if per_q else float(max_dist), _tensor_ptr(out_xyz), _tensor_ptr(out_d2),
+            _tensor_ptr(out_ids), _tensor_ptr(out_counts)))
+
+    def nearest_search_noalloc(self, trees, qtree, q, k_max: int, out_xyz, out_d2, out_ids, out_counts, k=None,
+                               max_dist=float("inf")):
+        """nearest_search_device without any device allocation (the library's
+        static scratch array; at most IKD_NOALLOC_MAX_TREES trees, and with
+        per-query max_dist at most IKD_NOALLOC_MAX_QUERIES queries)."""
+        handles = (ctypes.c_void_p * len(trees))(*[t._h.value for t in trees])
+        per_q = not isinstance(max_dist, (int, float))
+        self._check(self._lib.ikd_batch_nearest_search_noalloc(
             self._h, handles, len(trees), _tensor_ptr(qtree), _tensor_ptr(q), q.shape[0],
             None if k is None else _tensor_ptr(k), int(k_max), _tensor_ptr(max_dist) if per_q else None,
             float("inf") if per_q else float(max_dist), _tensor_ptr(out_xyz), _tensor_ptr(out_d2),

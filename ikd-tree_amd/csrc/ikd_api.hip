@@ -219,7 +219,9 @@ struct ikd_batch_s {
     std::string err;
     int knn_lanes = -1;
     int knn_stack = 0;
-    int* h_flag = nullptr;  // pinned mirror of the overflow flag
+    int* h_flag = nullptr;  // pinned mirror of the overflow flags (the batch's, the scratch one)
+    char* na_scratch = nullptr;  // this device's allocation-free scratch (ikd_batch_nearest_search_noalloc)
+    int* na_flag = nullptr;      // its overflow flag, once the entry point ran on this batch
     ikd_counters_t ctr{};   // only scratch_grows and host_waits move
     hipMemPool_t mp = nullptr;
     Stager stg;
@@ -2023,6 +2025,44 @@ const char* ikd_last_error(ikd_tree_t t) {
 // ------------------------------------------------------------ multi-tree batches
 enum BatchBuf { BB_TMP = 0, BB_TREES, BB_THR, BB_NEXT, BB_FLAG, BB_CNT, BB_POOLS };
 
+// ------------------------------------- allocation-free multi-tree kNN scratch
+// ikd_batch_nearest_search_noalloc keeps everything beyond its inputs and
+// outputs in this array: 8 persistent-lane query counters and the overflow
+// flag, the tree table, and the per-query max_dist thresholds.  Each GPU holds
+// its own copy for the life of the process; every call on a GPU shares it.
+constexpr size_t NA_CTRL_BYTES = 256;
+constexpr size_t NA_TREES_OFF = NA_CTRL_BYTES;
+constexpr size_t NA_THR_OFF = NA_TREES_OFF + ((IKD_NOALLOC_MAX_TREES * sizeof(TreeRef) + 255) & ~(size_t)255);
+constexpr size_t NA_BYTES = NA_THR_OFF + (size_t)IKD_NOALLOC_MAX_QUERIES * sizeof(float);
+static __device__ __align__(256) char scratch[NA_BYTES];
+
+// The device address of `scratch` on the current device, resolved once per
+// device (host code never uses the array's name as an address).
+static char* noalloc_scratch(int dev) {
+    static std::mutex mu;
+    static char* ptr[64];
+    if (dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> g(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        ptr[dev] = static_cast<char*>(p);
+    }
+    return ptr[dev];
+}
+
+// The tree table travels as a kernel argument and is stored into the scratch.
+struct TreeTable {
+    TreeRef t[IKD_NOALLOC_MAX_TREES];
+};
+__global__ void k_store_trees(TreeTable tab, int n, TreeRef* __restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = tab.t[i];
+}
+
 // shared prologue of batch launches: compact over-deep trees, make the batch
 // stream wait for the trees' pending work, report the deepest tree and the
 // largest pool
@@ -2075,10 +2115,13 @@ int ikd_batch_create(ikd_batch_t* out) {
     b->mp = library_pool(dev);
     if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&b->ev, hipEventDisableTiming) != hipSuccess ||
-        hipHostMalloc(&b->h_flag, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc(&b->h_flag, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
         ikd_batch_destroy(b);
         return IKD_ERR_HIP;
     }
+    // set-up of the allocation-free entry point on this device (its absence
+    // only disables that entry point)
+    b->na_scratch = noalloc_scratch(dev);
     int* flag;
     int rc = ensure(b, BB_FLAG, 1, &flag);
     if (rc == IKD_OK) {
@@ -2191,6 +2234,96 @@ int ikd_batch_nearest_search_device(ikd_batch_t b, const ikd_tree_t* trees, int 
     return batch_end(b, trees, ntrees);
 }
 
+int ikd_batch_nearest_search_noalloc(ikd_batch_t b, const ikd_tree_t* trees, int ntrees,
+                                     const int32_t* d_query_tree, const float* d_queries, int64_t nq,
+                                     const int32_t* d_k, int k_max, const double* d_max_dist, double max_dist,
+                                     float* d_out_xyz, float* d_out_d2, uint32_t* d_out_ids,
+                                     int32_t* d_out_counts) {
+    if (!b) return IKD_ERR_ARG;
+    std::lock_guard<std::mutex> batch_lock_(b->mu);
+    auto* t = b;  // HIPCHK reports into b->err
+    if (!trees || ntrees <= 0 || nq < 0 || nq >= (1ll << 31)) {
+        b->err = "batch (no-alloc): bad tree list or query count";
+        return IKD_ERR_ARG;
+    }
+    if (ntrees > IKD_NOALLOC_MAX_TREES) {
+        b->err = "batch (no-alloc): more than IKD_NOALLOC_MAX_TREES trees";
+        return IKD_ERR_CAPACITY;
+    }
+    if (d_max_dist && nq > IKD_NOALLOC_MAX_QUERIES) {
+        b->err = "batch (no-alloc): per-query max_dist for more than IKD_NOALLOC_MAX_QUERIES queries";
+        return IKD_ERR_CAPACITY;
+    }
+    if (k_max < 1 || k_max > 64) {
+        b->err = "batch (no-alloc): k_max must be in 1..64";
+        return k_max < 1 ? IKD_ERR_ARG : IKD_ERR_UNSUPPORTED;
+    }
+    if (nq == 0) return IKD_OK;
+    if (!d_query_tree || !d_queries || !d_out_xyz || !d_out_d2 || !d_out_ids || !d_out_counts) {
+        b->err = "batch (no-alloc): null device pointer";
+        return IKD_ERR_ARG;
+    }
+    if (!b->na_scratch) {
+        b->err = "batch (no-alloc): the scratch array's device address is unavailable";
+        return IKD_ERR_HIP;
+    }
+    int* ctrl = reinterpret_cast<int*>(b->na_scratch);  // [0, 8) query counters, [8] overflow flag
+    TreeRef* table = reinterpret_cast<TreeRef*>(b->na_scratch + NA_TREES_OFF);
+    float* thr = reinterpret_cast<float*>(b->na_scratch + NA_THR_OFF);
+    TreeTable tab;
+    int depth = 0;
+    long long steps = 16;
+    for (int i = 0; i < ntrees; ++i) {
+        ikd_tree_t tr = trees[i];
+        if (!tr) {
+            b->err = "batch (no-alloc): null tree handle";
+            return IKD_ERR_ARG;
+        }
+        std::lock_guard<std::recursive_mutex> g(tr->mu);
+        if (tr->max_depth + 2 > MAX_DEPTH) {
+            // the compaction that fixes this allocates: leave it to any update call
+            b->err = "batch (no-alloc): a tree is deeper than the traversal stacks; update it once first";
+            return IKD_ERR_UNSUPPORTED;
+        }
+        depth = std::max(depth, tr->max_depth);
+        steps = std::max(steps, tr->pool_used + 16);
+        tab.t[i] = TreeRef{tr->pool.hot, tr->pool.ids};
+        HIPCHK(hipEventRecord(tr->ev, tr->st));
+        HIPCHK(hipStreamWaitEvent(b->st, tr->ev, 0));
+    }
+    hipLaunchKernelGGL(k_store_trees, dim3(1), dim3(IKD_NOALLOC_MAX_TREES), 0, b->st, tab, ntrees, table);
+    launch_fill_i32(ctrl + 8, 0, 1, b->st);
+    KnnArgs a{};
+    a.trees = table;
+    a.qtree = d_query_tree;
+    a.ntrees = ntrees;
+    a.q = d_queries;
+    a.nq = (int)nq;
+    a.k = k_max;
+    a.qk = d_k;
+    a.thr = max_dist_threshold(max_dist);
+    if (d_max_dist) {
+        launch_thresholds(d_max_dist, (int)nq, thr, b->st);
+        a.qthr = thr;
+    }
+    a.out_stride = k_max;
+    a.out_xyz = d_out_xyz;
+    a.out_d2 = d_out_d2;
+    a.out_ids = d_out_ids;
+    a.out_counts = d_out_counts;
+    a.overflow = ctrl + 8;
+    a.max_steps = (int)std::min<long long>(steps, INT_MAX);
+    const int stack = std::min(std::max(depth + 2, b->knn_stack), MAX_DEPTH);
+    const int lanes = b->knn_lanes >= 0 ? b->knn_lanes : auto_lanes((int)nq);
+    if (launch_knn(a, k_max, lanes, stack, b->st, ctrl, 256 * 10) != 0) {
+        b->err = "batch (no-alloc): unsupported launch shape";
+        return IKD_ERR_UNSUPPORTED;
+    }
+    HIPCHK(hipGetLastError());
+    b->na_flag = ctrl + 8;
+    return batch_end(b, trees, ntrees);
+}
+
 int ikd_batch_radius_search_device(ikd_batch_t b, const ikd_tree_t* trees, int ntrees, const int32_t* d_query_tree,
                                    const float* d_centers, const float* d_radii, int64_t nq, int64_t* d_out_offsets,
                                    float* d_out_xyz, uint32_t* d_out_ids, int64_t capacity, int64_t* out_total) {
@@ -2259,9 +2392,11 @@ int ikd_batch_synchronize(ikd_batch_t b) {
     auto* t = b;
     int* flag;
     CHK(ensure(b, BB_FLAG, 1, &flag));
+    b->h_flag[1] = 0;
     HIPCHK(hipMemcpyAsync(b->h_flag, flag, sizeof(int), hipMemcpyDeviceToHost, b->st));
+    if (b->na_flag) HIPCHK(hipMemcpyAsync(b->h_flag + 1, b->na_flag, sizeof(int), hipMemcpyDeviceToHost, b->st));
     HIPCHK(hipStreamSynchronize(b->st));
-    if (*b->h_flag) {
+    if (b->h_flag[0] || b->h_flag[1]) {
         launch_fill_i32(flag, 0, 1, b->st);
         HIPCHK(hipStreamSynchronize(b->st));
         b->err = "batch: device traversal stack overflow";

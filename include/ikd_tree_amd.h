@@ -251,6 +251,26 @@ int ikd_batch_nearest_search_device(ikd_batch_t b, const ikd_tree_t* trees, int 
                                     const double* d_max_dist, double max_dist, float* d_out_xyz, float* d_out_d2,
                                     uint32_t* d_out_ids, int32_t* d_out_counts);
 
+/* Allocation-free form of ikd_batch_nearest_search_device, with the same
+ * arguments and results.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It calls no allocating
+ * routine: the tree table, the per-query thresholds and the launch counters
+ * live in a static device array of each GPU (about 4 MiB, held until the
+ * process ends) that every call on that GPU shares, so calls that overlap on
+ * one GPU race; ikd_batch_synchronize reports a traversal overflow of the
+ * batch's last such call.  Limits: ntrees <= IKD_NOALLOC_MAX_TREES and, with
+ * d_max_dist, nq <= IKD_NOALLOC_MAX_QUERIES (IKD_ERR_CAPACITY beyond them);
+ * a tree deeper than the traversal stacks gives IKD_ERR_UNSUPPORTED (any
+ * update call compacts it). */
+#define IKD_NOALLOC_MAX_TREES 128
+#define IKD_NOALLOC_MAX_QUERIES (1 << 20)
+int ikd_batch_nearest_search_noalloc(ikd_batch_t b, const ikd_tree_t* trees, int ntrees,
+                                     const int32_t* d_query_tree, const float* d_queries, int64_t nq,
+                                     const int32_t* d_k, int k_max, const double* d_max_dist, double max_dist,
+                                     float* d_out_xyz, float* d_out_d2, uint32_t* d_out_ids,
+                                     int32_t* d_out_counts);
+
 /* Radius_Search (ikd_Tree.cpp:406-411) for nq queries spread over trees, in
  * one count launch and one fill launch: query i searches
  * trees[d_query_tree[i]] around d_centers[3i..] with radius d_radii[i] (an

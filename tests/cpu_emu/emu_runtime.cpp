@@ -284,6 +284,10 @@ hipError_t hipEventDestroy(hipEvent_t e) {
 }
 hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
 hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+hipError_t hipGetSymbolAddress(void** p, const void* symbol) {
+    *p = const_cast<void*>(symbol);
+    return hipSuccess;
+}
 hipError_t hipGetLastError() { return hipSuccess; }
 hipError_t hipDeviceSynchronize() { return hipSuccess; }
 const char* hipGetErrorString(hipError_t) { return "emulated HIP error"; }

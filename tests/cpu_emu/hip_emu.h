@@ -26,6 +26,8 @@
 #define __launch_bounds__(...)
 #define __shared__ static
 #define __restrict__ __restrict
+#define __align__(n) alignas(n)
+#define HIP_SYMBOL(x) (&(x))
 
 // ---- vector types ------------------------------------------------------------
 struct dim3 {
@@ -152,6 +154,7 @@ hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned flags);
 hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s);
 hipError_t hipEventSynchronize(hipEvent_t e);
+hipError_t hipGetSymbolAddress(void** p, const void* symbol);
 hipError_t hipGetLastError();
 hipError_t hipDeviceSynchronize();
 const char* hipGetErrorString(hipError_t e);
