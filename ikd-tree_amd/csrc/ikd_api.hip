@@ -201,13 +201,17 @@ struct ikd_tree_s {
     ikd_counters_t ctr{};     // event counts since creation (ikd_get_counters)
     hipMemPool_t mp = nullptr;  // stream-ordered allocations (null: hipMalloc / hipFree)
     Stager stg;                 // pinned staging of small host-to-device copies
+    // The host copies of the header and root summary match the device: the
+    // last call that changed the tree ended with a refresh and succeeded.
+    // Add_Points, the update path of every scan, skips its opening wait then.
+    bool hdr_valid = false;
     // One call at a time per tree: the C ABI locks this for the whole call, so
     // reads from many host threads (FAST-LIO's OpenMP loop of Nearest_Search)
     // serialize onto the tree's stream instead of sharing its scratch slots.
     // Recursive: public entry points call one another.
     std::recursive_mutex mu;
     // scratch
-    DevBuf b[80];
+    DevBuf b[96];
 };
 
 // A multi-tree launch context: its own stream, scratch and error slot.  A
@@ -238,9 +242,9 @@ enum Buf {
     B_PFLAG, B_PSCAN, B_ISEG, B_ISEGD, B_IO0, B_IO1, B_IO2, B_IO3, B_IO4, B_VSTART, B_DELBOX, B_DELFL, B_INSPT,
     B_INSFL, B_CBOX, B_CPTS, B_MISC, B_MORTK, B_MORTV, B_MORTK2, B_MORTV2, B_CNT64, B_OFF64, B_ADDX, B_ADDI, B_PNEXT, B_SPAIRS, B_SPSCAN,
     B_DEFER, B_DINFO, B_DBEFORE, B_DKEEP, B_PTS3, B_PEND, B_START, B_INV0, B_LASTP, B_PSEED, B_DBELOW, B_MAXDEL,
-    B_COUNT
+    B_SLISTS, B_COUNT
 };
-static_assert(B_COUNT <= 80, "scratch table");
+static_assert(B_COUNT <= 96, "scratch table");
 
 #define HIPCHK(x)                                                   \
     do {                                                            \
@@ -546,12 +550,14 @@ int run_build(ikd_tree_t t, const float4* pts, const int* seg_of, int m, const i
     hipStream_t st = t->st;
     if (maxlen <= SMALL_MAX && t->small_build) {
         // every subtree fits one workgroup: one launch builds them all
-        int *pairs, *pscan;
+        int *pairs, *pscan, *lists;
         CHK(ensure(t, B_SPAIRS, S, &pairs));
         CHK(ensure(t, B_SPSCAN, S, &pscan));
-        launch_small_pairs(d_seg, S, nullptr, pairs, st);
+        CHK(ensure(t, B_SLISTS, 2ll * S, &lists));
+        launch_small_pairs(d_seg, S, nullptr, pairs, t->d_hdr, st);
         CHK(exscan_i32(t, pairs, pscan, S));
-        launch_build_small(pts, d_seg, d_seg_depth, pscan, S, nullptr, nullptr, SmallLists{}, t->pool, t->d_hdr, st);
+        launch_build_small(pts, d_seg, d_seg_depth, pscan, S, nullptr, nullptr, SmallLists{}, lists, t->pool,
+                           t->d_hdr, st);
         launch_small_commit(pairs, pscan, S, t->d_hdr, st);
         HIPCHK(hipGetLastError());
         return IKD_OK;
@@ -647,23 +653,24 @@ int run_build(ikd_tree_t t, const float4* pts, const int* seg_of, int m, const i
         const int cur = Ls & 1;
         const int bound = (int)std::min<long long>((long long)S << std::min(Ls, 30), m);
         float4* gpts;
-        int *pairs, *pscan;
+        int *pairs, *pscan, *lists;
         CHK(ensure(t, B_PTS3, m, &gpts));
         CHK(ensure(t, B_SPAIRS, bound, &pairs));
         CHK(ensure(t, B_SPSCAN, bound, &pscan));
+        CHK(ensure(t, B_SLISTS, 2ll * bound, &lists));
         int* inv0;
         CHK(ensure(t, B_INV0, m, &inv0));
         launch_gather_list(pts, perm[cur][0], sid[cur], m, gpts, inv0, st);
         const int* count_dev = &t->d_hdr->seg_count[cur];
-        launch_small_pairs(seg[cur], bound, count_dev, pairs, st);
+        launch_small_pairs(seg[cur], bound, count_dev, pairs, t->d_hdr, st);
         CHK(exscan_i32(t, pairs, pscan, bound));
         // the level-synchronous lists are already sorted per segment
         SmallLists pre;
         pre.list1 = perm[cur][1];
         pre.list2 = perm[cur][2];
         pre.inv0 = inv0;
-        launch_build_small(gpts, seg[cur], segd[cur], pscan, bound, count_dev, perm[cur][0], pre, t->pool, t->d_hdr,
-                           st);
+        launch_build_small(gpts, seg[cur], segd[cur], pscan, bound, count_dev, perm[cur][0], pre, lists, t->pool,
+                           t->d_hdr, st);
         launch_small_commit(pairs, pscan, bound, t->d_hdr, st);
     }
     HIPCHK(hipGetLastError());
@@ -946,7 +953,7 @@ int compact_if_needed(ikd_tree_t t, bool force, bool fresh = false) {
 int insert_points(ikd_tree_t t, const float4* pts, int m, const int* extra_seeds = nullptr,
                   const int* n_extra_dev = nullptr, int extra_bound = 0) {
     if (m <= 0) {
-        if (extra_bound <= 0) return IKD_OK;
+        if (extra_bound <= 0) return refresh(t);  // callers rely on a fresh header and root summary
         CHK(refit_begin(t));
         CHK(refit_mark(t, extra_seeds, n_extra_dev, extra_bound, 0));
         CHK(refit_finish(t));
@@ -1199,18 +1206,21 @@ bool exact_voxels(float ds) {
     return ds > 0.0f && m == 0.5f;
 }
 
-int add_points_dev(ikd_tree_t t, const float* d_xyz, const uint32_t* d_ids, int n, int downsample, int* added) {
+// Ends with the header and root summary fresh.  hdr_valid: they already are
+// at the start.
+int add_points_dev(ikd_tree_t t, const float* d_xyz, const uint32_t* d_ids, int n, int downsample, int* added,
+                   bool hdr_valid) {
     *added = 0;
-    if (n <= 0) return IKD_OK;
+    if (n <= 0) return hdr_valid ? IKD_OK : refresh(t);
     uint32_t base = t->next_id;
     if (!d_ids) t->next_id += (uint32_t)n;
-    CHK(sync_hdr(t));
+    if (!hdr_valid) CHK(sync_hdr(t));
     if (!downsample) {
         float4* pts;
         CHK(ensure(t, B_PTS, n, &pts));
         launch_pack_points(d_xyz, d_ids, base, n, pts, t->st);
         CHK(insert_points(t, pts, n));
-        return compact_if_needed(t, false);
+        return compact_if_needed(t, false, true);  // the insert's rebuild pass refreshed both
     }
     if (!(t->prm.box_length > 0.0f)) {
         t->err = "Add_Points: downsample needs box_length > 0";
@@ -1250,7 +1260,8 @@ int add_points_dev(ikd_tree_t t, const float* d_xyz, const uint32_t* d_ids, int 
             start += idx;
         }
     }
-    return compact_if_needed(t, false);
+    // the last chunk ended with its insert's rebuild pass, which refreshed both
+    return compact_if_needed(t, false, true);
 }
 
 // Lanes per query so that a launch has about 2^17 lanes in flight: a small
@@ -1281,7 +1292,10 @@ int knn_dev(ikd_tree_t t, const float* d_q, int nq, int k, double max_dist, floa
         return IKD_ERR_UNSUPPORTED;
     }
     if (nq <= 0) return IKD_OK;
-    if (t->max_depth + 2 > MAX_DEPTH) CHK(compact_if_needed(t, true));
+    if (t->max_depth + 2 > MAX_DEPTH) {
+        t->hdr_valid = false;
+        CHK(compact_if_needed(t, true));
+    }
     KnnArgs a{};
     a.hot = t->pool.hot;
     a.ids = t->pool.ids;
@@ -1324,7 +1338,10 @@ int knn_dev(ikd_tree_t t, const float* d_q, int nq, int k, double max_dist, floa
 
 int range_dev(ikd_tree_t t, int mode, const float* d_boxes, const float* d_centers, const float* d_radii, int nq,
               int64_t* h_offsets, float* h_xyz, uint32_t* h_ids, int64_t capacity) {
-    if (t->max_depth + 2 > MAX_DEPTH) CHK(compact_if_needed(t, true));
+    if (t->max_depth + 2 > MAX_DEPTH) {
+        t->hdr_valid = false;
+        CHK(compact_if_needed(t, true));
+    }
     hipStream_t st = t->st;
     long long *cnt, *off;
     CHK(ensure(t, B_CNT64, nq + 1, &cnt));
@@ -1462,6 +1479,7 @@ int ikd_initialize(ikd_tree_t t, float d, float b, float l) {
 
 int ikd_build(ikd_tree_t t, const float* xyz, const uint32_t* ids, int64_t n) {
     CHECK_T();
+    t->hdr_valid = false;
     if (n < 0 || (n > 0 && !xyz) || n >= (1ll << 27)) return IKD_ERR_ARG;
     CHK(reset_pool(t));
     t->next_id = ids ? t->next_id : (uint32_t)n;
@@ -1586,9 +1604,12 @@ int ikd_add_points_device(ikd_tree_t t, const float* d_xyz, const uint32_t* d_id
     CHECK_T();
     if (n < 0 || n >= (1ll << 27) || (n > 0 && !d_xyz)) return IKD_ERR_ARG;
     int added = 0;
-    int rc = add_points_dev(t, d_xyz, d_ids, (int)n, downsample_on, &added);
+    const bool valid = t->hdr_valid;
+    t->hdr_valid = false;
+    int rc = add_points_dev(t, d_xyz, d_ids, (int)n, downsample_on, &added, valid);
     if (out_added) *out_added = added;
-    return rc;  // add_points_dev ends with the root summary fresh
+    if (rc == IKD_OK) t->hdr_valid = true;  // add_points_dev ends with both fresh
+    return rc;
 }
 
 int ikd_add_points(ikd_tree_t t, const float* xyz, const uint32_t* ids, int64_t n, int downsample_on,
@@ -1599,14 +1620,15 @@ int ikd_add_points(ikd_tree_t t, const float* xyz, const uint32_t* ids, int64_t 
     if (n == 0) return IKD_OK;
     float* dx;
     uint32_t* di = nullptr;
+    // the caller's buffers outlive the call, which ends with a wait
     CHK(upload(t, B_ADDX, xyz, 3 * n, &dx));
     if (ids) CHK(upload(t, B_ADDI, ids, n, &di));
-    HIPCHK(wait_stream(t));
     return ikd_add_points_device(t, dx, di, n, downsample_on, out_added);
 }
 
 int ikd_delete_points(ikd_tree_t t, const float* xyz, int64_t n) {
     CHECK_T();
+    t->hdr_valid = false;
     if (n < 0 || n >= (1ll << 31) || (n > 0 && !xyz)) return IKD_ERR_ARG;
     if (n == 0) return IKD_OK;
     float* dx;
@@ -1641,21 +1663,28 @@ int ikd_delete_points(ikd_tree_t t, const float* xyz, int64_t n) {
 
 int ikd_delete_point_boxes(ikd_tree_t t, const float* boxes, int64_t nb, int32_t* out_deleted) {
     CHECK_T();
+    const bool valid = t->hdr_valid;
+    t->hdr_valid = false;
     if (nb < 0 || nb >= (1ll << 31) || (nb > 0 && !boxes)) return IKD_ERR_ARG;
     if (out_deleted) *out_deleted = 0;
-    if (nb == 0) return IKD_OK;
-    CHK(read_root(t));
+    if (nb == 0) {
+        t->hdr_valid = valid;
+        return IKD_OK;
+    }
+    if (!valid) CHK(read_root(t));
     long long before = live_count(t);
     float* db;
     CHK(upload(t, B_IO4, boxes, 6 * nb, &db));
     CHK(delete_boxes_dev(t, db, (int)nb, 0));
-    CHK(compact_if_needed(t, false));
+    CHK(compact_if_needed(t, false));  // ends with the header and root summary fresh
     if (out_deleted) *out_deleted = (int32_t)(before - live_count(t));
+    t->hdr_valid = true;
     return IKD_OK;
 }
 
 int ikd_add_point_boxes(ikd_tree_t t, const float* boxes, int64_t nb) {
     CHECK_T();
+    t->hdr_valid = false;
     if (nb < 0 || nb >= (1ll << 31) || (nb > 0 && !boxes)) return IKD_ERR_ARG;
     if (nb == 0) return IKD_OK;
     CHK(sync_hdr(t));
@@ -1777,6 +1806,7 @@ int ikd_list_payload_ids(ikd_tree_t t, uint32_t* out_ids, int64_t capacity, int6
 
 int ikd_relabel_payload_ids(ikd_tree_t t, const uint32_t* old_ids, int64_t n) {
     CHECK_T();
+    t->hdr_valid = false;
     if (n < 0 || n >= (1ll << 31) || (n > 0 && !old_ids)) return IKD_ERR_ARG;
     for (int64_t i = 1; i < n; ++i)
         if (old_ids[i] <= old_ids[i - 1]) {
@@ -1884,6 +1914,7 @@ int ikd_stream(ikd_tree_t t, void** out) {
 
 int ikd_rebuild_deferred(ikd_tree_t t, int64_t max_points, int32_t* out_rebuilt) {
     CHECK_T();
+    t->hdr_valid = false;
     if (out_rebuilt) *out_rebuilt = 0;
     CHK(sync_hdr(t));
     CHK(check_count(t, "deferred rebuilds", t->h_hdr->defer_count, 0, INT_MAX));
@@ -1991,6 +2022,7 @@ int ikd_pool_stats(ikd_tree_t t, int64_t* used, int64_t* capacity, int32_t* max_
 
 int ikd_acquire_removed_points(ikd_tree_t t, float* out_xyz, uint32_t* out_ids, int64_t capacity, int64_t* out_n) {
     CHECK_T();
+    t->hdr_valid = false;
     if (!out_n || capacity < 0 || (capacity > 0 && !out_xyz)) return IKD_ERR_ARG;
     CHK(sync_hdr(t));
     const long long n = t->h_hdr->removed_count;
@@ -2078,6 +2110,7 @@ static int batch_begin(ikd_batch_t b, const ikd_tree_t* trees, int ntrees, int* 
         }
         std::lock_guard<std::recursive_mutex> g(tr->mu);
         if (tr->max_depth + 2 > MAX_DEPTH) {
+            tr->hdr_valid = false;
             int rc = compact_if_needed(tr, true);
             if (rc != IKD_OK) {
                 b->err = "batch: " + tr->err;

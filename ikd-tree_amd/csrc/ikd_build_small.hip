@@ -12,12 +12,19 @@
 // Node allocation is deterministic: segment g takes the pair range that
 // starts at pair_used + exclusive_scan(pairs)[g], where pairs(m) is the number
 // of nodes with children in the median tree of m points.
+#include <algorithm>
+
 #include "ikd_internal.h"
 
 namespace ikd {
 
-// Two shapes: 64 threads for segments of at most 128 points (most insert
-// groups hold a handful), 256 threads for the rest up to SMALL_MAX.
+// Three shapes: one lane per segment of at most MICRO_MAX points (most insert
+// groups hold one to a few points), a 64-thread workgroup for segments of at
+// most TINY_MAX points, a 256-thread one for the rest up to SMALL_MAX.  The
+// lane kernel also lists the larger segments for the two workgroup kernels,
+// which loop over their list, so no workgroup is launched only to find its
+// segment belongs to the other shape.
+constexpr int MICRO_MAX = 8;
 constexpr int TINY_MAX = 128;
 
 // Nodes with at least one child in the median-split tree of m points.  A node
@@ -53,8 +60,12 @@ __device__ int median_tree_pairs(int m) {
 // count_dev (optional) holds the number of live segments among the S slots;
 // slots past it take no pairs.
 __global__ void k_small_pairs(const int4* __restrict__ seg, int S, const int* __restrict__ count_dev,
-                              int* __restrict__ pairs) {
+                              int* __restrict__ pairs, DevHdr* hdr) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g == 0) {  // the size-class lists of this build start empty
+        hdr->small_lists[0] = 0;
+        hdr->small_lists[1] = 0;
+    }
     if (g >= S) return;
     const int count = count_dev ? *count_dev : S;
     pairs[g] = g < count ? median_tree_pairs(seg[g].y - seg[g].x + 1) : 0;
@@ -163,7 +174,8 @@ template <int SB, int SMAX>
 __global__ __launch_bounds__(SB) void k_build_small(const float4* __restrict__ pts, const int4* __restrict__ seg,
                                                      const int* __restrict__ seg_depth,
                                                      const int* __restrict__ pair_scan,
-                                                     const int* __restrict__ count_dev,
+                                                     const int* __restrict__ list,
+                                                     const int* __restrict__ list_count,
                                                      const uint32_t* __restrict__ orig, SmallLists pre, Pool pool,
                                                      DevHdr* hdr) {
     constexpr int SMAX_SEG = SMAX / 2 + 1;
@@ -179,14 +191,15 @@ __global__ __launch_bounds__(SB) void k_build_small(const float4* __restrict__ p
     __shared__ unsigned long long wtot[SB / 64];
     __shared__ unsigned short rnk[SMAX];
 
-    const int g = blockIdx.x;
     const int t = threadIdx.x;
-    if (count_dev && g >= *count_dev) return;
+    const int nlist = *list_count;
+    for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
+    const int g = list[li];
     const int4 sg = seg[g];
     const int l0 = sg.x;
     const int m = sg.y - sg.x + 1;
-    // each shape takes its own size class; the host guarantees m <= SMALL_MAX
-    if (m <= 0 || m > SMAX || (SMAX > TINY_MAX && m <= TINY_MAX)) return;
+    // the list holds this shape's size class; the host guarantees m <= SMALL_MAX
+    if (m <= 0 || m > SMAX) continue;
     const int depth0 = seg_depth[g];
     int next_pair = hdr->pair_used + pair_scan[g];
 
@@ -388,22 +401,154 @@ __global__ __launch_bounds__(SB) void k_build_small(const float4* __restrict__ p
         ns = tot_children;
         cur ^= 1;
     }
+    __syncthreads();  // the next segment of the list reuses the LDS arrays
+    }
+}
+
+// One lane builds one segment of at most MICRO_MAX points: the median tree of
+// k_build_small (axis of the longest extent, ties to the lower axis; node =
+// the element of rank (n-1)>>1 in (coordinate, position) order; both sides
+// keep the segment's order), in registers.  Pairs are taken from the
+// segment's range one internal node at a time.  Larger segments (or every
+// segment when micro is off) go to the size-class lists.
+__global__ void k_build_micro(const float4* __restrict__ pts, const int4* __restrict__ seg,
+                              const int* __restrict__ seg_depth, const int* __restrict__ pair_scan, int S,
+                              const int* __restrict__ count_dev, int micro, Pool pool, DevHdr* hdr,
+                              int* __restrict__ tiny_list, int* __restrict__ small_list) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= S || (count_dev && g >= *count_dev)) return;
+    const int4 sg = seg[g];
+    const int m = sg.y - sg.x + 1;
+    if (m <= 0) return;
+    if (!micro || m > MICRO_MAX) {
+        if (m <= TINY_MAX) tiny_list[atomicAdd(&hdr->small_lists[0], 1)] = g;
+        else small_list[atomicAdd(&hdr->small_lists[1], 1)] = g;
+        return;
+    }
+    float4 p[MICRO_MAX];
+    int idx[MICRO_MAX];  // positions, each work range in segment order
+#pragma unroll
+    for (int i = 0; i < MICRO_MAX; ++i) {
+        if (i < m) p[i] = pts[sg.x + i];
+        idx[i] = i;
+    }
+    int next_pair = hdr->pair_used + pair_scan[g];
+    // work ranges [lo, hi) of idx with their slot node, parent and depth
+    int wlo[MICRO_MAX], whi[MICRO_MAX], wnode[MICRO_MAX], wpar[MICRO_MAX], wdep[MICRO_MAX];
+    int nw = 1;
+    wlo[0] = 0;
+    whi[0] = m;
+    wnode[0] = sg.z;
+    wpar[0] = sg.w;
+    wdep[0] = seg_depth[g];
+    int deepest = 0;
+    while (nw > 0) {
+        --nw;
+        const int lo = wlo[nw], hi = whi[nw], node = wnode[nw], parent = wpar[nw], depth = wdep[nw];
+        const int cnt = hi - lo;
+        // bounds in key order, as the sorted lists give them (-0 below +0)
+        const float4 q0 = p[idx[lo]];
+        float mn[3] = {q0.x, q0.y, q0.z}, mx[3] = {q0.x, q0.y, q0.z};
+        for (int i = lo + 1; i < hi; ++i) {
+            const float4 q = p[idx[i]];
+            const float c[3] = {q.x, q.y, q.z};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                if (float_key(c[a]) < float_key(mn[a])) mn[a] = c[a];
+                if (float_key(c[a]) > float_key(mx[a])) mx[a] = c[a];
+            }
+        }
+        const float ext[3] = {mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]};
+        int axis = 0;
+        if (ext[1] > ext[axis]) axis = 1;
+        if (ext[2] > ext[axis]) axis = 2;
+        // the element of rank (cnt-1)>>1 in (coordinate, position) order
+        const int want = (cnt - 1) >> 1;
+        int piv = -1;
+        for (int i = lo; i < hi; ++i) {
+            const float4 a = p[idx[i]];
+            const uint32_t ka = float_key(axis == 0 ? a.x : (axis == 1 ? a.y : a.z));
+            int rank = 0;
+            for (int j = lo; j < hi; ++j) {
+                const float4 b = p[idx[j]];
+                const uint32_t kb = float_key(axis == 0 ? b.x : (axis == 1 ? b.y : b.z));
+                rank += (kb < ka || (kb == ka && idx[j] < idx[i])) ? 1 : 0;
+            }
+            if (rank == want) piv = i;
+        }
+        const int pos = idx[piv];
+        const float pk = (axis == 0 ? p[pos].x : (axis == 1 ? p[pos].y : p[pos].z));
+        const uint32_t kp = float_key(pk);
+        // stable split: ranks below the pivot's first, then the ones above
+        int tmp[MICRO_MAX];
+        int nl = 0, nr = 0;
+        for (int i = lo; i < hi; ++i) {
+            if (i == piv) continue;
+            const float4 b = p[idx[i]];
+            const uint32_t kb = float_key(axis == 0 ? b.x : (axis == 1 ? b.y : b.z));
+            if (kb < kp || (kb == kp && idx[i] < pos)) tmp[nl++] = idx[i];
+        }
+        for (int i = lo; i < hi; ++i) {
+            if (i == piv) continue;
+            const float4 b = p[idx[i]];
+            const uint32_t kb = float_key(axis == 0 ? b.x : (axis == 1 ? b.y : b.z));
+            if (!(kb < kp || (kb == kp && idx[i] < pos))) tmp[nl + nr++] = idx[i];
+        }
+        for (int i = 0; i < nl + nr; ++i) idx[lo + (i < nl ? i : i + 1)] = tmp[i];
+        idx[lo + nl] = pos;
+        const int hl = nl > 0, hr = nr > 0;
+        const int pair = (hl || hr) ? next_pair++ : 0;
+        const uint32_t link = ((uint32_t)pair << LK_PAIR_SHIFT) | (hl ? LK_HASL : 0u) | (hr ? LK_HASR : 0u);
+        const float4 P4 = p[pos];
+        *reinterpret_cast<float4*>(pool.hot[node].f) = make_float4(P4.x, P4.y, P4.z, bitsf(link));
+        pool.ids[node] = fbits(P4.w);
+        pool.par[node] = parent;
+        pool.cnt[node] = make_int4(cnt, 0, 0, depth);
+        pool.meta[node] = (uint32_t)axis;
+        const float box[6] = {mn[0], mn[1], mn[2], mx[0], mx[1], mx[2]};
+        write_child_box_s(pool.hot, parent, node & 1, box);
+        deepest = depth > deepest ? depth : deepest;
+        if (hr) {
+            wlo[nw] = lo + nl + 1;
+            whi[nw] = hi;
+            wnode[nw] = 2 * pair + 1;
+            wpar[nw] = node;
+            wdep[nw] = depth + 1;
+            ++nw;
+        }
+        if (hl) {
+            wlo[nw] = lo;
+            whi[nw] = lo + nl;
+            wnode[nw] = 2 * pair;
+            wpar[nw] = node;
+            wdep[nw] = depth + 1;
+            ++nw;
+        }
+    }
+    atomicMax(&hdr->max_depth, deepest);
 }
 
 static inline int blocks_for(long long n, int bs) { return (int)((n + bs - 1) / bs); }
 
-void launch_small_pairs(const int4* seg, int S, const int* count_dev, int* pairs, hipStream_t s) {
+void launch_small_pairs(const int4* seg, int S, const int* count_dev, int* pairs, DevHdr* hdr, hipStream_t s) {
     if (S <= 0) return;
-    hipLaunchKernelGGL(k_small_pairs, dim3(blocks_for(S, 256)), dim3(256), 0, s, seg, S, count_dev, pairs);
+    hipLaunchKernelGGL(k_small_pairs, dim3(blocks_for(S, 256)), dim3(256), 0, s, seg, S, count_dev, pairs, hdr);
 }
 void launch_build_small(const float4* pts, const int4* seg, const int* seg_depth, const int* pair_scan, int S,
-                        const int* count_dev, const uint32_t* orig, SmallLists pre, Pool pool, DevHdr* hdr,
-                        hipStream_t s) {
+                        const int* count_dev, const uint32_t* orig, SmallLists pre, int* lists, Pool pool,
+                        DevHdr* hdr, hipStream_t s) {
     if (S <= 0) return;
-    hipLaunchKernelGGL((k_build_small<64, TINY_MAX>), dim3(S), dim3(64), 0, s, pts, seg, seg_depth, pair_scan,
-                       count_dev, orig, pre, pool, hdr);
-    hipLaunchKernelGGL((k_build_small<256, SMALL_MAX>), dim3(S), dim3(256), 0, s, pts, seg, seg_depth, pair_scan,
-                       count_dev, orig, pre, pool, hdr);
+    // the lane build follows the position tie rule only: presorted or
+    // original-order inputs go to the workgroup builds
+    const int micro = (!orig && !pre.inv0) ? 1 : 0;
+    hipLaunchKernelGGL(k_build_micro, dim3(blocks_for(S, 256)), dim3(256), 0, s, pts, seg, seg_depth, pair_scan, S,
+                       count_dev, micro, pool, hdr, lists, lists + S);
+    // each workgroup loops over its shape's list; the grids only need to be
+    // wide enough to fill the device when a list is long
+    hipLaunchKernelGGL((k_build_small<64, TINY_MAX>), dim3(std::min(S, 2048)), dim3(64), 0, s, pts, seg, seg_depth,
+                       pair_scan, lists, &hdr->small_lists[0], orig, pre, pool, hdr);
+    hipLaunchKernelGGL((k_build_small<256, SMALL_MAX>), dim3(std::min(S, 512)), dim3(256), 0, s, pts, seg,
+                       seg_depth, pair_scan, lists + S, &hdr->small_lists[1], orig, pre, pool, hdr);
 }
 void launch_small_commit(const int* pairs, const int* pair_scan, int S, DevHdr* hdr, hipStream_t s) {
     if (S <= 0) return;

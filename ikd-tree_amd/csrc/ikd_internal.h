@@ -34,6 +34,7 @@ struct DevHdr {
     int removed_count;  // points rebuilds removed since the last acquire (kept across calls)
     float alpha_bal;    // the root's alpha_bal / alpha_del as of its last Update (cpp:1315-1321)
     float alpha_del;
+    int small_lists[2]; // small build: segments listed for the 64- and 256-thread workgroup builds
 };
 static_assert(sizeof(DevHdr) % sizeof(int) == 0, "the header is cleared as ints");
 constexpr int DEFER_CAP = 4096;  // deferred-rebuild list entries
@@ -85,7 +86,7 @@ void launch_build_side(const BuildLevel& L, DevHdr* hdr, hipStream_t s);
 void launch_build_flags(const BuildLevel& L, hipStream_t s);
 void launch_build_scatter(const BuildLevel& L, hipStream_t s);
 
-void launch_small_pairs(const int4* seg, int S, const int* count_dev, int* pairs, hipStream_t s);
+void launch_small_pairs(const int4* seg, int S, const int* count_dev, int* pairs, DevHdr* hdr, hipStream_t s);
 // Presorted lists for the small build (all null: the kernel sorts): the
 // segments' y- and z-ordered lists of point indices (list1, list2) and, for
 // every point, its position in the x-ordered list the points were gathered in.
@@ -94,9 +95,10 @@ struct SmallLists {
     const uint32_t* list2 = nullptr;
     const int* inv0 = nullptr;
 };
+// lists: 2 S ints of scratch (the size-class lists)
 void launch_build_small(const float4* pts, const int4* seg, const int* seg_depth, const int* pair_scan, int S,
-                        const int* count_dev, const uint32_t* orig, SmallLists pre, Pool pool, DevHdr* hdr,
-                        hipStream_t s);
+                        const int* count_dev, const uint32_t* orig, SmallLists pre, int* lists, Pool pool,
+                        DevHdr* hdr, hipStream_t s);
 void launch_gather_list(const float4* pts, const uint32_t* perm, const int* sid, int m, float4* out, int* inv,
                         hipStream_t s);
 void launch_small_commit(const int* pairs, const int* pair_scan, int S, DevHdr* hdr, hipStream_t s);
