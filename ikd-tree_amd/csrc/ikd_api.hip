@@ -140,6 +140,34 @@ struct RootInfo {
     int4 child;  // TreeSize of left, right child (-1 if absent), link word, pad
     float box[8];
 };
+// Subtrees to rebuild whose summaries come back with the header in one copy
+// (more than this: a second read).
+constexpr int TOPINFO_CAP = 1024;
+// What one host wait brings back: the header, the root summary and, after a
+// refit, (node, live points, depth, parent) and the node count of the first
+// TOPINFO_CAP topmost violators.  One device block and its pinned mirror.
+struct DevStatus {
+    DevHdr hdr;
+    RootInfo root;
+    int4 top_info[TOPINFO_CAP];
+    int top_before[TOPINFO_CAP];
+};
+constexpr size_t STATUS_SHORT = offsetof(DevStatus, top_info);  // header + root summary
+__global__ void k_top_info(Pool pool, const int* __restrict__ top, DevStatus* s) {
+    const int n = min(s->hdr.top_count, TOPINFO_CAP);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int u = top[2 * i];
+        const int4 c = pool.cnt[u];
+        s->top_info[i] = make_int4(u, c.x - c.y, c.w, pool.par[u]);
+        s->top_before[i] = top[2 * i + 1];
+    }
+}
+// Enumerate: the walk must have written exactly the live points the
+// subtrees' counts promise (overflow bit 32 otherwise, reported by the next
+// header read).
+__global__ void k_enum_check(DevHdr* hdr, int expect) {
+    if (hdr->rec_count != expect) atomicOr(&hdr->overflow, 32);
+}
 __global__ void k_root_info(Pool pool, RootInfo* out) {
     RootInfo r = {};
     uint32_t lk0 = fbits(pool.hot[SENTINEL].f[3]);
@@ -181,9 +209,11 @@ struct ikd_tree_s {
     // node pool
     long long cap = 0;
     Pool pool{};
-    DevHdr* d_hdr = nullptr;
-    DevHdr* h_hdr = nullptr;  // pinned mirror
-    RootInfo* d_root = nullptr;
+    DevStatus* d_status = nullptr;  // header, root summary, rebuild list
+    DevStatus* h_status = nullptr;  // its pinned mirror
+    DevHdr* d_hdr = nullptr;        // &d_status->hdr
+    DevHdr* h_hdr = nullptr;
+    RootInfo* d_root = nullptr;  // &d_status->root
     RootInfo* h_root = nullptr;
     uint32_t next_id = 0;
     long long rebuilds = 0;
@@ -242,7 +272,7 @@ enum Buf {
     B_PFLAG, B_PSCAN, B_ISEG, B_ISEGD, B_IO0, B_IO1, B_IO2, B_IO3, B_IO4, B_VSTART, B_DELBOX, B_DELFL, B_INSPT,
     B_INSFL, B_CBOX, B_CPTS, B_MISC, B_MORTK, B_MORTV, B_MORTK2, B_MORTV2, B_CNT64, B_OFF64, B_ADDX, B_ADDI, B_PNEXT, B_SPAIRS, B_SPSCAN,
     B_DEFER, B_DINFO, B_DBEFORE, B_DKEEP, B_PTS3, B_PEND, B_START, B_INV0, B_LASTP, B_PSEED, B_DBELOW, B_MAXDEL,
-    B_SLISTS, B_COUNT
+    B_SLISTS, B_ENUMROOT, B_ENUMGRP, B_COUNT
 };
 static_assert(B_COUNT <= 96, "scratch table");
 
@@ -336,7 +366,13 @@ int ensure(Owner* t, int slot, long long count, T** out) {
 // on the latency path of an update.
 hipError_t wait_stream(ikd_tree_t t) {
     t->ctr.host_waits += 1;
-    return hipStreamSynchronize(t->st);
+    const hipError_t e = hipStreamSynchronize(t->st);
+    if (e == hipSuccess) {
+        // every staged copy has run: the staging buffer is free again
+        t->stg.used = 0;
+        t->stg.pending = false;
+    }
+    return e;
 }
 
 // Copies `bytes` of host memory to device `dst` on the owner's stream through
@@ -392,17 +428,23 @@ int check_count(Owner* t, const char* what, long long v, long long lo, long long
     return IKD_ERR_INTERNAL;
 }
 
-int sync_hdr(ikd_tree_t t) {
-    HIPCHK(hipMemcpyAsync(t->h_hdr, t->d_hdr, sizeof(DevHdr), hipMemcpyDeviceToHost, t->st));
+// Copies the first `bytes` of the status block back and waits.
+int read_status(ikd_tree_t t, size_t bytes) {
+    HIPCHK(hipMemcpyAsync(t->h_status, t->d_status, bytes, hipMemcpyDeviceToHost, t->st));
     HIPCHK(wait_stream(t));
     t->pool_used = 2ll * t->h_hdr->pair_used;
     t->max_depth = t->h_hdr->max_depth;
-    if (t->h_hdr->overflow) {
-        t->err = "device traversal stack overflow";
+    if (const int o = t->h_hdr->overflow) {
+        // each bit names the kernel family whose bounded buffer ran out
+        t->err = (o & 32)   ? "enumerate: the walk disagrees with the subtree counts or outgrew its frontier"
+                 : (o & 4)  ? "Add_Points: a downsample walk outgrew its stack"
+                 : (o & 16) ? "update: a walk outgrew its stack or buffer"
+                            : "device traversal stack overflow";
         return IKD_ERR_INTERNAL;
     }
     return IKD_OK;
 }
+int sync_hdr(ikd_tree_t t) { return read_status(t, sizeof(DevHdr)); }
 
 int read_root(ikd_tree_t t) {
     hipLaunchKernelGGL(k_root_info, dim3(1), dim3(1), 0, t->st, t->pool, t->d_root);
@@ -410,11 +452,10 @@ int read_root(ikd_tree_t t) {
     HIPCHK(wait_stream(t));
     return IKD_OK;
 }
-// Header and root summary in one host wait.
+// Header and root summary in one copy and one host wait.
 int refresh(ikd_tree_t t) {
     hipLaunchKernelGGL(k_root_info, dim3(1), dim3(1), 0, t->st, t->pool, t->d_root);
-    HIPCHK(hipMemcpyAsync(t->h_root, t->d_root, sizeof(RootInfo), hipMemcpyDeviceToHost, t->st));
-    return sync_hdr(t);
+    return read_status(t, STATUS_SHORT);
 }
 bool has_root(ikd_tree_t t) { return (t->h_root->child.z & (int)LK_HASL) != 0; }
 long long live_count(ikd_tree_t t) {
@@ -451,8 +492,54 @@ int sort_keys(ikd_tree_t t, const unsigned long long* kin, unsigned long long* k
     HIPCHK(prim_sort_keys_u64(tmp, bytes, kin, kout, n, 0, end_bit, t->st));
     return IKD_OK;
 }
+// Exclusive scans of up to SCAN_ONE_MAX values run as one workgroup in one
+// launch: each of the 1024 threads sums a run of consecutive values, the run
+// totals are scanned across the wavefronts, then each thread rewrites its run.
+// (The device-wide scan costs two launches and a temporary-size query, which
+// dominates at these sizes.)  in == out is allowed.
+constexpr int SCAN_ONE_MAX = 1 << 17;
+__device__ __forceinline__ int scan_shfl(int x, int src) { return __shfl(x, src); }
+__device__ __forceinline__ long long scan_shfl(long long x, int src) {
+    const unsigned lo = (unsigned)__shfl((int)(unsigned)x, src);
+    const unsigned hi = (unsigned)__shfl((int)(unsigned)((unsigned long long)x >> 32), src);
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ unsigned long long scan_shfl(unsigned long long x, int src) {
+    return (unsigned long long)scan_shfl((long long)x, src);
+}
+template <class T>
+__global__ __launch_bounds__(1024) void k_scan_one(const T* in, T* out, int n) {
+    __shared__ T wave_tot[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int chunk = (n + 1023) >> 10;
+    const int b = min(t * chunk, n), e = min(b + chunk, n);
+    T sum = 0;
+#pragma unroll 8
+    for (int i = b; i < e; ++i) sum += in[i];
+    T inc = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const T o = scan_shfl(inc, (lane - d) & 63);  // lanes below d ignore what they read
+        if (lane >= d) inc += o;
+    }
+    if (lane == 63) wave_tot[w] = inc;
+    __syncthreads();
+    T run = inc - sum;
+    for (int i = 0; i < w; ++i) run += wave_tot[i];
+#pragma unroll 8
+    for (int i = b; i < e; ++i) {
+        const T v = in[i];
+        out[i] = run;
+        run += v;
+    }
+}
+
 int exscan_u64(ikd_tree_t t, const unsigned long long* in, unsigned long long* out, int n) {
     if (n <= 0) return IKD_OK;
+    if (n <= SCAN_ONE_MAX) {
+        hipLaunchKernelGGL(k_scan_one<unsigned long long>, dim3(1), dim3(1024), 0, t->st, in, out, n);
+        return IKD_OK;
+    }
     size_t bytes = 0;
     HIPCHK(prim_exscan_u64(nullptr, bytes, in, out, n, t->st));
     void* tmp;
@@ -462,6 +549,10 @@ int exscan_u64(ikd_tree_t t, const unsigned long long* in, unsigned long long* o
 }
 int exscan_i32(ikd_tree_t t, const int* in, int* out, int n) {
     if (n <= 0) return IKD_OK;
+    if (n <= SCAN_ONE_MAX) {
+        hipLaunchKernelGGL(k_scan_one<int>, dim3(1), dim3(1024), 0, t->st, in, out, n);
+        return IKD_OK;
+    }
     size_t bytes = 0;
     HIPCHK(prim_exscan_i32(nullptr, bytes, in, out, n, t->st));
     void* tmp;
@@ -472,6 +563,10 @@ int exscan_i32(ikd_tree_t t, const int* in, int* out, int n) {
 template <class Owner>
 int exscan_i64(Owner* t, const long long* in, long long* out, int n) {
     if (n <= 0) return IKD_OK;
+    if (n <= SCAN_ONE_MAX) {
+        hipLaunchKernelGGL(k_scan_one<long long>, dim3(1), dim3(1024), 0, t->st, in, out, n);
+        return IKD_OK;
+    }
     size_t bytes = 0;
     HIPCHK(prim_exscan_i64(nullptr, bytes, in, out, n, t->st));
     void* tmp;
@@ -685,6 +780,10 @@ int run_build(ikd_tree_t t, const float4* pts, const int* seg_of, int m, const i
 // Subtree nodes one workgroup enumerates: a level costs it about what a
 // launch costs, so it only pays while every level fits one pass of 1024 threads.
 constexpr long long ENUM_BLOCK_MAX = 1024;
+// Grouped walk: nodes per workgroup, and the largest subtree it takes (a
+// larger one walks one level per launch over the whole grid instead).
+constexpr long long ENUM_GROUP_NODES = 2048;
+constexpr long long ENUM_GROUP_ROOT_MAX = 16384;
 
 // Room for `extra` more records beyond the current count (fresh in h_hdr):
 // the record survives the growth, unlike scratch.
@@ -714,12 +813,38 @@ int reserve_removed(ikd_tree_t t, long long extra) {
 // tree's deepest level, so max_depth - min_depth + 1 levels cover them).
 // record: a rebuild's flatten (DELETE_POINTS_REC, cpp:638); plain flattens
 // (NOT_RECORD) pass false.
+// nodes (nullable): each root's subtree node count; when every subtree is
+// small, one launch walks them all (a workgroup per group of subtrees).
+// Queues the walk without waiting: a count that disagrees with the walk is
+// reported by the next header read.
 int enumerate_live(ikd_tree_t t, const std::vector<int>& roots, const std::vector<long long>& live,
                    long long total_nodes, long long total_live, int min_depth, bool record, float4** out_pts,
-                   int** out_seg) {
+                   int** out_seg, const std::vector<int>* nodes = nullptr) {
     hipStream_t st = t->st;
     int R = (int)roots.size();
     long long cap = std::max<long long>(total_nodes, 1);
+    // grouped walk: consecutive roots share a workgroup up to ENUM_GROUP_NODES nodes
+    bool grouped = nodes != nullptr && R > 0;
+    std::vector<int4> groups;
+    if (grouped) {
+        long long off = 0, gn = 0;
+        int first = 0;
+        for (int i = 0; i < R && grouped; ++i) {
+            const long long ni = std::max<long long>((*nodes)[i], 1);
+            if (ni > ENUM_GROUP_ROOT_MAX) grouped = false;
+            if (i > first && gn + ni > ENUM_GROUP_NODES) {
+                groups.push_back(make_int4(first, i - first, (int)off, (int)gn));
+                off += gn;
+                first = i;
+                gn = 0;
+            }
+            gn += ni;
+        }
+        if (grouped) {
+            groups.push_back(make_int4(first, R - first, (int)off, (int)gn));
+            cap = std::max(cap, off + gn);
+        }
+    }
     int4 *f0, *f1;
     CHK(ensure(t, B_FRONT0, cap, &f0));
     CHK(ensure(t, B_FRONT1, cap, &f1));
@@ -734,9 +859,6 @@ int enumerate_live(ikd_tree_t t, const std::vector<int>& roots, const std::vecto
         h[i] = make_int4(roots[i], (int)base, i, 0);
         base += live[i];
     }
-    CHK(stage_copy(t, f0, h.data(), R * sizeof(int4)));
-    int init[3] = {R, 0, 0};
-    CHK(stage_copy(t, t->d_hdr->front3, init, sizeof(init)));
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, t->d_hdr, 2);
     const int levels = std::max(t->max_depth - std::max(min_depth, 0), 0) + 2;
     float4* rec = nullptr;
@@ -747,25 +869,32 @@ int enumerate_live(ikd_tree_t t, const std::vector<int>& roots, const std::vecto
         rec = t->removed;
         rec_cap = (int)t->removed_cap;
     }
-    if (cap <= ENUM_BLOCK_MAX) {
-        // small subtrees: one workgroup walks all levels
-        launch_enum_preorder_block(t->pool, f0, f1, R, pts, segof, t->d_hdr, (int)cap, std::max(n, 1), levels, rec,
-                                   rec_cap, st);
+    if (grouped) {
+        int4 *droots, *dgroups;
+        CHK(ensure(t, B_ENUMROOT, R, &droots));
+        CHK(ensure(t, B_ENUMGRP, (long long)groups.size(), &dgroups));
+        CHK(stage_copy(t, droots, h.data(), R * sizeof(int4)));
+        CHK(stage_copy(t, dgroups, groups.data(), groups.size() * sizeof(int4)));
+        launch_enum_groups(t->pool, droots, dgroups, (int)groups.size(), f0, f1, pts, segof, t->d_hdr, std::max(n, 1),
+                           levels, rec, rec_cap, st);
     } else {
-        for (int L = 0; L < levels; ++L) {
-            int4* fin = (L & 1) ? f1 : f0;
-            int4* fout = (L & 1) ? f0 : f1;
-            launch_enum_preorder(t->pool, fin, L, fout, pts, segof, t->d_hdr, (int)cap, std::max(n, 1), rec, rec_cap,
-                                 st);
+        CHK(stage_copy(t, f0, h.data(), R * sizeof(int4)));
+        int init[3] = {R, 0, 0};
+        CHK(stage_copy(t, t->d_hdr->front3, init, sizeof(init)));
+        if (cap <= ENUM_BLOCK_MAX) {
+            // small subtrees: one workgroup walks all levels
+            launch_enum_preorder_block(t->pool, f0, f1, R, pts, segof, t->d_hdr, (int)cap, std::max(n, 1), levels,
+                                       rec, rec_cap, st);
+        } else {
+            for (int L = 0; L < levels; ++L) {
+                int4* fin = (L & 1) ? f1 : f0;
+                int4* fout = (L & 1) ? f0 : f1;
+                launch_enum_preorder(t->pool, fin, L, fout, pts, segof, t->d_hdr, (int)cap, std::max(n, 1), rec,
+                                     rec_cap, st);
+            }
         }
     }
-    CHK(sync_hdr(t));
-    if (t->h_hdr->rec_count != total_live || (t->h_hdr->overflow & 32)) {
-        char msg[160];
-        snprintf(msg, sizeof msg, "enumerate found %d live points, counts say %lld", t->h_hdr->rec_count, total_live);
-        t->err = msg;
-        return IKD_ERR_INTERNAL;
-    }
+    hipLaunchKernelGGL(k_enum_check, dim3(1), dim3(1), 0, st, t->d_hdr, n);
     HIPCHK(hipGetLastError());
     *out_pts = pts;
     *out_seg = segof;
@@ -824,6 +953,7 @@ int rebuild_listed(ikd_tree_t t, const std::vector<int4>& hinfo, const std::vect
     int min_depth = INT_MAX;
     std::vector<int4> segs;
     std::vector<int> segd;
+    std::vector<int> nodes;
     long long total_nodes = 0, total_live = 0;
     int maxlen = 0;
     bool root_rebuilt = false;
@@ -846,6 +976,7 @@ int rebuild_listed(ikd_tree_t t, const std::vector<int4>& hinfo, const std::vect
         segd.push_back(depth);
         roots.push_back(u);
         lives.push_back(live);
+        nodes.push_back(hbefore[i]);
         min_depth = std::min(min_depth, depth);
         total_nodes += hbefore[i];
         total_live += live;
@@ -855,7 +986,7 @@ int rebuild_listed(ikd_tree_t t, const std::vector<int4>& hinfo, const std::vect
     if (!roots.empty()) {
         float4* pts;
         int* segof;
-        CHK(enumerate_live(t, roots, lives, total_nodes, total_live, min_depth, true, &pts, &segof));
+        CHK(enumerate_live(t, roots, lives, total_nodes, total_live, min_depth, true, &pts, &segof, &nodes));
         int4* dseg;
         int* dsegd;
         CHK(ensure(t, B_ISEG, (long long)segs.size(), &dseg));
@@ -876,22 +1007,32 @@ int rebuild_listed(ikd_tree_t t, const std::vector<int4>& hinfo, const std::vect
 int rebuild_tops(ikd_tree_t t) {
     hipStream_t st = t->st;
     for (int iter = 0; iter < 8; ++iter) {
-        CHK(refresh(t));
+        // the header, the root summary and the first subtrees to rebuild in one wait
+        // (the refit's list; after a pass of rebuilds the list is empty)
+        const int* top = iter == 0 ? reinterpret_cast<const int*>(t->b[B_TOP].p) : nullptr;
+        hipLaunchKernelGGL(k_root_info, dim3(1), dim3(1), 0, st, t->pool, t->d_root);
+        if (top) hipLaunchKernelGGL(k_top_info, dim3(1), dim3(256), 0, st, t->pool, top, t->d_status);
+        CHK(read_status(t, top ? sizeof(DevStatus) : STATUS_SHORT));
         int ntop = t->h_hdr->top_count;
         CHK(check_count(t, "rebuild: subtrees", ntop, 0, t->pool_used));
         if (ntop == 0) return IKD_OK;
-        int* top;
-        int4* info;
-        int* before;
-        CHK(ensure(t, B_TOP, 2 * ntop, &top));
-        CHK(ensure(t, B_INFO, ntop, &info));
-        CHK(ensure(t, B_SEEDS, ntop, &before));
-        hipLaunchKernelGGL(k_node_info, dim3(nblk(ntop)), dim3(BS), 0, st, t->pool, top, ntop, info, before);
         std::vector<int4> hinfo(ntop);
         std::vector<int> hbefore(ntop);
-        HIPCHK(hipMemcpyAsync(hinfo.data(), info, ntop * sizeof(int4), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(hbefore.data(), before, ntop * sizeof(int), hipMemcpyDeviceToHost, st));
-        HIPCHK(wait_stream(t));
+        if (top && ntop <= TOPINFO_CAP) {
+            std::memcpy(hinfo.data(), t->h_status->top_info, ntop * sizeof(int4));
+            std::memcpy(hbefore.data(), t->h_status->top_before, ntop * sizeof(int));
+        } else {
+            int* topb;
+            int4* info;
+            int* before;
+            CHK(ensure(t, B_TOP, 2 * ntop, &topb));
+            CHK(ensure(t, B_INFO, ntop, &info));
+            CHK(ensure(t, B_SEEDS, ntop, &before));
+            hipLaunchKernelGGL(k_node_info, dim3(nblk(ntop)), dim3(BS), 0, st, t->pool, topb, ntop, info, before);
+            HIPCHK(hipMemcpyAsync(hinfo.data(), info, ntop * sizeof(int4), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(hbefore.data(), before, ntop * sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPCHK(wait_stream(t));
+        }
         CHK(rebuild_listed(t, hinfo, hbefore));
         t->rebuilds += ntop;
         // The refit already handed every ancestor the post-rebuild counts, so
@@ -1412,14 +1553,16 @@ int ikd_create(float delete_param, float balance_param, float box_length, ikd_tr
     }
     t->mp = library_pool(dev);
     if (hipStreamCreateWithFlags(&t->st, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc(&t->d_hdr, sizeof(DevHdr)) != hipSuccess ||
-        hipHostMalloc(&t->h_hdr, sizeof(DevHdr), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc(&t->d_root, sizeof(RootInfo)) != hipSuccess ||
-        hipHostMalloc(&t->h_root, sizeof(RootInfo), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc(&t->d_status, sizeof(DevStatus)) != hipSuccess ||
+        hipHostMalloc(&t->h_status, sizeof(DevStatus), hipHostMallocDefault) != hipSuccess ||
         hipEventCreateWithFlags(&t->ev, hipEventDisableTiming) != hipSuccess) {
         ikd_destroy(t);
         return IKD_ERR_HIP;
     }
+    t->d_hdr = &t->d_status->hdr;
+    t->h_hdr = &t->h_status->hdr;
+    t->d_root = &t->d_status->root;
+    t->h_root = &t->h_status->root;
     // a zero header first: the pool's initialisation keeps only the record count
     launch_fill_i32(reinterpret_cast<int*>(t->d_hdr), 0, sizeof(DevHdr) / sizeof(int), t->st);
     int rc = reset_pool(t);
@@ -1446,10 +1589,8 @@ int ikd_destroy(ikd_tree_t t) {
         hipStreamSynchronize(t->st);
     }
     stage_release(t);
-    if (t->d_hdr) hipFree(t->d_hdr);
-    if (t->h_hdr) hipHostFree(t->h_hdr);
-    if (t->d_root) hipFree(t->d_root);
-    if (t->h_root) hipHostFree(t->h_root);
+    if (t->d_status) hipFree(t->d_status);
+    if (t->h_status) hipHostFree(t->h_status);
     if (t->ev) hipEventDestroy(t->ev);
     if (t->st) hipStreamDestroy(t->st);
     delete t;

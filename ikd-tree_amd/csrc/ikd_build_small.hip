@@ -408,14 +408,26 @@ __global__ __launch_bounds__(SB) void k_build_small(const float4* __restrict__ p
 // One lane builds one segment of at most MICRO_MAX points: the median tree of
 // k_build_small (axis of the longest extent, ties to the lower axis; node =
 // the element of rank (n-1)>>1 in (coordinate, position) order; both sides
-// keep the segment's order), in registers.  Pairs are taken from the
+// keep the segment's order).  The lane keeps the work order as 4-bit
+// positions packed in one register and its pending ranges in LDS (nothing
+// indexed at run time lives in private memory); pairs are taken from the
 // segment's range one internal node at a time.  Larger segments (or every
 // segment when micro is off) go to the size-class lists.
-__global__ void k_build_micro(const float4* __restrict__ pts, const int4* __restrict__ seg,
-                              const int* __restrict__ seg_depth, const int* __restrict__ pair_scan, int S,
-                              const int* __restrict__ count_dev, int micro, Pool pool, DevHdr* hdr,
-                              int* __restrict__ tiny_list, int* __restrict__ small_list) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+constexpr int MICRO_BS = 256;
+constexpr int MICRO_STACK = 6;  // pending ranges: at most one per level below the current one
+__device__ __forceinline__ float micro_coord(const float4& q, int axis) {
+    return axis == 0 ? q.x : (axis == 1 ? q.y : q.z);
+}
+__global__ __launch_bounds__(MICRO_BS) void k_build_micro(const float4* __restrict__ pts, const int4* __restrict__ seg,
+                                                          const int* __restrict__ seg_depth,
+                                                          const int* __restrict__ pair_scan, int S,
+                                                          const int* __restrict__ count_dev, int micro, Pool pool,
+                                                          DevHdr* hdr, int* __restrict__ tiny_list,
+                                                          int* __restrict__ small_list) {
+    // per lane: (lo | hi << 4 | level << 8), slot node, slot parent
+    __shared__ int stk_r[MICRO_STACK][MICRO_BS], stk_node[MICRO_STACK][MICRO_BS], stk_par[MICRO_STACK][MICRO_BS];
+    const int tid = threadIdx.x;
+    const int g = blockIdx.x * blockDim.x + tid;
     if (g >= S || (count_dev && g >= *count_dev)) return;
     const int4 sg = seg[g];
     const int m = sg.y - sg.x + 1;
@@ -425,32 +437,26 @@ __global__ void k_build_micro(const float4* __restrict__ pts, const int4* __rest
         else small_list[atomicAdd(&hdr->small_lists[1], 1)] = g;
         return;
     }
-    float4 p[MICRO_MAX];
-    int idx[MICRO_MAX];  // positions, each work range in segment order
-#pragma unroll
-    for (int i = 0; i < MICRO_MAX; ++i) {
-        if (i < m) p[i] = pts[sg.x + i];
-        idx[i] = i;
-    }
+    const float4* P = pts + sg.x;
+    uint64_t perm = 0x76543210ull;  // nibble i: the position at work-order slot i
     int next_pair = hdr->pair_used + pair_scan[g];
-    // work ranges [lo, hi) of idx with their slot node, parent and depth
-    int wlo[MICRO_MAX], whi[MICRO_MAX], wnode[MICRO_MAX], wpar[MICRO_MAX], wdep[MICRO_MAX];
+    const int depth0 = seg_depth[g];
     int nw = 1;
-    wlo[0] = 0;
-    whi[0] = m;
-    wnode[0] = sg.z;
-    wpar[0] = sg.w;
-    wdep[0] = seg_depth[g];
-    int deepest = 0;
+    stk_r[0][tid] = m << 4;
+    stk_node[0][tid] = sg.z;
+    stk_par[0][tid] = sg.w;
+    int deepest = depth0;
     while (nw > 0) {
         --nw;
-        const int lo = wlo[nw], hi = whi[nw], node = wnode[nw], parent = wpar[nw], depth = wdep[nw];
+        const int rr = stk_r[nw][tid];
+        const int lo = rr & 15, hi = (rr >> 4) & 15, depth = depth0 + (rr >> 8);
+        const int node = stk_node[nw][tid], parent = stk_par[nw][tid];
         const int cnt = hi - lo;
         // bounds in key order, as the sorted lists give them (-0 below +0)
-        const float4 q0 = p[idx[lo]];
+        const float4 q0 = P[(perm >> (4 * lo)) & 15];
         float mn[3] = {q0.x, q0.y, q0.z}, mx[3] = {q0.x, q0.y, q0.z};
         for (int i = lo + 1; i < hi; ++i) {
-            const float4 q = p[idx[i]];
+            const float4 q = P[(perm >> (4 * i)) & 15];
             const float c[3] = {q.x, q.y, q.z};
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -464,42 +470,39 @@ __global__ void k_build_micro(const float4* __restrict__ pts, const int4* __rest
         if (ext[2] > ext[axis]) axis = 2;
         // the element of rank (cnt-1)>>1 in (coordinate, position) order
         const int want = (cnt - 1) >> 1;
-        int piv = -1;
+        int pos = -1;
+        uint32_t kp = 0;
         for (int i = lo; i < hi; ++i) {
-            const float4 a = p[idx[i]];
-            const uint32_t ka = float_key(axis == 0 ? a.x : (axis == 1 ? a.y : a.z));
+            const int pi = (int)((perm >> (4 * i)) & 15);
+            const uint32_t ka = float_key(micro_coord(P[pi], axis));
             int rank = 0;
             for (int j = lo; j < hi; ++j) {
-                const float4 b = p[idx[j]];
-                const uint32_t kb = float_key(axis == 0 ? b.x : (axis == 1 ? b.y : b.z));
-                rank += (kb < ka || (kb == ka && idx[j] < idx[i])) ? 1 : 0;
+                const int pj = (int)((perm >> (4 * j)) & 15);
+                const uint32_t kb = float_key(micro_coord(P[pj], axis));
+                rank += (kb < ka || (kb == ka && pj < pi)) ? 1 : 0;
             }
-            if (rank == want) piv = i;
+            if (rank == want) {
+                pos = pi;
+                kp = ka;
+            }
         }
-        const int pos = idx[piv];
-        const float pk = (axis == 0 ? p[pos].x : (axis == 1 ? p[pos].y : p[pos].z));
-        const uint32_t kp = float_key(pk);
-        // stable split: ranks below the pivot's first, then the ones above
-        int tmp[MICRO_MAX];
+        // stable split: ranks below the pivot's first, then the pivot, then the ones above
+        uint64_t left = 0, right = 0;
         int nl = 0, nr = 0;
         for (int i = lo; i < hi; ++i) {
-            if (i == piv) continue;
-            const float4 b = p[idx[i]];
-            const uint32_t kb = float_key(axis == 0 ? b.x : (axis == 1 ? b.y : b.z));
-            if (kb < kp || (kb == kp && idx[i] < pos)) tmp[nl++] = idx[i];
+            const int pi = (int)((perm >> (4 * i)) & 15);
+            if (pi == pos) continue;
+            const uint32_t kb = float_key(micro_coord(P[pi], axis));
+            if (kb < kp || (kb == kp && pi < pos)) left |= (uint64_t)pi << (4 * nl++);
+            else right |= (uint64_t)pi << (4 * nr++);
         }
-        for (int i = lo; i < hi; ++i) {
-            if (i == piv) continue;
-            const float4 b = p[idx[i]];
-            const uint32_t kb = float_key(axis == 0 ? b.x : (axis == 1 ? b.y : b.z));
-            if (!(kb < kp || (kb == kp && idx[i] < pos))) tmp[nl + nr++] = idx[i];
-        }
-        for (int i = 0; i < nl + nr; ++i) idx[lo + (i < nl ? i : i + 1)] = tmp[i];
-        idx[lo + nl] = pos;
+        const uint64_t run = left | ((uint64_t)pos << (4 * nl)) | (right << (4 * (nl + 1)));
+        const uint64_t mask = ((1ull << (4 * cnt)) - 1) << (4 * lo);
+        perm = (perm & ~mask) | (run << (4 * lo));
         const int hl = nl > 0, hr = nr > 0;
         const int pair = (hl || hr) ? next_pair++ : 0;
         const uint32_t link = ((uint32_t)pair << LK_PAIR_SHIFT) | (hl ? LK_HASL : 0u) | (hr ? LK_HASR : 0u);
-        const float4 P4 = p[pos];
+        const float4 P4 = P[pos];
         *reinterpret_cast<float4*>(pool.hot[node].f) = make_float4(P4.x, P4.y, P4.z, bitsf(link));
         pool.ids[node] = fbits(P4.w);
         pool.par[node] = parent;
@@ -508,20 +511,17 @@ __global__ void k_build_micro(const float4* __restrict__ pts, const int4* __rest
         const float box[6] = {mn[0], mn[1], mn[2], mx[0], mx[1], mx[2]};
         write_child_box_s(pool.hot, parent, node & 1, box);
         deepest = depth > deepest ? depth : deepest;
+        const int lv = (rr >> 8) + 1;
         if (hr) {
-            wlo[nw] = lo + nl + 1;
-            whi[nw] = hi;
-            wnode[nw] = 2 * pair + 1;
-            wpar[nw] = node;
-            wdep[nw] = depth + 1;
+            stk_r[nw][tid] = (lo + nl + 1) | (hi << 4) | (lv << 8);
+            stk_node[nw][tid] = 2 * pair + 1;
+            stk_par[nw][tid] = node;
             ++nw;
         }
         if (hl) {
-            wlo[nw] = lo;
-            whi[nw] = lo + nl;
-            wnode[nw] = 2 * pair;
-            wpar[nw] = node;
-            wdep[nw] = depth + 1;
+            stk_r[nw][tid] = lo | ((lo + nl) << 4) | (lv << 8);
+            stk_node[nw][tid] = 2 * pair;
+            stk_par[nw][tid] = node;
             ++nw;
         }
     }
@@ -541,7 +541,7 @@ void launch_build_small(const float4* pts, const int4* seg, const int* seg_depth
     // the lane build follows the position tie rule only: presorted or
     // original-order inputs go to the workgroup builds
     const int micro = (!orig && !pre.inv0) ? 1 : 0;
-    hipLaunchKernelGGL(k_build_micro, dim3(blocks_for(S, 256)), dim3(256), 0, s, pts, seg, seg_depth, pair_scan, S,
+    hipLaunchKernelGGL(k_build_micro, dim3(blocks_for(S, MICRO_BS)), dim3(MICRO_BS), 0, s, pts, seg, seg_depth, pair_scan, S,
                        count_dev, micro, pool, hdr, lists, lists + S);
     // each workgroup loops over its shape's list; the grids only need to be
     // wide enough to fill the device when a list is long

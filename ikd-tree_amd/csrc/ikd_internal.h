@@ -124,6 +124,10 @@ void launch_enum_preorder(Pool pool, const int4* front, int lvl, int4* front_nex
 void launch_enum_preorder_block(Pool pool, int4* f0, int4* f1, int n_roots, float4* out_pts, int* out_seg,
                                 DevHdr* hdr, int bound, int out_cap, int max_levels, float4* rec, int rec_cap,
                                 hipStream_t s);
+// groups: (first root, roots, frontier offset, frontier entries) per workgroup
+void launch_enum_groups(Pool pool, const int4* roots, const int4* groups, int n_groups, int4* f0, int4* f1,
+                        float4* out_pts, int* out_seg, DevHdr* hdr, int out_cap, int max_levels, float4* rec,
+                        int rec_cap, hipStream_t s);
 
 // ---------------------------------------------------------------- insert
 void launch_insert_descend(Pool pool, const float4* pts, int n, unsigned long long* keys, uint32_t* vals,

@@ -43,6 +43,15 @@ struct SlotBufs {
     uint32_t* out_ids = nullptr;
     int32_t* out_cnt = nullptr;
     long long cap_pts = 0, cap_res = 0;
+    // pinned host mirrors: the scan goes up and the results come down by
+    // DMA without stopping the host (pageable copies wait for the stream)
+    float* h_xyz_in = nullptr;
+    float* h_xyz = nullptr;
+    float* h_d2 = nullptr;
+    uint32_t* h_ids = nullptr;
+    int32_t* h_cnt = nullptr;
+    long long hcap_pts = 0, hcap_res = 0;
+    hipEvent_t ev = nullptr;  // behind the result copies
 };
 
 // device bytes a request needs beyond its map: the scan, the kNN results and
@@ -81,7 +90,50 @@ struct ikd_server_s {
         *p = nullptr;
         return hipMallocAsync((void**)p, (size_t)count * elem, st) == hipSuccess;
     }
+    template <class T>
+    static bool regrow_host(T** p, long long count, size_t elem) {
+        if (*p) hipHostFree(*p);
+        *p = nullptr;
+        return hipHostMalloc((void**)p, (size_t)count * elem, hipHostMallocDefault) == hipSuccess;
+    }
+    // The slot's previous request ended with a host wait, so no copy still
+    // uses a pinned buffer freed here.
+    int grow_host(SlotBufs& b, long long n, int k, ikd_request_t* r) {
+        if (!b.ev && hipEventCreateWithFlags(&b.ev, hipEventDisableTiming) != hipSuccess) {
+            r->status = IKD_ERR_HIP;
+            return IKD_ERR_HIP;
+        }
+        if (n > b.hcap_pts) {
+            long long c = std::max(n, b.hcap_pts + b.hcap_pts / 2);
+            if (!regrow_host(&b.h_xyz_in, c, 12) || !regrow_host(&b.h_cnt, c, 4)) {
+                b.hcap_pts = 0;
+                r->status = IKD_ERR_NOMEM;
+                return IKD_ERR_NOMEM;
+            }
+            b.hcap_pts = c;
+        }
+        long long res = n * std::max(k, 1);
+        if (res > b.hcap_res) {
+            long long c = std::max(res, b.hcap_res + b.hcap_res / 2);
+            if (!regrow_host(&b.h_xyz, c, 12) || !regrow_host(&b.h_d2, c, 4) || !regrow_host(&b.h_ids, c, 4)) {
+                b.hcap_res = 0;
+                r->status = IKD_ERR_NOMEM;
+                return IKD_ERR_NOMEM;
+            }
+            b.hcap_res = c;
+        }
+        return IKD_OK;
+    }
+    static void release(SlotBufs& b) {
+        for (void* p : {(void*)b.xyz, (void*)b.out_xyz, (void*)b.out_d2, (void*)b.out_ids, (void*)b.out_cnt})
+            if (p) hipFree(p);
+        for (void* p : {(void*)b.h_xyz_in, (void*)b.h_xyz, (void*)b.h_d2, (void*)b.h_ids, (void*)b.h_cnt})
+            if (p) hipHostFree(p);
+        if (b.ev) hipEventDestroy(b.ev);
+        b = SlotBufs{};
+    }
     int grow(SlotBufs& b, long long n, int k, ikd_request_t* r, hipStream_t st) {
+        if (grow_host(b, n, k, r) != IKD_OK) return r->status;
         if (n > b.cap_pts) {
             long long c = std::max(n, b.cap_pts + b.cap_pts / 2);
             if (!regrow(&b.xyz, c, 12, st) || !regrow(&b.out_cnt, c, 4, st)) {
@@ -115,21 +167,24 @@ struct ikd_server_s {
         if (n > 0) {
             rc = grow(b, n, r->k, r, st);
             if (rc != IKD_OK) return rc;
-            if (hipMemcpyAsync(b.xyz, r->xyz, n * 12, hipMemcpyHostToDevice, st) != hipSuccess) return IKD_ERR_HIP;
+            std::memcpy(b.h_xyz_in, r->xyz, n * 12);
+            if (hipMemcpyAsync(b.xyz, b.h_xyz_in, n * 12, hipMemcpyHostToDevice, st) != hipSuccess) return IKD_ERR_HIP;
         }
-        if (r->k > 0 && n > 0) {
+        const bool knn = r->k > 0 && n > 0;
+        const long long res = knn ? n * r->k : 0;
+        if (knn) {
             rc = ikd_nearest_search_device(t, b.xyz, n, r->k, r->max_dist, b.out_xyz, b.out_d2, b.out_ids,
                                            b.out_cnt);
             if (rc != IKD_OK) return rc;
-            const long long res = n * r->k;
-            if (r->out_d2 && hipMemcpyAsync(r->out_d2, b.out_d2, res * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+            if (r->out_d2 && hipMemcpyAsync(b.h_d2, b.out_d2, res * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
                 return IKD_ERR_HIP;
-            if (r->out_ids && hipMemcpyAsync(r->out_ids, b.out_ids, res * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+            if (r->out_ids && hipMemcpyAsync(b.h_ids, b.out_ids, res * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
                 return IKD_ERR_HIP;
-            if (r->out_xyz && hipMemcpyAsync(r->out_xyz, b.out_xyz, res * 12, hipMemcpyDeviceToHost, st) != hipSuccess)
+            if (r->out_xyz && hipMemcpyAsync(b.h_xyz, b.out_xyz, res * 12, hipMemcpyDeviceToHost, st) != hipSuccess)
                 return IKD_ERR_HIP;
-            if (r->out_counts && hipMemcpyAsync(r->out_counts, b.out_cnt, n * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+            if (r->out_counts && hipMemcpyAsync(b.h_cnt, b.out_cnt, n * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
                 return IKD_ERR_HIP;
+            if (hipEventRecord(b.ev, st) != hipSuccess) return IKD_ERR_HIP;
         }
         if (r->downsample >= 0 && n > 0) {
             int32_t added = 0;
@@ -138,6 +193,14 @@ struct ikd_server_s {
             r->added = added;
         }
         r->t_insert_done_ns = now_ns();
+        if (knn) {
+            // the insert's own waits have usually passed the result copies already
+            if (hipEventSynchronize(b.ev) != hipSuccess) return IKD_ERR_HIP;
+            if (r->out_d2) std::memcpy(r->out_d2, b.h_d2, res * 4);
+            if (r->out_ids) std::memcpy(r->out_ids, b.h_ids, res * 4);
+            if (r->out_xyz) std::memcpy(r->out_xyz, b.h_xyz, res * 12);
+            if (r->out_counts) std::memcpy(r->out_counts, b.h_cnt, n * 4);
+        }
         if (r->n_trim > 0) {
             int32_t del = 0;
             rc = ikd_delete_point_boxes(t, r->trim_boxes, r->n_trim, &del);
@@ -308,13 +371,7 @@ int ikd_server_destroy(ikd_server_t s) {
     for (auto& w : s->workers) w.join();
     // the workers' last requests ended with a wait on their streams
     hipDeviceSynchronize();
-    for (auto& b : s->bufs) {
-        if (b.xyz) hipFree(b.xyz);
-        if (b.out_xyz) hipFree(b.out_xyz);
-        if (b.out_d2) hipFree(b.out_d2);
-        if (b.out_ids) hipFree(b.out_ids);
-        if (b.out_cnt) hipFree(b.out_cnt);
-    }
+    for (auto& b : s->bufs) ikd_server_s::release(b);
     delete s;
     return IKD_OK;
 }

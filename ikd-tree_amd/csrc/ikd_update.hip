@@ -501,6 +501,62 @@ void launch_enum_preorder_block(Pool pool, int4* f0, int4* f1, int n_roots, floa
                        bound, out_cap, max_levels, rec, rec_cap);
 }
 
+// The same walk for many small subtrees in one launch: workgroup g walks the
+// roots groups[g].x .. groups[g].x + groups[g].y - 1 (entries of `roots`)
+// level by level in its own frontier area, entries [groups[g].z,
+// groups[g].z + groups[g].w) of f0 and f1 (the group's node count, which
+// bounds every level).  Output positions come with the root entries, so the
+// groups never meet.
+__global__ __launch_bounds__(256) void k_enum_groups(Pool pool, const int4* __restrict__ roots,
+                                                     const int4* __restrict__ groups, int4* __restrict__ f0,
+                                                     int4* __restrict__ f1, float4* __restrict__ out_pts,
+                                                     int* __restrict__ out_seg, DevHdr* hdr, int out_cap,
+                                                     int max_levels, float4* __restrict__ rec, int rec_cap) {
+    __shared__ int next_count, written;
+    const int4 g = groups[blockIdx.x];
+    const int bound = g.w;
+    int4* fin = f0 + g.z;
+    int4* fout = f1 + g.z;
+    if (threadIdx.x == 0) written = 0;
+    int nf = min(g.y, bound);
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) fin[i] = roots[g.x + i];
+    if (g.y > bound && threadIdx.x == 0) atomicOr(&hdr->overflow, 32);
+    __syncthreads();
+    for (int lvl = 0; lvl < max_levels && nf > 0; ++lvl) {
+        if (threadIdx.x == 0) next_count = 0;
+        __syncthreads();
+        int mine = 0;
+        for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+            EnumStep st;
+            mine += enum_visit(pool, fin[i], out_pts, out_seg, hdr, out_cap, rec, rec_cap, st);
+            for (int c = 0; c < st.pushes; ++c) {
+                const int pos = atomicAdd(&next_count, 1);
+                if (pos < bound) fout[pos] = st.child[c];
+                else atomicOr(&hdr->overflow, 32);
+            }
+        }
+        if (mine) atomicAdd(&written, mine);
+        __syncthreads();
+        nf = min(next_count, bound);
+        int4* tmp = fin;
+        fin = fout;
+        fout = tmp;
+        __syncthreads();  // every thread has read next_count before it is reset
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(&hdr->rec_count, written);
+        if (nf > 0) atomicOr(&hdr->overflow, 32);  // deeper than the levels allowed
+    }
+}
+
+void launch_enum_groups(Pool pool, const int4* roots, const int4* groups, int n_groups, int4* f0, int4* f1,
+                        float4* out_pts, int* out_seg, DevHdr* hdr, int out_cap, int max_levels, float4* rec,
+                        int rec_cap, hipStream_t s) {
+    if (n_groups <= 0) return;
+    hipLaunchKernelGGL(k_enum_groups, dim3(n_groups), dim3(256), 0, s, pool, roots, groups, f0, f1, out_pts, out_seg,
+                       hdr, out_cap, max_levels, rec, rec_cap);
+}
+
 // ================================================================== insert
 // Marks node u as wholly deleted, as Push_Down writes a child of a
 // tree_deleted parent (cpp:1125-1131).
