@@ -41,12 +41,13 @@ EXPORTS = (
     "ikd_add_points", "ikd_add_points_device", "ikd_delete_points", "ikd_delete_point_boxes",
     "ikd_add_point_boxes", "ikd_size",
     "ikd_validnum", "ikd_tree_range", "ikd_root_alpha", "ikd_flatten", "ikd_synchronize", "ikd_stream",
-    "ikd_pool_stats", "ikd_get_counters", "ikd_acquire_removed_points", "ikd_set_tuning", "ikd_dump_nodes", "ikd_last_error",
+    "ikd_pool_stats", "ikd_get_counters", "ikd_device_bytes", "ikd_acquire_removed_points", "ikd_set_tuning", "ikd_dump_nodes", "ikd_last_error",
     "ikd_batch_create", "ikd_batch_destroy", "ikd_batch_nearest_search_device", "ikd_batch_synchronize",
     "ikd_batch_stream", "ikd_batch_set_tuning", "ikd_batch_last_error", "ikd_batch_radius_search_device",
     "ikd_batch_nearest_search_noalloc",
     "ikd_now_ns", "ikd_server_create", "ikd_server_add_map", "ikd_server_submit", "ikd_server_wait",
     "ikd_server_pending", "ikd_server_destroy", "ikd_server_set_maintenance", "ikd_server_stats",
+    "ikd_server_memory",
     "ikd_rebuild_deferred", "ikd_list_payload_ids", "ikd_relabel_payload_ids",
 )
 IKD_TUNE_QUERY_ORDER = 1
@@ -121,6 +122,7 @@ def load_library(path: str | None = None):
         "ikd_stream": [vp, ctypes.POINTER(vp)],
         "ikd_pool_stats": [vp, vp, vp, vp, vp],
         "ikd_get_counters": [vp, vp],
+        "ikd_device_bytes": [vp, ctypes.POINTER(ctypes.c_int64)],
         "ikd_acquire_removed_points": [vp, vp, vp, i64, vp],
         "ikd_set_tuning": [vp, i32, i32],
         "ikd_dump_nodes": [vp, vp, i64, vp],
@@ -372,6 +374,12 @@ class KD_TREE:
         c = Counters()
         self._check(self._lib.ikd_get_counters(self._h, ctypes.byref(c)))
         return c.as_dict()
+
+    def device_bytes(self) -> int:
+        """Device memory the tree holds (node pool, scratch, records)."""
+        v = ctypes.c_int64(0)
+        self._check(self._lib.ikd_device_bytes(self._h, ctypes.byref(v)))
+        return int(v.value)
 
     def set_tuning(self, key: int, value: int):
         self._check(self._lib.ikd_set_tuning(self._h, key, value))

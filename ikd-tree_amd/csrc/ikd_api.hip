@@ -2190,6 +2190,17 @@ int ikd_get_counters(ikd_tree_t t, ikd_counters_t* out) {
     return IKD_OK;
 }
 
+int ikd_device_bytes(ikd_tree_t t, int64_t* out_bytes) {
+    CHECK_T();
+    if (!out_bytes) return IKD_ERR_ARG;
+    int64_t b = (int64_t)sizeof(DevStatus);
+    b += t->cap * (int64_t)(sizeof(NodeHot) + sizeof(uint32_t) + sizeof(int32_t) + sizeof(int4) + sizeof(uint32_t));
+    for (const DevBuf& d : t->b) b += (int64_t)d.bytes;
+    b += t->removed_cap * (int64_t)sizeof(float4);
+    *out_bytes = b;
+    return IKD_OK;
+}
+
 const char* ikd_last_error(ikd_tree_t t) {
     if (!t) return "null tree handle";
     return t->err.c_str();

@@ -68,6 +68,8 @@ struct ikd_server_s {
     std::condition_variable cv_work, cv_done;
     std::vector<ikd_tree_t> maps;
     std::vector<char> busy;
+    std::vector<int64_t> map_bytes;  // each map's device bytes, as of its last request or maintenance
+    int64_t resident = 0;            // their sum
     std::deque<ikd_request_t*> pending;
     std::deque<ikd_request_t*> done;
     std::vector<std::thread> workers;
@@ -222,8 +224,10 @@ struct ikd_server_s {
                     ikd_request_t* c = *it;
                     if (busy[c->map]) continue;
                     const int64_t nb = request_bytes(c);
-                    // admit when it fits; an idle server always admits its oldest runnable request
-                    if (in_flight_bytes + nb > budget && in_flight > 0) continue;
+                    // admit when the resident maps, the running requests and
+                    // this one fit; an idle server always admits its oldest
+                    // runnable request
+                    if (resident + in_flight_bytes + nb > budget && in_flight > 0) continue;
                     r = c;
                     need = nb;
                     pending.erase(it);
@@ -274,6 +278,13 @@ struct ikd_server_s {
                 maintenance_runs += nreb;
                 if (mrc != IKD_OK || nreb == 0 || maintain_points < 0) break;
             }
+            // the map may have grown (pool, scratch) over the request and its maintenance
+            int64_t mb = 0;
+            lk.unlock();
+            ikd_device_bytes(tree, &mb);
+            lk.lock();
+            resident += mb - map_bytes[map];
+            map_bytes[map] = mb;
             busy[map] = 0;
             in_flight_bytes -= need;
             --in_flight;
@@ -299,9 +310,14 @@ int ikd_server_create(int slots, int64_t hbm_budget_bytes, ikd_server_t* out) {
 
 int ikd_server_add_map(ikd_server_t s, ikd_tree_t tree, int32_t* out_map) {
     if (!s || !tree || !out_map) return IKD_ERR_ARG;
+    int64_t mb = 0;
+    const int rc = ikd_device_bytes(tree, &mb);
+    if (rc != IKD_OK) return rc;
     std::lock_guard<std::mutex> g(s->mu);
     s->maps.push_back(tree);
     s->busy.push_back(0);
+    s->map_bytes.push_back(mb);
+    s->resident += mb;
     *out_map = (int32_t)(s->maps.size() - 1);
     return IKD_OK;
 }
@@ -350,6 +366,14 @@ int ikd_server_stats(ikd_server_t s, int64_t* out_rebuilds) {
     if (!s) return IKD_ERR_ARG;
     std::lock_guard<std::mutex> g(s->mu);
     if (out_rebuilds) *out_rebuilds = s->maintenance_runs;
+    return IKD_OK;
+}
+
+int ikd_server_memory(ikd_server_t s, int64_t* out_resident, int64_t* out_in_flight_bytes) {
+    if (!s) return IKD_ERR_ARG;
+    std::lock_guard<std::mutex> g(s->mu);
+    if (out_resident) *out_resident = s->resident;
+    if (out_in_flight_bytes) *out_in_flight_bytes = s->in_flight_bytes;
     return IKD_OK;
 }
 

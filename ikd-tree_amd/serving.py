@@ -51,6 +51,7 @@ def _lib():
             "ikd_server_destroy": [vp],
             "ikd_server_set_maintenance": [vp, i64],
             "ikd_server_stats": [vp, ctypes.POINTER(i64)],
+            "ikd_server_memory": [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)],
         }.items():
             fn = getattr(lib, name)
             fn.argtypes = args
@@ -123,6 +124,20 @@ class Server:
         v = ctypes.c_int64(0)
         self._lib.ikd_server_stats(self._h, ctypes.byref(v))
         return int(v.value)
+
+    def memory(self):
+        """(device bytes of the registered maps, estimate of the running requests)."""
+        res, fl = ctypes.c_int64(0), ctypes.c_int64(0)
+        rc = self._lib.ikd_server_memory(self._h, ctypes.byref(res), ctypes.byref(fl))
+        if rc != 0:
+            raise _pkg.IkdError(rc, "memory failed")
+        return int(res.value), int(fl.value)
+
+    @staticmethod
+    def request_bytes(n: int, k: int, downsample: int) -> int:
+        """The server's device-memory estimate of one request (scan, kNN
+        results, insert scratch)."""
+        return n * 12 + n * max(k, 0) * 20 + n * 4 + (n * 160 if downsample >= 0 else 0) + (1 << 20)
 
     def add_map(self, tree, defer_rebuild_points: int | None = None) -> int:
         """Register a map; with defer_rebuild_points its large rebuilds are

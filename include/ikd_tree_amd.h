@@ -166,6 +166,11 @@ int ikd_pool_stats(ikd_tree_t t, int64_t* used_nodes, int64_t* capacity_nodes, i
 int ikd_acquire_removed_points(ikd_tree_t t, float* out_xyz, uint32_t* out_ids, int64_t capacity, int64_t* out_n);
 /* Diagnostics (no counterpart in the reference): the tree's event counts. */
 int ikd_get_counters(ikd_tree_t t, ikd_counters_t* out);
+/* Device memory the tree holds: node pool, scratch, removed-point record and
+ * status block, in bytes (a host-side figure; no device call).  No
+ * counterpart in the reference, whose nodes live in host memory; used by the
+ * server's HBM admission. */
+int ikd_device_bytes(ikd_tree_t t, int64_t* out_bytes);
 
 /* Launch tuning (no counterpart in the reference; results never depend on it):
  *  IKD_TUNE_QUERY_ORDER 1 sorts batches of >= 4096 kNN queries into
@@ -345,6 +350,11 @@ int ikd_server_pending(ikd_server_t s, int64_t* out_pending, int64_t* out_in_fli
 int ikd_server_set_maintenance(ikd_server_t s, int64_t max_points);
 /* Deferred rebuilds the workers have run so far. */
 int ikd_server_stats(ikd_server_t s, int64_t* out_rebuilds);
+/* Device bytes the registered maps hold (as of their last request or
+ * maintenance step) and the estimate of the requests now running.  A request
+ * is admitted while resident + running + its own estimate fits the budget;
+ * with nothing running the oldest runnable request is always admitted. */
+int ikd_server_memory(ikd_server_t s, int64_t* out_resident, int64_t* out_in_flight_bytes);
 /* Finish every submitted request, then stop the workers. */
 int ikd_server_destroy(ikd_server_t s);
 

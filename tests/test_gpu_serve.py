@@ -102,3 +102,43 @@ def test_bad_requests_rejected(ikd, serving):
     with pytest.raises(ikd.IkdError):
         srv.submit(m, np.zeros((4, 3), np.float32), k=65)
     srv.close()
+
+
+def test_budget_counts_resident_maps(ikd, serving, scenes):
+    """The admission budget covers the maps' own device memory: with room for
+    the resident maps plus one request, requests for four different maps run
+    one at a time (each starts after the previous one finished); with a large
+    budget the same requests finish too.  The resident figure follows the maps'
+    growth and equals the sum of their device_bytes()."""
+    n_maps = 4
+    sc = [scenes.MappingScene(1105 + m) for m in range(n_maps)]
+    scans = [[s.scan(f)[:6000] for f in range(4)] for s in sc]
+    for budget_kind in ("tight", "large"):
+        trees = []
+        for m in range(n_maps):
+            t = ikd.KD_TREE(0.5, 0.6, 0.5)
+            t.Build(scans[m][0])
+            trees.append(t)
+        probe = serving.Server(slots=1, hbm_budget_bytes=1 << 40)
+        for t in trees:
+            probe.add_map(t)
+        resident0, _ = probe.memory()
+        probe.close()
+        assert resident0 == sum(t.device_bytes() for t in trees)
+        one = serving.Server.request_bytes(6000, 5, 1)
+        budget = resident0 + one + (1 << 20) if budget_kind == "tight" else 64 << 30
+        srv = serving.Server(slots=4, hbm_budget_bytes=budget)
+        ids = [srv.add_map(t) for t in trees]
+        tickets = [srv.submit(ids[m], scans[m][j], 5, 5.0, 1, tag=j) for j in range(1, 4) for m in range(n_maps)]
+        srv.drain()
+        resident, running = srv.memory()
+        srv.close()
+        assert all(t.status == 0 for t in tickets)
+        assert running == 0
+        assert resident == sum(t.device_bytes() for t in trees) >= resident0
+        if budget_kind == "tight":
+            order = sorted(tickets, key=lambda t: t.req.t_start_ns)
+            for a, b in zip(order, order[1:]):
+                assert b.req.t_start_ns >= a.req.t_done_ns, "two requests ran at once under a one-request budget"
+        for t in trees:
+            t.close()
