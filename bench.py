@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--frames", type=int, default=200, help="scans in the mapping-pipeline phase")
     ap.add_argument("--trees", type=int, default=64, help="trees in the multi-tree batching phase (0 skips it)")
     ap.add_argument("--robots", type=int, default=16, help="maps in the serving phase (0 skips it)")
+    ap.add_argument("--box-points", type=int, default=5_000_000,
+                    help="map points of the box/radius/trim phase (0 skips it)")
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="simulated arrival window of the serving phase")
     ap.add_argument("--serve-defer", type=int, default=1_000,
                     help="serving: subtrees above this many live points are rebuilt between requests")
@@ -168,7 +170,7 @@ def main():
 
     def tune(q):
         if args.no_tune:
-            return (args.lanes if args.lanes else -1, 0, 0), {}
+            return (args.lanes if args.lanes else -1, 0, 0, 0), {}
         best, ms = autotune.autotune_knn(tree, q, k, args.max_dist)
         return best, ms
 
@@ -177,6 +179,7 @@ def main():
         tree.set_tuning(ikd.IKD_TUNE_LANES_PER_QUERY, lanes)
         tree.set_tuning(ikd.IKD_TUNE_KNN_STACK, cfg[1])
         tree.set_tuning(ikd.IKD_TUNE_QUERY_ORDER, cfg[2])
+        tree.set_tuning(ikd.IKD_TUNE_KNN_STAGE, cfg[3])
         return (lanes,) + tuple(cfg[1:])
 
     def timed_launches(queries, outs, idx):
@@ -246,9 +249,9 @@ def main():
         "launches": len(timed_scans),
         "roofline_achieved_gbs": p_gbs,
         "roofline_frac": p_gbs / HBM_PEAK_GBS,
-        "kernel": knn_kernel_name(k, p_cfg[0]),
-        "tuned": {"lanes_per_query": p_cfg[0], "stack": p_cfg[1], "morton": p_cfg[2]},
-        "sweep_ms": {f"lanes{a}_stack{b}_order{c}": v for (a, b, c), v in p_sweep.items()},
+        "kernel": knn_kernel_name(k, p_cfg[0], p_cfg[3]),
+        "tuned": {"lanes_per_query": p_cfg[0], "stack": p_cfg[1], "morton": p_cfg[2], "lds_levels": p_cfg[3]},
+        "sweep_ms": {f"lanes{a}_stack{b}_order{c}_lds{d}": v for (a, b, c, d), v in p_sweep.items()},
     }
 
     progress("phase A done")
@@ -266,6 +269,10 @@ def main():
     serving = (serving_phase(ikd, scenes, args.robots, args.serve_seconds, args.serve_defer, args.serve_chunk,
                              args.serve_slots)
                if args.robots > 0 else None)
+
+    progress("phase D done")
+    # ================================================ phase E: box/radius/trims on a 5M map
+    box_radius = box_radius_phase(ikd, scenes, args.box_points, cpu=not args.no_cpu) if args.box_points > 0 else None
 
     result = {
         "metric": METRIC,
@@ -293,8 +300,8 @@ def main():
             "queries_per_step": nq_total / args.steps,
             "k": k,
             "max_dist_m": args.max_dist,
-            "tuned": {"lanes_per_query": b_cfg[0], "stack": b_cfg[1], "morton": b_cfg[2]},
-            "sweep_ms": {f"lanes{a}_stack{b}_order{c}": v for (a, b, c), v in b_sweep.items()},
+            "tuned": {"lanes_per_query": b_cfg[0], "stack": b_cfg[1], "morton": b_cfg[2], "lds_levels": b_cfg[3]},
+            "sweep_ms": {f"lanes{a}_stack{b}_order{c}_lds{d}": v for (a, b, c, d), v in b_sweep.items()},
         },
         "roofline": {
             "bound": "hbm",
@@ -303,7 +310,7 @@ def main():
             "unit": "GB/s",
             "frac": b_gbs / HBM_PEAK_GBS,
             "traffic": measured_traffic(),
-            "kernel": knn_kernel_name(k, b_cfg[0]),
+            "kernel": knn_kernel_name(k, b_cfg[0], b_cfg[3]),
             "algorithmic_bytes_per_launch": b_bpl,
             "avg_launch_ms": b_avg * 1e3,
             "bytes_per_query": alg_bytes / nq_total,
@@ -317,11 +324,12 @@ def main():
         "pipeline": pipe,
         "batching": batching,
         "serving": serving,
+        "box_radius_5M": box_radius,
         "build_2M_s": build_s,
         "pool": map_pool,
     }
 
-    progress("phase D done")
+    progress("phase E done")
     if not args.no_cpu:
         cpu_idx = timed_scans[: args.cpu_scans]
         gpu_out = tuple(np.concatenate([po[j][m].cpu().numpy() for j in cpu_idx]) for m in (1, 2, 3))
@@ -592,6 +600,120 @@ def serving_phase(ikd, scenes, n_robots, seconds, defer_points, chunk_points, sl
     return res
 
 
+def _trim_slabs(pose, half):
+    """Six half-open slabs: everything outside a cube of side 2*half around pose."""
+    big = 1e7
+    lo, hi = pose - half, pose + half
+    return np.array([[-big, -big, -big, lo[0], big, big], [hi[0], -big, -big, big, big, big],
+                     [-big, -big, -big, big, lo[1], big], [-big, hi[1], -big, big, big, big],
+                     [-big, -big, -big, big, big, lo[2]], [-big, -big, hi[2], big, big, big]], np.float32)
+
+
+def box_radius_phase(ikd, scenes, n_points, frames=20, trim_every=10, seed=1003, reps=5, cpu=True):
+    """Box/radius and deletion stress on one map: raw scans of mapping scene
+    `seed` up to n_points points, Build, then 1000 Box_Search boxes (centred on
+    map points, sides log-uniform 1-20 m) and 1000 Radius_Search balls (centred
+    on map points, r uniform 0.2-3 m), each set in one call (count pass, scan,
+    fill pass, results copied back to the host): median wall time of `reps`
+    calls.  Then `frames` mapping frames (kNN k=5, max_dist 5 m of the scan,
+    Add_Points with 0.5 m voxels) with Delete_Point_Boxes of everything outside
+    a 200 m cube around the sensor every trim_every frames: wall time per frame
+    and per trim.  With cpu, the oracle runs a sample of the same boxes and
+    balls (cpu_range_baseline)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    rng = np.random.default_rng(seed)
+    sc = scenes.MappingScene(seed)
+    with ThreadPoolExecutor(max_workers=16) as pool:
+        pts, f = _scene_points(scenes, seed, 0, int(SCALE * n_points), pool)
+        frame_scans = list(pool.map(sc.scan, range(f, f + frames)))
+    n = pts.shape[0]
+    g = ikd.KD_TREE(0.5, 0.6, 0.5)
+    t0 = time.perf_counter()
+    g.Build(pts)
+    build_s = time.perf_counter() - t0
+    c = pts[rng.integers(0, n, 1000)]
+    side = np.exp(rng.uniform(0.0, np.log(20.0), (1000, 3))).astype(np.float32)
+    boxes = np.concatenate([c - side / 2, c + side / 2], axis=1).astype(np.float32)
+    cen = pts[rng.integers(0, n, 1000)]
+    rad = (0.2 + 2.8 * rng.random(1000)).astype(np.float32)
+
+    def timed(fn):
+        out = fn()  # warm-up: scratch reaches its working size
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            out = fn()
+            ts.append(time.perf_counter() - t)
+        return float(np.median(ts)), out
+
+    box_s, box_out = timed(lambda: g.box_search_batch(boxes))
+    rad_s, rad_out = timed(lambda: g.radius_search_batch(cen, rad))
+    n_box, n_rad = int(box_out[0][-1]), int(rad_out[0][-1])
+    res = {
+        "map_points": int(n),
+        "build_s": build_s,
+        "box": {"queries": 1000, "results": n_box, "call_s": box_s, "queries_per_s": 1000 / box_s,
+                "result_points_per_s": n_box / box_s},
+        "radius": {"queries": 1000, "results": n_rad, "call_s": rad_s, "queries_per_s": 1000 / rad_s,
+                   "result_points_per_s": n_rad / rad_s},
+    }
+    if cpu:
+        res["cpu_baseline"] = cpu_range_baseline(pts, boxes, cen, rad, box_out, rad_out)
+    frame_ms, trim_ms, trimmed = [], [], []
+    rb0 = g.pool_stats()["rebuilds"]
+    for i, s in enumerate(frame_scans):
+        t = time.perf_counter()
+        g.knn(s, 5, 5.0)
+        g.Add_Points(s, True)
+        frame_ms.append((time.perf_counter() - t) * 1e3)
+        if i % trim_every == trim_every - 1:
+            slabs = _trim_slabs(sc.pose(f + i), 100.0)
+            t = time.perf_counter()
+            trimmed.append(int(g.Delete_Point_Boxes(slabs)))
+            trim_ms.append((time.perf_counter() - t) * 1e3)
+    res["mapping"] = {"frames": frames, "frame_ms": {"p50": float(np.median(frame_ms)), "max": float(max(frame_ms))},
+                      "trim_ms": trim_ms, "trimmed_points": trimmed,
+                      "rebuilds": int(g.pool_stats()["rebuilds"] - rb0), "validnum_end": int(g.validnum())}
+    g.close()
+    return res
+
+
+def cpu_range_baseline(pts, boxes, cen, rad, box_out, rad_out, sample=100):
+    """The oracle on the host: Build of the same map (not timed), then the
+    first `sample` boxes and balls one Box_Search / Radius_Search call each
+    (ikd_Tree.cpp:399-411), timed, and their results compared with the GPU's
+    (ids as sets; radius ids within the reference's sqrt rounding band of the
+    radius may differ, counted separately)."""
+    from oracle import OracleTree
+
+    o = OracleTree(0.5, 0.6, 0.5)
+    ids = np.arange(pts.shape[0], dtype=np.uint32)
+    o.Build(pts, ids)
+    boff, _, bids = box_out
+    roff, _, rids = rad_out
+    t = time.perf_counter()
+    ob = [o.Box_Search(boxes[i]) for i in range(sample)]
+    box_s = time.perf_counter() - t
+    t = time.perf_counter()
+    orr = [o.Radius_Search(cen[i], float(rad[i])) for i in range(sample)]
+    rad_s = time.perf_counter() - t
+    box_bad = sum(set(ob[i][1].tolist()) != set(bids[boff[i]:boff[i + 1]].tolist()) for i in range(sample))
+    rad_bad = band = 0
+    for i in range(sample):
+        a, b = set(orr[i][1].tolist()), set(rids[roff[i]:roff[i + 1]].tolist())
+        if a != b:
+            d = np.sqrt(((pts[list(a ^ b)].astype(np.float64) - cen[i]) ** 2).sum(axis=1))
+            if np.all(np.abs(d - rad[i]) <= 1e-5 * max(1.0, float(rad[i]))):
+                band += 1
+            else:
+                rad_bad += 1
+    return {"kind": "port", "cores": 1, "sample": f"first {sample} boxes and {sample} balls of the phase",
+            "box_queries_per_s": sample / box_s, "radius_queries_per_s": sample / rad_s,
+            "box_mismatches": int(box_bad), "radius_mismatches": int(rad_bad),
+            "radius_band_differences": int(band)}
+
+
 def copy_calibration(torch, mib):
     """A plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
     counters per dispatch give the scale factor from counter units to bytes."""
@@ -617,10 +739,10 @@ def measured_traffic():
     return None
 
 
-def knn_kernel_name(k, lanes):
+def knn_kernel_name(k, lanes, lds_levels=0):
     """The kernel symbol a launch shape runs (as rocprofv3 names it)."""
     if lanes == 1:
-        return f"ikd::k_knn<{knn_bucket(k)}>"
+        return f"ikd::k_knn_staged<{knn_bucket(k)}>" if lds_levels > 0 else f"ikd::k_knn<{knn_bucket(k)}>"
     if lanes == 0:
         return f"ikd::k_knn_persist<{knn_bucket(k)}>"
     if lanes < 0:

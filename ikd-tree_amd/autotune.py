@@ -2,10 +2,12 @@
 
 Sweeps the lane schedule (0: persistent lanes that fetch queries; 1, 2, 4, 8:
 lanes per query, i.e. 64, 32, 16, 8 queries per wavefront), the per-lane LDS
-stack depth (32, 64) and the query order (input order or Morton order) on a representative device-resident batch, timing each
-configuration with HIP events on the tree's stream.  Every configuration
-returns identical results (the selection order is total), so the tuner only
-changes speed; ``verify=True`` checks that on the sweep batch.
+stack depth (the tree depth, or 64), the query order (input order or Morton
+order) and, for one lane per query, the tree levels staged in LDS (0: none)
+on a representative device-resident batch, timing each configuration with HIP
+events on the tree's stream.  Every configuration returns identical results
+(the selection order is total), so the tuner only changes speed;
+``verify=True`` checks that on the sweep batch.
 """
 from __future__ import annotations
 
@@ -13,10 +15,10 @@ import itertools
 
 
 def autotune_knn(tree, queries, k: int, max_dist: float = float("inf"), reps: int = 3, verify: bool = True,
-                 lanes=(0, 1, 2, 4, 8), stacks=(0, 64), orders=(0, 1)):
+                 lanes=(0, 1, 2, 4, 8), stacks=(0, 64), orders=(0, 1), stages=(4, 6, 8)):
     """Time the kNN launch on ``queries`` (a contiguous cuda float32 tensor [n, 3])
-    for every configuration, apply the fastest to ``tree`` and return
-    (best_config, {config: ms})."""
+    for every configuration (lanes, stack, order, staged levels), apply the
+    fastest to ``tree`` and return (best_config, {config: ms})."""
     import torch
     import importlib
 
@@ -30,10 +32,14 @@ def autotune_knn(tree, queries, k: int, max_dist: float = float("inf"), reps: in
     ref = None
     ext = torch.cuda.ExternalStream(tree.stream())
     timings = {}
-    for ln, st, od in itertools.product(lanes, stacks, orders):
+    configs = [(ln, st, od, 0) for ln, st, od in itertools.product(lanes, stacks, orders)]
+    if 1 in lanes:
+        configs += [(1, st, od, lv) for st, od, lv in itertools.product(stacks, orders, stages) if lv > 0]
+    for ln, st, od, lv in configs:
         tree.set_tuning(ikd.IKD_TUNE_LANES_PER_QUERY, ln)
         tree.set_tuning(ikd.IKD_TUNE_KNN_STACK, st)
         tree.set_tuning(ikd.IKD_TUNE_QUERY_ORDER, od)
+        tree.set_tuning(ikd.IKD_TUNE_KNN_STAGE, lv)
         tree.knn_device(queries, k, max_dist, *outs)  # warm
         tree.synchronize()
         if verify:
@@ -41,7 +47,7 @@ def autotune_knn(tree, queries, k: int, max_dist: float = float("inf"), reps: in
             if ref is None:
                 ref = cur
             elif not all(torch.equal(x, y) for x, y in zip(ref, cur)):
-                raise AssertionError(f"kNN results changed under tuning lanes={ln} stack={st} order={od}")
+                raise AssertionError(f"kNN results changed under tuning lanes={ln} stack={st} order={od} stage={lv}")
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record(ext)
@@ -49,9 +55,10 @@ def autotune_knn(tree, queries, k: int, max_dist: float = float("inf"), reps: in
             tree.knn_device(queries, k, max_dist, *outs)
         e1.record(ext)
         tree.synchronize()
-        timings[(ln, st, od)] = e0.elapsed_time(e1) / reps
+        timings[(ln, st, od, lv)] = e0.elapsed_time(e1) / reps
     best = min(timings, key=timings.get)
     tree.set_tuning(ikd.IKD_TUNE_LANES_PER_QUERY, best[0])
     tree.set_tuning(ikd.IKD_TUNE_KNN_STACK, best[1])
     tree.set_tuning(ikd.IKD_TUNE_QUERY_ORDER, best[2])
+    tree.set_tuning(ikd.IKD_TUNE_KNN_STAGE, best[3])
     return best, timings

@@ -224,6 +224,7 @@ struct ikd_tree_s {
     int knn_lanes = -1;   // lanes per query: -1 picks from the batch size; 0 persistent; else 1, 2, 4, 8, 16
     int small_build = 1;  // build subtrees of <= SMALL_MAX points with one workgroup each
     int fold_lanes = 0;   // lanes per voxel in the downsample fold: 0 picks by map size; else 1, 4, 8, 16
+    int knn_stage = 0;    // one-lane kNN: tree levels staged in each workgroup's LDS (0: none)
     int record_removed = 0;     // 1: rebuilds record the deleted points they drop (acquire_removed_points)
     float4* removed = nullptr;  // that record (x, y, z, id bits), hdr->removed_count entries
     long long removed_cap = 0;
@@ -260,7 +261,7 @@ struct ikd_batch_s {
     hipMemPool_t mp = nullptr;
     Stager stg;
     std::mutex mu;
-    DevBuf b[8];
+    DevBuf b[16];  // BatchBuf slots
 };
 
 namespace {
@@ -272,7 +273,7 @@ enum Buf {
     B_PFLAG, B_PSCAN, B_ISEG, B_ISEGD, B_IO0, B_IO1, B_IO2, B_IO3, B_IO4, B_VSTART, B_DELBOX, B_DELFL, B_INSPT,
     B_INSFL, B_CBOX, B_CPTS, B_MISC, B_MORTK, B_MORTV, B_MORTK2, B_MORTV2, B_CNT64, B_OFF64, B_ADDX, B_ADDI, B_PNEXT, B_SPAIRS, B_SPSCAN,
     B_DEFER, B_DINFO, B_DBEFORE, B_DKEEP, B_PTS3, B_PEND, B_START, B_INV0, B_LASTP, B_PSEED, B_DBELOW, B_MAXDEL,
-    B_SLISTS, B_ENUMROOT, B_ENUMGRP, B_COUNT
+    B_SLISTS, B_ENUMROOT, B_ENUMGRP, B_TOPT, B_TOPI, B_TCNT64, B_TOFF64, B_RTASKS, B_COUNT
 };
 static_assert(B_COUNT <= 96, "scratch table");
 
@@ -496,8 +497,10 @@ int sort_keys(ikd_tree_t t, const unsigned long long* kin, unsigned long long* k
 // launch: each of the 1024 threads sums a run of consecutive values, the run
 // totals are scanned across the wavefronts, then each thread rewrites its run.
 // (The device-wide scan costs two launches and a temporary-size query, which
-// dominates at these sizes.)  in == out is allowed.
-constexpr int SCAN_ONE_MAX = 1 << 17;
+// dominates at these sizes; from a few tiles of values on, its many
+// workgroups finish first, measured on the serving workload.)  in == out is
+// allowed.
+constexpr int SCAN_ONE_MAX = 8192;
 __device__ __forceinline__ int scan_shfl(int x, int src) { return __shfl(x, src); }
 __device__ __forceinline__ long long scan_shfl(long long x, int src) {
     const unsigned lo = (unsigned)__shfl((int)(unsigned)x, src);
@@ -784,6 +787,9 @@ constexpr long long ENUM_BLOCK_MAX = 1024;
 // larger one walks one level per launch over the whole grid instead).
 constexpr long long ENUM_GROUP_NODES = 2048;
 constexpr long long ENUM_GROUP_ROOT_MAX = 16384;
+// Range search: subtrees a query covers whole with at most this many nodes are
+// written by one lane each after the walks (the walks stop there).
+constexpr int RANGE_TASK_NODES = 256;
 
 // Room for `extra` more records beyond the current count (fresh in h_hdr):
 // the record survives the growth, unlike scratch.
@@ -1470,6 +1476,25 @@ int knn_dev(ikd_tree_t t, const float* d_q, int nq, int k, double max_dist, floa
     if (t->knn_stack > stack) stack = t->knn_stack;
     if (stack > MAX_DEPTH) stack = MAX_DEPTH;
     int lanes = t->knn_lanes >= 0 ? t->knn_lanes : auto_lanes(nq);
+    if (t->knn_stage > 0 && lanes == 1) {
+        // the staged levels, capped so that one workgroup's LDS (stacks and
+        // table) stays within 64 KiB
+        int levels = t->knn_stage;
+        const long long stack_b = ((long long)stack * 64 * 6 + 15) / 16 * 16;
+        while (levels > 0 && stack_b + ((1ll << levels) - 1) * (long long)(sizeof(NodeHot) + sizeof(int)) > 65536)
+            --levels;
+        if (levels > 0) {
+            const int slots = (1 << levels) - 1;
+            NodeHot* top;
+            int* top_idx;
+            CHK(ensure(t, B_TOPT, slots, &top));
+            CHK(ensure(t, B_TOPI, slots, &top_idx));
+            launch_top_table(t->pool.hot, levels, top, top_idx, t->st);
+            a.top = top;
+            a.top_idx = top_idx;
+            a.top_slots = slots;
+        }
+    }
     int* next;
     CHK(ensure(t, B_PNEXT, 8, &next));
     if (launch_knn(a, k, lanes, stack, t->st, next, 256 * 10) != 0) return IKD_ERR_UNSUPPORTED;
@@ -1484,11 +1509,16 @@ int range_dev(ikd_tree_t t, int mode, const float* d_boxes, const float* d_cente
         CHK(compact_if_needed(t, true));
     }
     hipStream_t st = t->st;
-    long long *cnt, *off;
+    long long *cnt, *off, *tcnt, *toff;
     CHK(ensure(t, B_CNT64, nq + 1, &cnt));
     CHK(ensure(t, B_OFF64, nq + 1, &off));
-    launch_fill_i32(reinterpret_cast<int*>(cnt + nq), 0, 2, st);  // the scan's trailing zero
+    CHK(ensure(t, B_TCNT64, nq + 1, &tcnt));
+    CHK(ensure(t, B_TOFF64, nq + 1, &toff));
+    launch_fill_i32(reinterpret_cast<int*>(cnt + nq), 0, 2, st);  // the scans' trailing zeros
+    launch_fill_i32(reinterpret_cast<int*>(tcnt + nq), 0, 2, st);
     RangeArgs a{};
+    a.task_nodes = RANGE_TASK_NODES;
+    a.tcounts = tcnt;
     a.max_steps = t->pool_used + 16;
     a.pool = t->pool;
     a.boxes = d_boxes;
@@ -1501,10 +1531,14 @@ int range_dev(ikd_tree_t t, int mode, const float* d_boxes, const float* d_cente
     a.stack = std::min(t->max_depth + 2, MAX_DEPTH);
     launch_range(a, 0, st);
     CHK(exscan_i64(t, cnt, off, nq + 1));
+    CHK(exscan_i64(t, tcnt, toff, nq + 1));
     HIPCHK(hipMemcpyAsync(h_offsets, off, (nq + 1) * sizeof(long long), hipMemcpyDeviceToHost, st));
+    long long ntask = 0;
+    HIPCHK(hipMemcpyAsync(&ntask, toff + nq, sizeof(long long), hipMemcpyDeviceToHost, st));
     CHK(sync_hdr(t));
     long long total = h_offsets[nq];
     CHK(check_count(t, "range search: results", total, 0, (long long)nq * std::max<long long>(t->pool_used, 1)));
+    CHK(check_count(t, "range search: covered subtrees", ntask, 0, total));
     if (!h_xyz) return IKD_OK;
     if (capacity < total) {
         t->err = "output capacity below the result count";
@@ -1515,10 +1549,15 @@ int range_dev(ikd_tree_t t, int mode, const float* d_boxes, const float* d_cente
     uint32_t* di;
     CHK(ensure(t, B_IO0, 3 * total, &dx));
     CHK(ensure(t, B_IO1, total, &di));
+    int4* tasks;
+    CHK(ensure(t, B_RTASKS, std::max<long long>(ntask, 1), &tasks));
     a.offsets = off;
+    a.toffsets = toff;
+    a.tasks = tasks;
     a.out_xyz = dx;
     a.out_ids = di;
     launch_range(a, 1, st);
+    launch_range_tasks(a, ntask, st);
     HIPCHK(hipMemcpyAsync(h_xyz, dx, 3 * total * sizeof(float), hipMemcpyDeviceToHost, st));
     if (h_ids) HIPCHK(hipMemcpyAsync(h_ids, di, total * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     return sync_hdr(t);
@@ -2146,6 +2185,10 @@ int ikd_set_tuning(ikd_tree_t t, int key, int value) {
             if (!(value == 0 || value == 1 || value == 4 || value == 8 || value == 16)) return IKD_ERR_ARG;
             t->fold_lanes = value;
             return IKD_OK;
+        case IKD_TUNE_KNN_STAGE:
+            if (value < 0 || value > 12) return IKD_ERR_ARG;
+            t->knn_stage = value;
+            return IKD_OK;
         default:
             return IKD_ERR_ARG;
     }
@@ -2207,7 +2250,9 @@ const char* ikd_last_error(ikd_tree_t t) {
 }
 
 // ------------------------------------------------------------ multi-tree batches
-enum BatchBuf { BB_TMP = 0, BB_TREES, BB_THR, BB_NEXT, BB_FLAG, BB_CNT, BB_POOLS };
+enum BatchBuf { BB_TMP = 0, BB_TREES, BB_THR, BB_NEXT, BB_FLAG, BB_CNT, BB_POOLS, BB_TCNT, BB_TOFF, BB_TASKS,
+                BB_COUNT };
+static_assert(BB_COUNT <= 16, "batch scratch table");
 
 // ------------------------------------- allocation-free multi-tree kNN scratch
 // ikd_batch_nearest_search_noalloc keeps everything beyond its inputs and
@@ -2533,12 +2578,18 @@ int ikd_batch_radius_search_device(ikd_batch_t b, const ikd_tree_t* trees, int n
     Pool* d_pools;
     long long* cnt;
     int* flag;
+    long long *tcnt, *toff;
     CHK(ensure(b, BB_POOLS, ntrees, &d_pools));
     CHK(ensure(b, BB_CNT, nq + 1, &cnt));
+    CHK(ensure(b, BB_TCNT, nq + 1, &tcnt));
+    CHK(ensure(b, BB_TOFF, nq + 1, &toff));
     CHK(ensure(b, BB_FLAG, 1, &flag));
     CHK(stage_copy(b, d_pools, pools.data(), ntrees * sizeof(Pool)));
     launch_fill_i32(reinterpret_cast<int*>(cnt + nq), 0, 2, b->st);
+    launch_fill_i32(reinterpret_cast<int*>(tcnt + nq), 0, 2, b->st);
     RangeArgs a{};
+    a.task_nodes = RANGE_TASK_NODES;
+    a.tcounts = tcnt;
     a.pools = d_pools;
     a.qtree = d_query_tree;
     a.ntrees = ntrees;
@@ -2552,15 +2603,26 @@ int ikd_batch_radius_search_device(ikd_batch_t b, const ikd_tree_t* trees, int n
     a.stack = std::min(depth + 2, MAX_DEPTH);
     launch_range(a, 0, b->st);
     CHK(exscan_i64(b, cnt, reinterpret_cast<long long*>(d_out_offsets), (int)nq + 1));
-    long long total = 0;
+    CHK(exscan_i64(b, tcnt, toff, (int)nq + 1));
+    long long total = 0, ntask = 0;
     HIPCHK(hipMemcpyAsync(&total, d_out_offsets + nq, sizeof(long long), hipMemcpyDeviceToHost, b->st));
+    HIPCHK(hipMemcpyAsync(&ntask, toff + nq, sizeof(long long), hipMemcpyDeviceToHost, b->st));
     HIPCHK(hipStreamSynchronize(b->st));
     *out_total = total;
+    if (total < 0 || ntask < 0 || ntask > total) {
+        b->err = "batch: device result counts out of range";
+        return IKD_ERR_INTERNAL;
+    }
     if (d_out_xyz && d_out_ids && capacity >= total && total > 0) {
+        int4* tasks;
+        CHK(ensure(b, BB_TASKS, std::max<long long>(ntask, 1), &tasks));
         a.offsets = reinterpret_cast<const long long*>(d_out_offsets);
+        a.toffsets = toff;
+        a.tasks = tasks;
         a.out_xyz = d_out_xyz;
         a.out_ids = d_out_ids;
         launch_range(a, 1, b->st);
+        launch_range_tasks(a, ntask, b->st);
         HIPCHK(hipGetLastError());
     }
     CHK(batch_end(b, trees, ntrees));

@@ -209,7 +209,16 @@ struct KnnArgs {
     int* overflow;
     int max_steps;          // visit bound per query (the pool size): a corrupt tree cannot hang a lane
     int stack;              // LDS stack entries per lane (set by launch_knn)
+    // optional, one-lane single-tree launches: the tree's top levels staged in
+    // each workgroup's LDS (top_slots = 2^levels - 1 records, launch_top_table)
+    const NodeHot* top;
+    const int* top_idx;
+    int top_slots;
 };
+// Copies the tree's first `levels` levels into a complete binary layout (slot 0
+// the root, slot s's children at 2s+1 and 2s+2): top[s] the node's record,
+// top_idx[s] its index or -1 where the tree has no node.
+void launch_top_table(const NodeHot* hot, int levels, NodeHot* top, int* top_idx, hipStream_t s);
 // lanes_per_query 0 (persistent lanes), 1 (one lane per query) or 2, 4, 8, 16 (a team of lanes
 // shares one query's traversal); stack = LDS entries per lane (1-lane) or
 // per-lane share of the team stack.
@@ -239,8 +248,17 @@ struct RangeArgs {
     const Pool* pools;    // multi-tree launch: pool table with qtree[q] per query (else null)
     const int* qtree;
     int ntrees;           // a query naming no valid tree finds nothing
+    // Covered-subtree tasks (task_nodes > 0): the walks stop at subtrees the
+    // query covers whole with at most task_nodes nodes; the count pass counts
+    // them per query (tcounts), the fill pass reserves each one's output range
+    // in pre-order and lists it (tasks at toffsets), launch_range_tasks writes them.
+    int task_nodes;
+    long long* tcounts;
+    const long long* toffsets;
+    int4* tasks;          // (node, query, output offset low, high)
 };
 void launch_range(const RangeArgs& a, int fill, hipStream_t s);
+void launch_range_tasks(const RangeArgs& a, long long ntask, hipStream_t s);
 
 // ---------------------------------------------------------------- misc
 void launch_fill_i32(int* p, int v, long long n, hipStream_t s);

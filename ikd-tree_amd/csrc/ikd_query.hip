@@ -292,6 +292,169 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs a) {
     finish_query<KM>(a, c, best, visits, boxes);
 }
 
+// ---- one lane per query, top levels staged in LDS -------------------------
+// The same walk as k_knn; the first levels of the tree are read from a copy in
+// the workgroup's LDS instead of global memory.  A node handle is its index
+// (>= 0), -1 for none, or -2 - s for slot s of the staged table, whose
+// children are slots 2s+1 and 2s+2 down to the table's last level and node
+// indices below it.  The best list keeps node indices (top_idx), so results
+// and output are those of k_knn.
+__global__ void k_top_table(const NodeHot* __restrict__ hot, int slots, NodeHot* __restrict__ top,
+                            int* __restrict__ top_idx) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= slots) return;
+    // the path from the root: the bits of s + 1 below its leading one, top down
+    const uint32_t p = (uint32_t)s + 1u;
+    const int depth = 31 - __builtin_clz(p);
+    const uint32_t lk0 = fbits(hot[SENTINEL].f[3]);
+    int n = (lk0 & LK_HASL) ? ROOT : -1;
+    for (int d = depth - 1; d >= 0 && n >= 0; --d) {
+        const int side = (int)((p >> d) & 1u);
+        const uint32_t lk = fbits(hot[n].f[3]);
+        n = (lk & lk_has(side)) ? (int)(2 * lk_pair(lk)) + side : -1;
+    }
+    top_idx[s] = n;
+    float4* o = reinterpret_cast<float4*>(top[s].f);
+    if (n >= 0) {
+        const float4* r = reinterpret_cast<const float4*>(hot[n].f);
+        o[0] = r[0];
+        o[1] = r[1];
+        o[2] = r[2];
+        o[3] = r[3];
+    } else {
+        o[0] = o[1] = o[2] = o[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+void launch_top_table(const NodeHot* hot, int levels, NodeHot* top, int* top_idx, hipStream_t s) {
+    const int slots = (1 << levels) - 1;
+    hipLaunchKernelGGL(k_top_table, dim3(blocks_for(slots, 256)), dim3(256), 0, s, hot, slots, top, top_idx);
+}
+
+// bytes of the LDS stacks rounded up to whole 16-byte rows (the table follows)
+IKD_HD size_t lds_stack_bytes16(int stack) { return ((size_t)stack * 64 * (sizeof(int) + sizeof(unsigned short)) + 15) & ~(size_t)15; }
+static inline size_t lds_top_bytes(int slots) { return (size_t)slots * (sizeof(NodeHot) + sizeof(int)); }
+
+template <int KM>
+__global__ __launch_bounds__(64) void k_knn_staged(KnnArgs a) {
+    extern __shared__ int lds_dyn[];
+    const LdsStack st = lds_stack(a.stack);
+    const int slots = a.top_slots;
+    float4* t4 = reinterpret_cast<float4*>(reinterpret_cast<char*>(lds_dyn) + lds_stack_bytes16(a.stack));
+    int* tix = reinterpret_cast<int*>(t4 + 4 * slots);
+    const int lane = threadIdx.x;
+    {
+        const gv4* src = reinterpret_cast<const gv4*>(a.top);
+        for (int i = lane; i < 4 * slots; i += 64) {
+            const gv4 v = gload4(src + i);
+            t4[i] = make_float4(v.x, v.y, v.z, v.w);
+        }
+        for (int i = lane; i < slots; i += 64) tix[i] = a.top_idx[i];
+    }
+    __syncthreads();
+    const int chunk = xcd_chunk(blockIdx.x, (a.nq + 63) / 64);
+    const int gq = chunk * 64 + lane;
+    if (gq >= a.nq) return;
+    const QueryCtx c = load_query(a, gq);
+    BestList<KM> best;
+    best.init(c.k);
+    unsigned int visits = 0, boxes = 0;
+
+    const uint32_t lk0 = fbits(c.hot[SENTINEL].f[3]);
+    int sp = 0;
+    int n = -1;
+    float bd = 0.0f;
+    if ((lk0 & LK_HASL) && !(lk0 & LK_TDL)) {
+        n = -2;  // the root: slot 0
+        bd = calc_box_dist(c.hot[SENTINEL].f + 4, c.qx, c.qy, c.qz);
+    }
+    while (true) {
+        if ((int)visits > a.max_steps) {
+            atomicOr(a.overflow, 2);
+            break;
+        }
+        if (n != -1 && bd <= c.thr) {
+            ++visits;
+            float4 r0, r1, r2, r3;
+            int node, cl, cr;
+            if (n <= -2) {
+                const int s = -2 - n;
+                r0 = t4[4 * s];
+                r1 = t4[4 * s + 1];
+                r2 = t4[4 * s + 2];
+                r3 = t4[4 * s + 3];
+                node = tix[s];
+                const uint32_t lk = fbits(r0.w);
+                const bool inner = 2 * s + 2 < slots;
+                cl = inner ? -2 - (2 * s + 1) : (int)(2 * lk_pair(lk));
+                cr = inner ? -2 - (2 * s + 2) : (int)(2 * lk_pair(lk)) + 1;
+            } else {
+                const gv4* rec = reinterpret_cast<const gv4*>(c.hot[n].f);
+                const gv4 g0 = gload4(rec), g1 = gload4(rec + 1), g2 = gload4(rec + 2), g3 = gload4(rec + 3);
+                r0 = make_float4(g0.x, g0.y, g0.z, g0.w);
+                r1 = make_float4(g1.x, g1.y, g1.z, g1.w);
+                r2 = make_float4(g2.x, g2.y, g2.z, g2.w);
+                r3 = make_float4(g3.x, g3.y, g3.z, g3.w);
+                node = n;
+                const uint32_t lk = fbits(r0.w);
+                cl = (int)(2 * lk_pair(lk));
+                cr = cl + 1;
+            }
+            const uint32_t lk = fbits(r0.w);
+            if (!(lk & LK_PDEL)) {
+                const float d = calc_dist(c.qx, c.qy, c.qz, r0.x, r0.y, r0.z);
+                if (d <= c.thr) best.offer(c.hot, c.ids, d, node);
+            }
+            const bool hl = (lk & LK_HASL) != 0, hr = (lk & LK_HASR) != 0;
+            boxes += (hl ? 1u : 0u) + (hr ? 1u : 0u);
+            const bool el = hl && !(lk & LK_TDL), er = hr && !(lk & LK_TDR);
+            const float bl[6] = {r1.x, r1.y, r1.z, r1.w, r2.x, r2.y};
+            const float br[6] = {r2.z, r2.w, r3.x, r3.y, r3.z, r3.w};
+            const float dl = el ? calc_box_dist(bl, c.qx, c.qy, c.qz) : INFINITY;
+            const float dr = er ? calc_box_dist(br, c.qx, c.qy, c.qz) : INFINITY;
+            int nn = -1, fn = -1;
+            float dn = 0.0f, df = 0.0f;
+            if (el || er) {
+                const bool near_left = el && (!er || dl <= dr);
+                nn = near_left ? cl : cr;
+                dn = near_left ? dl : dr;
+                if (near_left ? er : el) {
+                    fn = near_left ? cr : cl;
+                    df = near_left ? dr : dl;
+                }
+            }
+            const float top = best.d[0];
+            if (fn != -1 && df <= top && df <= c.thr) {
+                if (sp < a.stack) {
+                    st.n[sp * 64 + lane] = fn;
+                    st.d[sp * 64 + lane] = dist_key(df);
+                    ++sp;
+                } else {
+                    atomicOr(a.overflow, 1);
+                }
+            }
+            if (nn != -1 && dn <= top && dn <= c.thr) {
+                n = nn;
+                bd = dn;
+                continue;
+            }
+        }
+        n = -1;
+        while (sp > 0) {
+            --sp;
+            const float d = dist_val(st.d[sp * 64 + lane]);
+            if (d <= best.d[0]) {
+                n = st.n[sp * 64 + lane];
+                bd = d;
+                break;
+            }
+        }
+        if (n == -1) break;
+    }
+
+    finish_query<KM>(a, c, best, visits, boxes);
+}
+
 // ---- a team of T lanes per query ------------------------------------------
 // Each lane of a team runs its own depth-first walk with its own deque of
 // deferred far children (a ring buffer in LDS holding at most one entry per
@@ -563,6 +726,11 @@ static int launch_knn_km(const KnnArgs& a0, int lanes, int stack, hipStream_t s,
     const size_t lds = lds_stack_bytes(stack);
     switch (lanes) {
         case 1:
+            if (a.top_slots > 0 && a.top && !a.trees) {
+                const size_t lds_t = lds_stack_bytes16(stack) + lds_top_bytes(a.top_slots);
+                hipLaunchKernelGGL((k_knn_staged<KM>), dim3(xcd_grid(blocks_for(a.nq, 64))), block, lds_t, s, a);
+                return 0;
+            }
             hipLaunchKernelGGL((k_knn<KM>), dim3(xcd_grid(blocks_for(a.nq, 64))), block, lds, s, a);
             return 0;
         case 0: {  // persistent lanes; the 8 per-XCD query counters were zeroed by the caller
@@ -694,22 +862,40 @@ __device__ __forceinline__ int classify(const RangeArgs& a, int qi, const float*
 // One lane per query, 64 per workgroup; the depth-first stack lives in
 // dynamic LDS as [stack][64] entries (node index, top bit = whole subtree).
 // Each pop pushes at most its two children, so depth + 2 entries suffice.
+// Stack entries: node index (< 2^28) with bit 31 = covered whole, bit 30 = a
+// covered-subtree task (handled when popped, so tasks keep pre-order).
+constexpr int RANGE_WHOLE = (int)0x80000000;
+constexpr int RANGE_TASK = 0x40000000;
+constexpr int RANGE_NODE = 0x3fffffff;
+
+__device__ __forceinline__ Pool range_pool(const RangeArgs& a, int qi, bool* ok) {
+    *ok = true;
+    if (!a.pools) return a.pool;
+    const int ti = a.qtree[qi];
+    if (ti < 0 || ti >= a.ntrees) {
+        *ok = false;
+        return a.pool;
+    }
+    return a.pools[ti];
+}
+
 __global__ __launch_bounds__(64) void k_range(RangeArgs a, int fill) {
     extern __shared__ int lds_dyn[];
     int* stk = lds_dyn;
     const int lane = threadIdx.x;
     const int qi = blockIdx.x * 64 + lane;
     if (qi >= a.nq) return;
-    Pool P = a.pool;
-    if (a.pools) {
-        const int ti = a.qtree[qi];
-        if (ti < 0 || ti >= a.ntrees) {
-            if (!fill) a.counts[qi] = 0;
-            return;
+    bool ok;
+    const Pool P = range_pool(a, qi, &ok);
+    if (!ok) {
+        if (!fill) {
+            a.counts[qi] = 0;
+            if (a.tcounts) a.tcounts[qi] = 0;
         }
-        P = a.pools[ti];
+        return;
     }
     const NodeHot* __restrict__ hot = P.hot;
+    const int G = a.task_nodes;
     float qx = 0, qy = 0, qz = 0, r = 0;
     if (a.mode == 1) {
         qx = a.centers[3 * qi];
@@ -717,9 +903,11 @@ __global__ __launch_bounds__(64) void k_range(RangeArgs a, int fill) {
         qz = a.centers[3 * qi + 2];
         r = a.radii[qi];
     }
-    long long total = 0;
+    long long total = 0, ntask = 0;
     long long pos = fill ? a.offsets[qi] : 0;
-    long long lim_end = fill ? a.offsets[qi + 1] : 0;
+    const long long lim_end = fill ? a.offsets[qi + 1] : 0;
+    long long tpos = (fill && G > 0) ? a.toffsets[qi] : 0;
+    const long long tlim = (fill && G > 0) ? a.toffsets[qi + 1] : 0;
     auto emit = [&](int u, const float* f) {
         if (fill) {
             if (pos >= lim_end) {
@@ -735,30 +923,55 @@ __global__ __launch_bounds__(64) void k_range(RangeArgs a, int fill) {
             ++total;
         }
     };
+    // a subtree the query covers whole: counted at once, or a task, or walked
+    auto covered = [&](int c) -> int {
+        if (G > 0 && P.cnt[c].x <= G) return c | RANGE_TASK;
+        if (G == 0 && !fill) {
+            const int4 cc = P.cnt[c];
+            total += cc.x - cc.y;
+            return -1;
+        }
+        return c | RANGE_WHOLE;
+    };
     uint32_t lk0 = fbits(hot[SENTINEL].f[3]);
     if ((lk0 & LK_HASL) && !(lk0 & LK_TDL)) {
         int sp = 0;
-        int c0 = classify(a, qi, hot[SENTINEL].f + 4, qx, qy, qz, r);
-        if (c0 == 1 && !fill) {
-            int4 cc = P.cnt[ROOT];
-            total += cc.x - cc.y;
-        } else if (c0 != 0) {
-            stk[lane] = ROOT | (c0 == 1 ? (int)0x80000000 : 0);
-            sp = 1;
+        const int c0 = classify(a, qi, hot[SENTINEL].f + 4, qx, qy, qz, r);
+        if (c0 != 0) {
+            const int e0 = c0 == 1 ? covered(ROOT) : ROOT;
+            if (e0 != -1) {
+                stk[lane] = e0;
+                sp = 1;
+            }
         }
-        float r2 = r * r;
+        const float r2 = r * r;
         long long steps = 0;
         while (sp > 0) {
             if (++steps > a.max_steps) {
                 atomicOr(a.overflow, 8);
                 break;
             }
-            int e = stk[--sp * 64 + lane];
-            bool whole = e < 0;
-            int u = e & 0x7fffffff;
+            const int e = stk[--sp * 64 + lane];
+            const int u = e & RANGE_NODE;
+            if (e & RANGE_TASK) {
+                const int4 cc = P.cnt[u];
+                const long long live = cc.x - cc.y;
+                if (live <= 0) continue;
+                if (!fill) {
+                    total += live;
+                    ++ntask;
+                } else if (tpos < tlim && pos + live <= lim_end) {
+                    a.tasks[tpos++] = make_int4(u, qi, (int)(unsigned)(pos & 0xffffffffll), (int)(pos >> 32));
+                    pos += live;
+                } else {
+                    atomicOr(a.overflow, 8);
+                }
+                continue;
+            }
+            const bool whole = (e & RANGE_WHOLE) != 0;
             float f[16];
             load_record(hot[u].f, f);
-            uint32_t lk = fbits(f[3]);
+            const uint32_t lk = fbits(f[3]);
             if (!(lk & LK_PDEL)) {
                 bool take;
                 if (whole) take = true;
@@ -766,30 +979,84 @@ __global__ __launch_bounds__(64) void k_range(RangeArgs a, int fill) {
                 else take = calc_dist(f[0], f[1], f[2], qx, qy, qz) <= r2;
                 if (take) emit(u, f);
             }
-            int pair = (int)lk_pair(lk);
+            const int pair = (int)lk_pair(lk);
 #pragma unroll
             for (int s = 1; s >= 0; --s) {
                 if (!(lk & lk_has(s)) || (lk & lk_td(s))) continue;
-                int c = 2 * pair + s;
-                int cls = whole ? 1 : classify(a, qi, f + 4 + 6 * s, qx, qy, qz, r);
+                const int c = 2 * pair + s;
+                const int cls = whole ? 1 : classify(a, qi, f + 4 + 6 * s, qx, qy, qz, r);
                 if (cls == 0) continue;
-                if (cls == 1 && !fill) {
-                    int4 cc = P.cnt[c];
-                    total += cc.x - cc.y;
-                    continue;
-                }
-                if (sp < a.stack) stk[sp++ * 64 + lane] = c | (cls == 1 ? (int)0x80000000 : 0);
+                const int ce = cls == 1 ? covered(c) : c;
+                if (ce == -1) continue;
+                if (sp < a.stack) stk[sp++ * 64 + lane] = ce;
                 else atomicOr(a.overflow, 8);
             }
         }
     }
-    if (!fill) a.counts[qi] = total;
+    if (!fill) {
+        a.counts[qi] = total;
+        if (a.tcounts) a.tcounts[qi] = ntask;
+    }
+}
+
+// One lane per covered-subtree task: every live point of the subtree in
+// pre-order (own point, left subtree, right subtree; deleted subtrees
+// skipped), written from the task's output offset.
+__global__ __launch_bounds__(64) void k_range_tasks(RangeArgs a, long long ntask) {
+    extern __shared__ int lds_dyn[];
+    int* stk = lds_dyn;
+    const int lane = threadIdx.x;
+    const long long ti = (long long)blockIdx.x * 64 + lane;
+    if (ti >= ntask) return;
+    const int4 t = a.tasks[ti];
+    bool ok;
+    const Pool P = range_pool(a, t.y, &ok);
+    if (!ok) return;
+    long long pos = (long long)(unsigned)t.z | ((long long)t.w << 32);
+    const long long lim_end = a.offsets[t.y + 1];
+    int sp = 1;
+    stk[lane] = t.x;
+    long long steps = 0;
+    while (sp > 0) {
+        if (++steps > a.max_steps) {
+            atomicOr(a.overflow, 8);
+            break;
+        }
+        const int u = stk[--sp * 64 + lane];
+        const gv4* rec = reinterpret_cast<const gv4*>(P.hot[u].f);
+        const gv4 r0 = gload4(rec);
+        const uint32_t lk = fbits(r0.w);
+        if (!(lk & LK_PDEL)) {
+            if (pos < lim_end) {
+                a.out_xyz[3 * pos] = r0.x;
+                a.out_xyz[3 * pos + 1] = r0.y;
+                a.out_xyz[3 * pos + 2] = r0.z;
+                a.out_ids[pos] = P.ids[u];
+                ++pos;
+            } else {
+                atomicOr(a.overflow, 8);
+            }
+        }
+        const int pair = (int)lk_pair(lk);
+#pragma unroll
+        for (int s = 1; s >= 0; --s) {
+            if (!(lk & lk_has(s)) || (lk & lk_td(s))) continue;
+            if (sp < a.stack) stk[sp++ * 64 + lane] = 2 * pair + s;
+            else atomicOr(a.overflow, 8);
+        }
+    }
 }
 
 void launch_range(const RangeArgs& a, int fill, hipStream_t s) {
     if (a.nq <= 0) return;
     hipLaunchKernelGGL(k_range, dim3(blocks_for(a.nq, 64)), dim3(64), (size_t)a.stack * 64 * sizeof(int), s, a,
                        fill);
+}
+
+void launch_range_tasks(const RangeArgs& a, long long ntask, hipStream_t s) {
+    if (ntask <= 0) return;
+    hipLaunchKernelGGL(k_range_tasks, dim3((unsigned)blocks_for(ntask, 64)), dim3(64),
+                       (size_t)a.stack * 64 * sizeof(int), s, a, ntask);
 }
 
 }  // namespace ikd

@@ -187,6 +187,15 @@ struct ikd_server_s {
             if (r->out_counts && hipMemcpyAsync(b.h_cnt, b.out_cnt, n * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
                 return IKD_ERR_HIP;
             if (hipEventRecord(b.ev, st) != hipSuccess) return IKD_ERR_HIP;
+            // The results are handed over before the insert is queued.  (Queuing
+            // the insert behind the search without this wait lets every map's
+            // work pile up in the shared hardware queues, which measured
+            // worse for the slowest requests.)
+            if (hipEventSynchronize(b.ev) != hipSuccess) return IKD_ERR_HIP;
+            if (r->out_d2) std::memcpy(r->out_d2, b.h_d2, res * 4);
+            if (r->out_ids) std::memcpy(r->out_ids, b.h_ids, res * 4);
+            if (r->out_xyz) std::memcpy(r->out_xyz, b.h_xyz, res * 12);
+            if (r->out_counts) std::memcpy(r->out_counts, b.h_cnt, n * 4);
         }
         if (r->downsample >= 0 && n > 0) {
             int32_t added = 0;
@@ -195,14 +204,6 @@ struct ikd_server_s {
             r->added = added;
         }
         r->t_insert_done_ns = now_ns();
-        if (knn) {
-            // the insert's own waits have usually passed the result copies already
-            if (hipEventSynchronize(b.ev) != hipSuccess) return IKD_ERR_HIP;
-            if (r->out_d2) std::memcpy(r->out_d2, b.h_d2, res * 4);
-            if (r->out_ids) std::memcpy(r->out_ids, b.h_ids, res * 4);
-            if (r->out_xyz) std::memcpy(r->out_xyz, b.h_xyz, res * 12);
-            if (r->out_counts) std::memcpy(r->out_counts, b.h_cnt, n * 4);
-        }
         if (r->n_trim > 0) {
             int32_t del = 0;
             rc = ikd_delete_point_boxes(t, r->trim_boxes, r->n_trim, &del);

@@ -178,6 +178,21 @@ class Server:
             out.extend(self.wait(-1))
         return out
 
+    def wait_idle(self, timeout_s: float = 60.0):
+        """Waits until no request is pending or running (workers hand a request
+        back before they finish the map's maintenance and accounting)."""
+        pend, fl = ctypes.c_int64(0), ctypes.c_int64(0)
+        end = time.monotonic() + timeout_s
+        while True:
+            rc = self._lib.ikd_server_pending(self._h, ctypes.byref(pend), ctypes.byref(fl))
+            if rc != 0:
+                raise _pkg.IkdError(rc, "pending failed")
+            if pend.value == 0 and fl.value == 0:
+                return
+            if time.monotonic() > end:
+                raise TimeoutError("server did not go idle")
+            time.sleep(0.001)
+
     def close(self):
         if getattr(self, "_h", None):
             self._lib.ikd_server_destroy(self._h)

@@ -205,6 +205,11 @@ int ikd_device_bytes(ikd_tree_t t, int64_t* out_bytes);
  *                       mode, ikd_Tree.cpp:638,1338-1341); 0 (default): no
  *                       record, and rebuilds skip deleted subtrees. */
 #define IKD_TUNE_RECORD_REMOVED 7
+/*  IKD_TUNE_KNN_STAGE   levels of the tree (0..12, default 0) that one-lane
+ *                       kNN launches copy into each workgroup's LDS and read
+ *                       from there (capped so that a workgroup's LDS stays
+ *                       within 64 KiB); the results are the same. */
+#define IKD_TUNE_KNN_STAGE 8
 int ikd_set_tuning(ikd_tree_t t, int key, int value);
 
 /* Run the deferred rebuilds that are still due, in the order they were

@@ -98,3 +98,27 @@ def test_facade_reclaims_payload_records(emu_lib, tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=900, env=env)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-3000:]
     assert r.stdout.startswith("ok")
+
+
+def _pcd_map(path, n, seed):
+    """A synthetic map as a binary_compressed PCD of PointXYZINormal fields."""
+    import importlib
+
+    import numpy as np
+    pcd = importlib.import_module("ikd-tree_amd.pcd")
+    rng = np.random.default_rng(seed)
+    f = {k: rng.uniform(-10, 10, n).astype(np.float32) for k in ("x", "y", "z")}
+    f.update({k: rng.uniform(-1, 1, n).astype(np.float32) for k in
+              ("intensity", "normal_x", "normal_y", "normal_z", "curvature")})
+    pcd.write_pcd(str(path), f, "binary_compressed")
+
+
+def test_pcd_map_through_the_facade(emu_lib, tmp_path):
+    """The Search demo's flow without PCL: a PCD map loaded into a PCL-layout
+    point type, built, searched; results keep their full records."""
+    _pcd_map(tmp_path / "map.pcd", 4000, 31)
+    exe = tmp_path / "pcd_payload"
+    env = _build_emu_exe("pcd_payload.cpp", exe)
+    r = subprocess.run([str(exe), str(tmp_path / "map.pcd")], capture_output=True, text=True, timeout=600, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-3000:]
+    assert r.stdout.startswith("ok")

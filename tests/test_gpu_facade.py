@@ -85,3 +85,17 @@ def test_reference_demo_call_sequences(tmp_path):
     out = subprocess.run([str(exe), str(ind), str(outd)], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout + out.stderr
     check_outputs(inputs, outd, out.stdout)
+
+
+def test_pcd_map_through_the_facade(tmp_path):
+    """The Search demo's flow without PCL on the GPU: a 200k-point PCD map
+    loaded into PointXYZINormal records, built, searched; results keep their
+    full records, distances equal brute force, the box is exact."""
+    from test_emu_host import _pcd_map
+    _pcd_map(tmp_path / "map.pcd", 200000, 32)
+    exe = tmp_path / "pcd_payload"
+    r = _build_exe("pcd_payload.cpp", exe)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe), str(tmp_path / "map.pcd")], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.startswith("ok")

@@ -144,6 +144,52 @@ def test_knn_lanes_per_query_invariance(ikd, k, max_dist):
                                       b.view(np.uint32) if b.dtype == np.float32 else b), f"lanes={lanes}"
 
 
+@pytest.mark.parametrize("k,max_dist", [(5, INF), (8, 2.0)])
+def test_knn_lds_staging_invariance(ikd, k, max_dist):
+    """Staging the tree's top levels in LDS (IKD_TUNE_KNN_STAGE 1-12, capped
+    by the LDS budget) returns the same bits as the plain one-lane walk, on a
+    tree with deleted points and deleted subtrees."""
+    rng = np.random.default_rng(340 + k)
+    pts = uniform_cloud(rng, 60000)
+    ids = np.arange(pts.shape[0], dtype=np.uint32)
+    g, o = _pair(ikd, pts, ids)
+    g.Delete_Point_Boxes(random_boxes(rng, 6, smin=5, smax=25))
+    g.Delete_Points(pts[rng.choice(pts.shape[0], 3000, replace=False)])
+    q = uniform_cloud(rng, 3000)
+    g.set_tuning(ikd.IKD_TUNE_LANES_PER_QUERY, 1)
+    ref = g.knn(q, k, max_dist)
+    for levels in (1, 3, 6, 9, 12):
+        g.set_tuning(ikd.IKD_TUNE_KNN_STAGE, levels)
+        out = g.knn(q, k, max_dist)
+        for a_, b_ in zip(out, ref):
+            assert np.array_equal(a_.view(np.uint32) if a_.dtype == np.float32 else a_,
+                                  b_.view(np.uint32) if b_.dtype == np.float32 else b_), f"levels={levels}"
+    live, lid = g.flatten()
+    bd, bi, bc = brute_knn(live, lid, q, k, max_dist, tie="xyz")
+    assert np.array_equal(ref[3], bc)
+    for i in range(q.shape[0]):
+        assert np.array_equal(ref[1][i, :bc[i]].view(np.uint32), bd[i, :bc[i]].view(np.uint32))
+
+
+def test_box_search_order_is_preorder(ikd):
+    """A box holding the whole tree returns the live points in flatten's
+    pre-order (own point, left subtree, right subtree), through the walk and
+    the covered-subtree tasks alike; boxes of every size return exact sets."""
+    rng = np.random.default_rng(350)
+    pts = uniform_cloud(rng, 50000)
+    g, o = _pair(ikd, pts)
+    g.Delete_Point_Boxes(random_boxes(rng, 4, smin=5, smax=30))
+    g.Delete_Points(pts[rng.choice(pts.shape[0], 2000, replace=False)])
+    live, lid = g.flatten()
+    big = np.array([[-1e6, -1e6, -1e6, 1e6, 1e6, 1e6]], np.float32)
+    off, xyz, got_ids = g.box_search_batch(big)
+    assert np.array_equal(got_ids[:off[1]], lid) and np.array_equal(xyz[:off[1]], live)
+    boxes = np.concatenate([random_boxes(rng, 60, smin=1, smax=10), random_boxes(rng, 60, smin=30, smax=90)])
+    off, xyz, _ = g.box_search_batch(boxes)
+    for b in range(boxes.shape[0]):
+        assert_same_multiset(xyz[off[b]:off[b + 1]], live[in_box(live, boxes[b])], f"box {b}")
+
+
 def test_knn_grid_ties(ikd):
     rng = np.random.default_rng(9)
     pts = grid_cloud(rng, 20000, extent=16)  # duplicates and exact distance ties

@@ -131,6 +131,7 @@ def test_budget_counts_resident_maps(ikd, serving, scenes):
         ids = [srv.add_map(t) for t in trees]
         tickets = [srv.submit(ids[m], scans[m][j], 5, 5.0, 1, tag=j) for j in range(1, 4) for m in range(n_maps)]
         srv.drain()
+        srv.wait_idle()
         resident, running = srv.memory()
         srv.close()
         assert all(t.status == 0 for t in tickets)
