@@ -469,6 +469,19 @@ def batching_phase(ikd, torch, dev, scenes, n_trees, seed=1004):
         t0 = time.perf_counter()
         run_batched()
         bt.append(time.perf_counter() - t0)
+    # the same launch recorded into a HIP graph and replayed
+    go_ = tuple(torch.empty_like(x) for x in bo)
+    graph = batch.knn_graph(trees, d_qt, d_q, kmax, *go_, k=d_k, max_dist=d_md)
+    graph.launch()
+    batch.synchronize()
+    gt = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        graph.launch()
+        batch.synchronize()
+        gt.append(time.perf_counter() - t0)
+    graph_same = all(torch.equal(a_, b_) for a_, b_ in zip(go_, bo))
+    graph.close()
 
     # plain: one call per tree and (k, max_dist) group, on each tree's stream
     groups = []
@@ -529,6 +542,8 @@ def batching_phase(ikd, torch, dev, scenes, n_trees, seed=1004):
         "radius_results": int(total),
         "knn_launches_plain": len(groups),
         "batched_knn_s": b_s,
+        "graph_replay_knn_s": float(np.median(gt)),
+        "graph_replay_identical": bool(graph_same),
         "plain_knn_s": p_s,
         "batched_knn_qps": nk / b_s,
         "plain_knn_qps": nk / p_s,

@@ -44,7 +44,8 @@ EXPORTS = (
     "ikd_pool_stats", "ikd_get_counters", "ikd_device_bytes", "ikd_acquire_removed_points", "ikd_set_tuning", "ikd_dump_nodes", "ikd_last_error",
     "ikd_batch_create", "ikd_batch_destroy", "ikd_batch_nearest_search_device", "ikd_batch_synchronize",
     "ikd_batch_stream", "ikd_batch_set_tuning", "ikd_batch_last_error", "ikd_batch_radius_search_device",
-    "ikd_batch_nearest_search_noalloc",
+    "ikd_batch_nearest_search_noalloc", "ikd_batch_knn_graph_create", "ikd_batch_knn_graph_launch",
+    "ikd_batch_knn_graph_destroy",
     "ikd_now_ns", "ikd_server_create", "ikd_server_add_map", "ikd_server_submit", "ikd_server_wait",
     "ikd_server_pending", "ikd_server_destroy", "ikd_server_set_maintenance", "ikd_server_stats",
     "ikd_server_memory",
@@ -134,6 +135,9 @@ def load_library(path: str | None = None):
         "ikd_batch_destroy": [vp],
         "ikd_batch_nearest_search_device": [vp, vp, i32, vp, vp, i64, vp, i32, vp, f64, vp, vp, vp, vp],
         "ikd_batch_nearest_search_noalloc": [vp, vp, i32, vp, vp, i64, vp, i32, vp, f64, vp, vp, vp, vp],
+        "ikd_batch_knn_graph_create": [vp, vp, i32, vp, vp, i64, vp, i32, vp, f64, vp, vp, vp, vp, ctypes.POINTER(vp)],
+        "ikd_batch_knn_graph_launch": [vp],
+        "ikd_batch_knn_graph_destroy": [vp],
         "ikd_batch_synchronize": [vp],
         "ikd_batch_radius_search_device": [vp, vp, i32, vp, vp, vp, i64, vp, vp, vp, i64, vp],
         "ikd_batch_stream": [vp, ctypes.POINTER(vp)],
@@ -424,6 +428,29 @@ class KD_TREE:
         return s.value or 0
 
 
+class KnnGraph:
+    """A multi-tree kNN launch recorded into a HIP graph (Batch.knn_graph)."""
+
+    def __init__(self, batch, h, trees):
+        self._batch = batch
+        self._h = h
+        self._trees = trees  # the trees must outlive the recording
+
+    def launch(self):
+        self._batch._check(self._batch._lib.ikd_batch_knn_graph_launch(self._h))
+
+    def close(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            self._batch._lib.ikd_batch_knn_graph_destroy(h)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Batch:
     """Many trees' kNN queries in one launch (the batched Nearest_Search).
 
@@ -481,6 +508,21 @@ class Batch:
             None if k is None else _tensor_ptr(k), int(k_max), _tensor_ptr(max_dist) if per_q else None,
             float("inf") if per_q else float(max_dist), _tensor_ptr(out_xyz), _tensor_ptr(out_d2),
             _tensor_ptr(out_ids), _tensor_ptr(out_counts)))
+
+    def knn_graph(self, trees, qtree, q, k_max: int, out_xyz, out_d2, out_ids, out_counts, k=None,
+                  max_dist=float("inf")):
+        """Records nearest_search_device with these buffers into a HIP graph;
+        the returned KnnGraph's launch() replays it (refill the buffers between
+        replays as needed)."""
+        handles = (ctypes.c_void_p * len(trees))(*[t._h.value for t in trees])
+        per_q = not isinstance(max_dist, (int, float))
+        g = ctypes.c_void_p()
+        self._check(self._lib.ikd_batch_knn_graph_create(
+            self._h, handles, len(trees), _tensor_ptr(qtree), _tensor_ptr(q), q.shape[0],
+            None if k is None else _tensor_ptr(k), int(k_max), _tensor_ptr(max_dist) if per_q else None,
+            float("inf") if per_q else float(max_dist), _tensor_ptr(out_xyz), _tensor_ptr(out_d2),
+            _tensor_ptr(out_ids), _tensor_ptr(out_counts), ctypes.byref(g)))
+        return KnnGraph(self, g, list(trees))
 
     def radius_search_device(self, trees, qtree, centers, radii, out_offsets, out_xyz=None, out_ids=None):
         """Radius_Search over many trees; returns the total result count.  With

@@ -2554,6 +2554,168 @@ int ikd_batch_nearest_search_noalloc(ikd_batch_t b, const ikd_tree_t* trees, int
     return batch_end(b, trees, ntrees);
 }
 
+// ------------------------------------------------------- recorded kNN launch
+// One multi-tree kNN launch recorded into a HIP graph: the graph holds only
+// kernels (the overflow flag's clear, per-query thresholds, the kNN), reads
+// the tree table from a device buffer of its own, and has its LDS stack fixed
+// at recording.  Each replay refreshes the table from the trees' current node
+// pools (they move when a pool grows), orders itself after the trees' pending
+// work and orders the trees' later work after itself, as the entry points do.
+struct ikd_knn_graph_s {
+    ikd_batch_t b = nullptr;
+    std::vector<ikd_tree_t> trees;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    TreeRef* d_table = nullptr;
+    TreeRef* h_table = nullptr;     // pinned staging of the table refresh
+    hipEvent_t ev_table = nullptr;  // behind the last refresh's copy
+    bool table_pending = false;
+    float* d_thr = nullptr;         // per-query thresholds (per-query max_dist)
+    int* d_ctrl = nullptr;          // persistent-lane query counters
+    int stack = 0;                  // LDS stack entries per lane of the recording
+};
+
+static void knn_graph_free(ikd_knn_graph_t g) {
+    if (!g) return;
+    if (g->table_pending) hipEventSynchronize(g->ev_table);
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    if (g->b && g->b->st) hipStreamSynchronize(g->b->st);  // no replay still reads the buffers
+    if (g->d_table) hipFree(g->d_table);
+    if (g->h_table) hipHostFree(g->h_table);
+    if (g->d_thr) hipFree(g->d_thr);
+    if (g->d_ctrl) hipFree(g->d_ctrl);
+    if (g->ev_table) hipEventDestroy(g->ev_table);
+    delete g;
+}
+
+// Levels a tree may grow by before a replay asks for a new recording.
+constexpr int GRAPH_DEPTH_HEADROOM = 8;
+
+int ikd_batch_knn_graph_create(ikd_batch_t b, const ikd_tree_t* trees, int ntrees, const int32_t* d_query_tree,
+                               const float* d_queries, int64_t nq, const int32_t* d_k, int k_max,
+                               const double* d_max_dist, double max_dist, float* d_out_xyz, float* d_out_d2,
+                               uint32_t* d_out_ids, int32_t* d_out_counts, ikd_knn_graph_t* out) {
+    if (!b || !out) return IKD_ERR_ARG;
+    *out = nullptr;
+    std::lock_guard<std::mutex> batch_lock_(b->mu);
+    auto* t = b;  // HIPCHK reports into b->err
+    if (!trees || ntrees <= 0 || nq <= 0 || nq >= (1ll << 31)) {
+        b->err = "recorded kNN: bad tree list or query count";
+        return IKD_ERR_ARG;
+    }
+    if (k_max < 1 || k_max > 64) {
+        b->err = "recorded kNN: k_max must be in 1..64";
+        return k_max < 1 ? IKD_ERR_ARG : IKD_ERR_UNSUPPORTED;
+    }
+    if (!d_query_tree || !d_queries || !d_out_xyz || !d_out_d2 || !d_out_ids || !d_out_counts) {
+        b->err = "recorded kNN: null device pointer";
+        return IKD_ERR_ARG;
+    }
+    int depth = 0;
+    for (int i = 0; i < ntrees; ++i) {
+        if (!trees[i]) {
+            b->err = "recorded kNN: null tree handle";
+            return IKD_ERR_ARG;
+        }
+        std::lock_guard<std::recursive_mutex> g(trees[i]->mu);
+        if (trees[i]->max_depth + 2 > MAX_DEPTH) {
+            b->err = "recorded kNN: a tree is deeper than the traversal stacks; update it once first";
+            return IKD_ERR_UNSUPPORTED;
+        }
+        depth = std::max(depth, trees[i]->max_depth);
+    }
+    ikd_knn_graph_t g = new ikd_knn_graph_s();
+    g->b = b;
+    g->trees.assign(trees, trees + ntrees);
+    g->stack = std::min(std::max(depth + 2 + GRAPH_DEPTH_HEADROOM, b->knn_stack), MAX_DEPTH);
+    int* flag;
+    if (ensure(b, BB_FLAG, 1, &flag) != IKD_OK || hipMalloc(&g->d_table, ntrees * sizeof(TreeRef)) != hipSuccess ||
+        hipHostMalloc(&g->h_table, ntrees * sizeof(TreeRef), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc(&g->d_ctrl, 8 * sizeof(int)) != hipSuccess ||
+        (d_max_dist && hipMalloc(&g->d_thr, nq * sizeof(float)) != hipSuccess) ||
+        hipEventCreateWithFlags(&g->ev_table, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        knn_graph_free(g);
+        b->err = "recorded kNN: allocation failed";
+        return IKD_ERR_NOMEM;
+    }
+    KnnArgs a{};
+    a.trees = g->d_table;
+    a.qtree = d_query_tree;
+    a.ntrees = ntrees;
+    a.q = d_queries;
+    a.nq = (int)nq;
+    a.k = k_max;
+    a.qk = d_k;
+    a.thr = max_dist_threshold(max_dist);
+    a.qthr = d_max_dist ? g->d_thr : nullptr;
+    a.out_stride = k_max;
+    a.out_xyz = d_out_xyz;
+    a.out_d2 = d_out_d2;
+    a.out_ids = d_out_ids;
+    a.out_counts = d_out_counts;
+    a.overflow = flag;
+    // the visit bound only stops a walk on a corrupt tree; the recording must
+    // outlive the trees' growth
+    a.max_steps = 1 << 30;
+    const int lanes = b->knn_lanes >= 0 ? b->knn_lanes : auto_lanes((int)nq);
+    if (hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        knn_graph_free(g);
+        b->err = "recorded kNN: stream capture could not begin";
+        return IKD_ERR_HIP;
+    }
+    launch_fill_i32(flag, 0, 1, b->st);
+    if (d_max_dist) launch_thresholds(d_max_dist, (int)nq, g->d_thr, b->st);
+    const int lrc = launch_knn(a, k_max, lanes, g->stack, b->st, g->d_ctrl, 256 * 10);
+    const hipError_t le = hipGetLastError();
+    const hipError_t ce = hipStreamEndCapture(b->st, &g->graph);
+    if (lrc != 0 || le != hipSuccess || ce != hipSuccess || !g->graph ||
+        hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        knn_graph_free(g);
+        b->err = lrc != 0 ? "recorded kNN: unsupported launch shape" : "recorded kNN: recording the launch failed";
+        return lrc != 0 ? IKD_ERR_UNSUPPORTED : IKD_ERR_HIP;
+    }
+    *out = g;
+    return IKD_OK;
+}
+
+int ikd_batch_knn_graph_launch(ikd_knn_graph_t g) {
+    if (!g) return IKD_ERR_ARG;
+    ikd_batch_t b = g->b;
+    std::lock_guard<std::mutex> batch_lock_(b->mu);
+    auto* t = b;
+    // the previous refresh's copy has read its staging before it is rewritten
+    if (g->table_pending) HIPCHK(hipEventSynchronize(g->ev_table));
+    g->table_pending = false;
+    const int n = (int)g->trees.size();
+    for (int i = 0; i < n; ++i) {
+        ikd_tree_t tr = g->trees[i];
+        std::lock_guard<std::recursive_mutex> lk(tr->mu);
+        if (tr->max_depth + 2 > g->stack) {
+            b->err = "recorded kNN: a tree grew deeper than the recorded launch allows; record it again";
+            return IKD_ERR_UNSUPPORTED;
+        }
+        g->h_table[i] = TreeRef{tr->pool.hot, tr->pool.ids};
+        HIPCHK(hipEventRecord(tr->ev, tr->st));
+        HIPCHK(hipStreamWaitEvent(b->st, tr->ev, 0));
+    }
+    HIPCHK(hipMemcpyAsync(g->d_table, g->h_table, n * sizeof(TreeRef), hipMemcpyHostToDevice, b->st));
+    HIPCHK(hipEventRecord(g->ev_table, b->st));
+    g->table_pending = true;
+    HIPCHK(hipGraphLaunch(g->exec, b->st));
+    return batch_end(b, g->trees.data(), n);
+}
+
+int ikd_batch_knn_graph_destroy(ikd_knn_graph_t g) {
+    if (!g) return IKD_ERR_ARG;
+    std::lock_guard<std::mutex> batch_lock_(g->b->mu);
+    knn_graph_free(g);
+    return IKD_OK;
+}
+
 int ikd_batch_radius_search_device(ikd_batch_t b, const ikd_tree_t* trees, int ntrees, const int32_t* d_query_tree,
                                    const float* d_centers, const float* d_radii, int64_t nq, int64_t* d_out_offsets,
                                    float* d_out_xyz, uint32_t* d_out_ids, int64_t capacity, int64_t* out_total) {

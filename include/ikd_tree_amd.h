@@ -281,6 +281,25 @@ int ikd_batch_nearest_search_noalloc(ikd_batch_t b, const ikd_tree_t* trees, int
                                      float* d_out_xyz, float* d_out_d2, uint32_t* d_out_ids,
                                      int32_t* d_out_counts);
 
+/* The same launch recorded once into a HIP graph and replayed (no
+ * counterpart in the reference; for fixed serving loops that re-run one query
+ * layout: the replay costs one graph launch instead of the entry point's host
+ * work).  Create records the kNN of ikd_batch_nearest_search_device with
+ * these exact arguments (the device buffers are read and written at every
+ * replay, so the caller refills d_queries etc. between replays).  Launch
+ * replays it on the batch stream, ordered after the trees' pending work and
+ * before their later work, with the tree table refreshed from the trees'
+ * current node pools; IKD_ERR_UNSUPPORTED when a tree has grown more than 8
+ * levels deeper than at the recording (record again then).  Overflow is
+ * reported by ikd_batch_synchronize.  The recording holds only kernels. */
+typedef struct ikd_knn_graph_s* ikd_knn_graph_t;
+int ikd_batch_knn_graph_create(ikd_batch_t b, const ikd_tree_t* trees, int ntrees, const int32_t* d_query_tree,
+                               const float* d_queries, int64_t nq, const int32_t* d_k, int k_max,
+                               const double* d_max_dist, double max_dist, float* d_out_xyz, float* d_out_d2,
+                               uint32_t* d_out_ids, int32_t* d_out_counts, ikd_knn_graph_t* out);
+int ikd_batch_knn_graph_launch(ikd_knn_graph_t g);
+int ikd_batch_knn_graph_destroy(ikd_knn_graph_t g);
+
 /* Radius_Search (ikd_Tree.cpp:406-411) for nq queries spread over trees, in
  * one count launch and one fill launch: query i searches
  * trees[d_query_tree[i]] around d_centers[3i..] with radius d_radii[i] (an

@@ -177,7 +177,52 @@ float emu_shfl_f(float v, int src_lane) {
     return r;
 }
 
+struct emu_node {
+    dim3 grid, block;
+    size_t shmem;
+    std::function<void()> body;
+};
+struct emu_graph {
+    std::vector<emu_node> nodes;
+};
+struct emu_graph_exec {
+    std::vector<emu_node> nodes;
+};
+static thread_local emu_graph* t_capture = nullptr;
+
+hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+    if (t_capture) return hipErrorInvalidValue;
+    t_capture = new emu_graph();
+    return hipSuccess;
+}
+hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) {
+    if (!t_capture) return hipErrorInvalidValue;
+    *g = t_capture;
+    t_capture = nullptr;
+    return hipSuccess;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, void*, char*, size_t) {
+    *e = new emu_graph_exec{g->nodes};
+    return hipSuccess;
+}
+hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+    for (const emu_node& n : e->nodes) emu_launch(n.grid, n.block, n.shmem, n.body);
+    return hipSuccess;
+}
+hipError_t hipGraphExecDestroy(hipGraphExec_t e) {
+    delete e;
+    return hipSuccess;
+}
+hipError_t hipGraphDestroy(hipGraph_t g) {
+    delete g;
+    return hipSuccess;
+}
+
 void emu_launch(dim3 grid, dim3 block, size_t shmem, const std::function<void()>& body) {
+    if (t_capture) {
+        t_capture->nodes.push_back(emu_node{grid, block, shmem, body});
+        return;
+    }
     std::lock_guard<std::mutex> g(g_launch_mu);
     const int n = (int)(block.x * block.y * block.z);
     if (n <= 0 || n > MAX_THREADS) {

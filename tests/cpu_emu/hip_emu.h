@@ -159,5 +159,20 @@ hipError_t hipGetLastError();
 hipError_t hipDeviceSynchronize();
 const char* hipGetErrorString(hipError_t e);
 
+// graphs: a capture on the calling thread records launches (with their
+// arguments) instead of running them; a replay runs them in order
+struct emu_graph;
+struct emu_graph_exec;
+typedef emu_graph* hipGraph_t;
+typedef emu_graph_exec* hipGraphExec_t;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal = 0, hipStreamCaptureModeThreadLocal = 1,
+                            hipStreamCaptureModeRelaxed = 2 };
+hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode mode);
+hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t* g);
+hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, void* error_node, char* log, size_t log_size);
+hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t s);
+hipError_t hipGraphExecDestroy(hipGraphExec_t e);
+hipError_t hipGraphDestroy(hipGraph_t g);
+
 void emu_launch(dim3 grid, dim3 block, size_t shmem, const std::function<void()>& body);
 #define hipLaunchKernelGGL(F, G, B, S, ST, ...) emu_launch(dim3(G), dim3(B), (S), [=]() { F(__VA_ARGS__); })

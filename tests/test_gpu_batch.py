@@ -188,3 +188,50 @@ def test_batch_radius_matches_single_tree_calls(ikd):
     if total > 1:
         with pytest.raises(ikd.IkdError):
             b.radius_search_device(*args, xyz[: total - 1], ids[: total - 1])
+
+
+@pytest.mark.parametrize("per_query", [False, True])
+def test_recorded_launch_replays(ikd, per_query):
+    """The multi-tree launch recorded into a HIP graph (Batch.knn_graph) gives
+    the entry point's results at every replay: the first, one after new
+    queries were written into the recorded buffer, and one after a tree grew
+    (its node pool moved).  Eager work and host synchronizations run between
+    the replays, and every replay is checked."""
+    dev = torch.device(DEV)
+    clouds, qt, q, qk, qd = _mixed_batch(21, [300, 8000, 60000], [500, 1500, 2500])
+    trees = []
+    for c in clouds:
+        t = ikd.KD_TREE(0.5, 0.6, 0.5)
+        t.Build(c)
+        trees.append(t)
+    kmax = 20
+    n = q.shape[0]
+    d_qt, d_q = torch.from_numpy(qt).to(dev), torch.from_numpy(q).to(dev)
+    d_k = torch.from_numpy(qk).to(dev)
+    d_md = torch.from_numpy(qd).to(dev) if per_query else 5.0
+    b = ikd.Batch()
+    rec = _outs(n, kmax, dev)
+    g = b.knn_graph(trees, d_qt, d_q, kmax, *rec, k=d_k, max_dist=d_md)
+    rng = np.random.default_rng(22)
+
+    def check(step):
+        ref = _outs(n, kmax, dev)
+        b.nearest_search_device(trees, d_qt, d_q, kmax, *ref, k=d_k, max_dist=d_md)
+        b.synchronize()
+        for x in rec:
+            x.fill_(-7)  # the replay must write every slot
+        g.launch()
+        b.synchronize()
+        for a_, r_ in zip(rec, ref):
+            assert torch.equal(a_.cpu(), r_.cpu()), f"replay {step}"
+
+    check(0)
+    d_q.copy_(torch.from_numpy(uniform_cloud(rng, n, -32, 32)).to(dev))
+    check(1)
+    big = uniform_cloud(rng, 150000, -30, 30)
+    trees[0].Add_Points(big, False)  # the pool of the 300-point tree grows (and moves)
+    check(2)
+    check(3)
+    g.close()
+    for t in trees:
+        t.close()
