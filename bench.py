@@ -170,7 +170,7 @@ def main():
 
     def tune(q):
         if args.no_tune:
-            return (args.lanes if args.lanes else -1, 0, 0, 0), {}
+            return (args.lanes if args.lanes else -1, 0, 0, 0, 1), {}
         best, ms = autotune.autotune_knn(tree, q, k, args.max_dist)
         return best, ms
 
@@ -180,6 +180,7 @@ def main():
         tree.set_tuning(ikd.IKD_TUNE_KNN_STACK, cfg[1])
         tree.set_tuning(ikd.IKD_TUNE_QUERY_ORDER, cfg[2])
         tree.set_tuning(ikd.IKD_TUNE_KNN_STAGE, cfg[3])
+        tree.set_tuning(ikd.IKD_TUNE_KNN_COMPACT, cfg[4])
         return (lanes,) + tuple(cfg[1:])
 
     def timed_launches(queries, outs, idx):
@@ -249,9 +250,10 @@ def main():
         "launches": len(timed_scans),
         "roofline_achieved_gbs": p_gbs,
         "roofline_frac": p_gbs / HBM_PEAK_GBS,
-        "kernel": knn_kernel_name(k, p_cfg[0], p_cfg[3]),
-        "tuned": {"lanes_per_query": p_cfg[0], "stack": p_cfg[1], "morton": p_cfg[2], "lds_levels": p_cfg[3]},
-        "sweep_ms": {f"lanes{a}_stack{b}_order{c}_lds{d}": v for (a, b, c, d), v in p_sweep.items()},
+        "kernel": knn_kernel_name(k, p_cfg[0], p_cfg[3], p_cfg[4]),
+        "tuned": {"lanes_per_query": p_cfg[0], "stack": p_cfg[1], "morton": p_cfg[2], "lds_levels": p_cfg[3],
+                  "compact_stack": p_cfg[4]},
+        "sweep_ms": {f"lanes{a}_stack{b}_order{c}_lds{d}_compact{e}": v for (a, b, c, d, e), v in p_sweep.items()},
     }
 
     progress("phase A done")
@@ -300,8 +302,10 @@ def main():
             "queries_per_step": nq_total / args.steps,
             "k": k,
             "max_dist_m": args.max_dist,
-            "tuned": {"lanes_per_query": b_cfg[0], "stack": b_cfg[1], "morton": b_cfg[2], "lds_levels": b_cfg[3]},
-            "sweep_ms": {f"lanes{a}_stack{b}_order{c}_lds{d}": v for (a, b, c, d), v in b_sweep.items()},
+            "tuned": {"lanes_per_query": b_cfg[0], "stack": b_cfg[1], "morton": b_cfg[2], "lds_levels": b_cfg[3],
+                      "compact_stack": b_cfg[4]},
+            "sweep_ms": {f"lanes{a}_stack{b}_order{c}_lds{d}_compact{e}": v
+                         for (a, b, c, d, e), v in b_sweep.items()},
         },
         "roofline": {
             "bound": "hbm",
@@ -310,7 +314,7 @@ def main():
             "unit": "GB/s",
             "frac": b_gbs / HBM_PEAK_GBS,
             "traffic": measured_traffic(),
-            "kernel": knn_kernel_name(k, b_cfg[0], b_cfg[3]),
+            "kernel": knn_kernel_name(k, b_cfg[0], b_cfg[3], b_cfg[4]),
             "algorithmic_bytes_per_launch": b_bpl,
             "avg_launch_ms": b_avg * 1e3,
             "bytes_per_query": alg_bytes / nq_total,
@@ -754,10 +758,13 @@ def measured_traffic():
     return None
 
 
-def knn_kernel_name(k, lanes, lds_levels=0):
-    """The kernel symbol a launch shape runs (as rocprofv3 names it)."""
+def knn_kernel_name(k, lanes, lds_levels=0, compact=1):
+    """The kernel symbol a launch shape runs (as rocprofv3 names it); the
+    compact-stack form applies on maps of at most 2^24 node slots."""
     if lanes == 1:
-        return f"ikd::k_knn_staged<{knn_bucket(k)}>" if lds_levels > 0 else f"ikd::k_knn<{knn_bucket(k)}>"
+        if lds_levels > 0:
+            return f"ikd::k_knn_staged<{knn_bucket(k)}>"
+        return f"ikd::k_knn<{knn_bucket(k)}, {'true' if compact else 'false'}>"
     if lanes == 0:
         return f"ikd::k_knn_persist<{knn_bucket(k)}>"
     if lanes < 0:

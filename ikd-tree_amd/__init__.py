@@ -59,6 +59,7 @@ IKD_TUNE_DEFER_REBUILD = 5
 IKD_TUNE_FOLD_LANES = 6
 IKD_TUNE_RECORD_REMOVED = 7
 IKD_TUNE_KNN_STAGE = 8
+IKD_TUNE_KNN_COMPACT = 9
 IKD_NOALLOC_MAX_TREES = 128
 IKD_NOALLOC_MAX_QUERIES = 1 << 20
 

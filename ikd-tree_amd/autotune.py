@@ -4,7 +4,8 @@ Sweeps the lane schedule (0: persistent lanes that fetch queries; 1, 2, 4, 8:
 lanes per query, i.e. 64, 32, 16, 8 queries per wavefront), the per-lane LDS
 stack depth (the tree depth, or 64), the query order (input order or Morton
 order) and, for one lane per query, the tree levels staged in LDS (0: none)
-on a representative device-resident batch, timing each configuration with HIP
+and the LDS stack entry size (4 or 6 bytes) on a representative
+device-resident batch, timing each configuration with HIP
 events on the tree's stream.  Every configuration returns identical results
 (the selection order is total), so the tuner only changes speed;
 ``verify=True`` checks that on the sweep batch.
@@ -17,8 +18,9 @@ import itertools
 def autotune_knn(tree, queries, k: int, max_dist: float = float("inf"), reps: int = 3, verify: bool = True,
                  lanes=(0, 1, 2, 4, 8), stacks=(0, 64), orders=(0, 1), stages=(4, 6, 8)):
     """Time the kNN launch on ``queries`` (a contiguous cuda float32 tensor [n, 3])
-    for every configuration (lanes, stack, order, staged levels), apply the
-    fastest to ``tree`` and return (best_config, {config: ms})."""
+    for every configuration (lanes, stack, order, staged levels, compact
+    stack entries), apply the fastest to ``tree`` and return
+    (best_config, {config: ms})."""
     import torch
     import importlib
 
@@ -32,14 +34,16 @@ def autotune_knn(tree, queries, k: int, max_dist: float = float("inf"), reps: in
     ref = None
     ext = torch.cuda.ExternalStream(tree.stream())
     timings = {}
-    configs = [(ln, st, od, 0) for ln, st, od in itertools.product(lanes, stacks, orders)]
+    configs = [(ln, st, od, 0, 1) for ln, st, od in itertools.product(lanes, stacks, orders)]
     if 1 in lanes:
-        configs += [(1, st, od, lv) for st, od, lv in itertools.product(stacks, orders, stages) if lv > 0]
-    for ln, st, od, lv in configs:
+        configs += [(1, st, od, 0, 0) for st, od in itertools.product(stacks, orders)]
+        configs += [(1, st, od, lv, 0) for st, od, lv in itertools.product(stacks, orders, stages) if lv > 0]
+    for ln, st, od, lv, cp in configs:
         tree.set_tuning(ikd.IKD_TUNE_LANES_PER_QUERY, ln)
         tree.set_tuning(ikd.IKD_TUNE_KNN_STACK, st)
         tree.set_tuning(ikd.IKD_TUNE_QUERY_ORDER, od)
         tree.set_tuning(ikd.IKD_TUNE_KNN_STAGE, lv)
+        tree.set_tuning(ikd.IKD_TUNE_KNN_COMPACT, cp)
         tree.knn_device(queries, k, max_dist, *outs)  # warm
         tree.synchronize()
         if verify:
@@ -47,7 +51,8 @@ def autotune_knn(tree, queries, k: int, max_dist: float = float("inf"), reps: in
             if ref is None:
                 ref = cur
             elif not all(torch.equal(x, y) for x, y in zip(ref, cur)):
-                raise AssertionError(f"kNN results changed under tuning lanes={ln} stack={st} order={od} stage={lv}")
+                raise AssertionError(f"kNN results changed under tuning lanes={ln} stack={st} order={od} stage={lv} "
+                                     f"compact={cp}")
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record(ext)
@@ -55,10 +60,11 @@ def autotune_knn(tree, queries, k: int, max_dist: float = float("inf"), reps: in
             tree.knn_device(queries, k, max_dist, *outs)
         e1.record(ext)
         tree.synchronize()
-        timings[(ln, st, od, lv)] = e0.elapsed_time(e1) / reps
+        timings[(ln, st, od, lv, cp)] = e0.elapsed_time(e1) / reps
     best = min(timings, key=timings.get)
     tree.set_tuning(ikd.IKD_TUNE_LANES_PER_QUERY, best[0])
     tree.set_tuning(ikd.IKD_TUNE_KNN_STACK, best[1])
     tree.set_tuning(ikd.IKD_TUNE_QUERY_ORDER, best[2])
     tree.set_tuning(ikd.IKD_TUNE_KNN_STAGE, best[3])
+    tree.set_tuning(ikd.IKD_TUNE_KNN_COMPACT, best[4])
     return best, timings

@@ -225,6 +225,7 @@ struct ikd_tree_s {
     int small_build = 1;  // build subtrees of <= SMALL_MAX points with one workgroup each
     int fold_lanes = 0;   // lanes per voxel in the downsample fold: 0 picks by map size; else 1, 4, 8, 16
     int knn_stage = 0;    // one-lane kNN: tree levels staged in each workgroup's LDS (0: none)
+    int knn_compact = 1;  // one-lane kNN: 4-byte LDS stack entries when the pool has <= 2^24 slots
     int record_removed = 0;     // 1: rebuilds record the deleted points they drop (acquire_removed_points)
     float4* removed = nullptr;  // that record (x, y, z, id bits), hdr->removed_count entries
     long long removed_cap = 0;
@@ -254,6 +255,7 @@ struct ikd_batch_s {
     std::string err;
     int knn_lanes = -1;
     int knn_stack = 0;
+    int knn_compact = 1;
     int* h_flag = nullptr;  // pinned mirror of the overflow flags (the batch's, the scratch one)
     char* na_scratch = nullptr;  // this device's allocation-free scratch (ikd_batch_nearest_search_noalloc)
     int* na_flag = nullptr;      // its overflow flag, once the entry point ran on this batch
@@ -1460,6 +1462,7 @@ int knn_dev(ikd_tree_t t, const float* d_q, int nq, int k, double max_dist, floa
     a.boxes = d_boxes;
     a.overflow = &t->d_hdr->overflow;
     a.max_steps = (int)std::min<long long>(t->pool_used + 16, INT_MAX);
+    a.compact_stack = (t->knn_compact && t->cap <= (1ll << 24)) ? 1 : 0;
     if (t->query_order && nq >= 4096) {
         unsigned long long *mk, *mk2;
         uint32_t *mv, *mv2;
@@ -2189,6 +2192,9 @@ int ikd_set_tuning(ikd_tree_t t, int key, int value) {
             if (value < 0 || value > 12) return IKD_ERR_ARG;
             t->knn_stage = value;
             return IKD_OK;
+        case IKD_TUNE_KNN_COMPACT:
+            t->knn_compact = value ? 1 : 0;
+            return IKD_OK;
         default:
             return IKD_ERR_ARG;
     }
@@ -2392,6 +2398,9 @@ int ikd_batch_set_tuning(ikd_batch_t b, int key, int value) {
             if (value < -1 || value > 16 || (value > 0 && (value & (value - 1)))) return IKD_ERR_ARG;
             b->knn_lanes = value;
             return IKD_OK;
+        case IKD_TUNE_KNN_COMPACT:
+            b->knn_compact = value ? 1 : 0;
+            return IKD_OK;
         default:
             return IKD_ERR_ARG;
     }
@@ -2425,7 +2434,11 @@ int ikd_batch_nearest_search_device(ikd_batch_t b, const ikd_tree_t* trees, int 
     long long steps;
     CHK(batch_begin(b, trees, ntrees, &depth, &steps));
     std::vector<TreeRef> refs((size_t)ntrees);
-    for (int i = 0; i < ntrees; ++i) refs[i] = TreeRef{trees[i]->pool.hot, trees[i]->pool.ids};
+    bool small_pools = true;
+    for (int i = 0; i < ntrees; ++i) {
+        refs[i] = TreeRef{trees[i]->pool.hot, trees[i]->pool.ids};
+        small_pools = small_pools && trees[i]->cap <= (1ll << 24);
+    }
     TreeRef* d_refs;
     int *next, *flag;
     CHK(ensure(b, BB_TREES, ntrees, &d_refs));
@@ -2434,6 +2447,7 @@ int ikd_batch_nearest_search_device(ikd_batch_t b, const ikd_tree_t* trees, int 
     CHK(stage_copy(b, d_refs, refs.data(), ntrees * sizeof(TreeRef)));
     KnnArgs a{};
     a.trees = d_refs;
+    a.compact_stack = (small_pools && b->knn_compact) ? 1 : 0;
     a.qtree = d_query_tree;
     a.ntrees = ntrees;
     a.q = d_queries;
@@ -2503,6 +2517,7 @@ int ikd_batch_nearest_search_noalloc(ikd_batch_t b, const ikd_tree_t* trees, int
     TreeTable tab;
     int depth = 0;
     long long steps = 16;
+    bool small_pools = true;
     for (int i = 0; i < ntrees; ++i) {
         ikd_tree_t tr = trees[i];
         if (!tr) {
@@ -2517,6 +2532,7 @@ int ikd_batch_nearest_search_noalloc(ikd_batch_t b, const ikd_tree_t* trees, int
         }
         depth = std::max(depth, tr->max_depth);
         steps = std::max(steps, tr->pool_used + 16);
+        small_pools = small_pools && tr->cap <= (1ll << 24);
         tab.t[i] = TreeRef{tr->pool.hot, tr->pool.ids};
         HIPCHK(hipEventRecord(tr->ev, tr->st));
         HIPCHK(hipStreamWaitEvent(b->st, tr->ev, 0));
@@ -2525,6 +2541,7 @@ int ikd_batch_nearest_search_noalloc(ikd_batch_t b, const ikd_tree_t* trees, int
     launch_fill_i32(ctrl + 8, 0, 1, b->st);
     KnnArgs a{};
     a.trees = table;
+    a.compact_stack = (small_pools && b->knn_compact) ? 1 : 0;
     a.qtree = d_query_tree;
     a.ntrees = ntrees;
     a.q = d_queries;

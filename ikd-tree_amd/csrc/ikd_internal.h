@@ -214,6 +214,9 @@ struct KnnArgs {
     const NodeHot* top;
     const int* top_idx;
     int top_slots;
+    // one-lane launches: 4-byte LDS stack entries (every tree of the launch
+    // has at most 2^24 node slots)
+    int compact_stack;
 };
 // Copies the tree's first `levels` levels into a complete binary layout (slot 0
 // the root, slot s's children at 2s+1 and 2s+2): top[s] the node's record,

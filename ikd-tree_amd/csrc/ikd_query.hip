@@ -229,9 +229,25 @@ static inline int xcd_grid(int nchunks) {
     return per * 8;
 }
 
-template <int KM>
+// Compact stacks (trees of at most 2^24 nodes): one 4-byte entry per
+// deferred child, the node index in the upper 24 bits and the box distance's
+// exponent bits below, i.e. the largest power of two not above it (0 for 0
+// and subnormals).  That lower bound admits a popped entry whenever the exact
+// distance would, and at most one visit more than the 16-bit key (the visit
+// evaluates the children's boxes exactly, and a child's box lies inside the
+// node's).  Two thirds of the LDS per lane keep more wavefronts resident.
+constexpr int COMPACT_NODE_BITS = 24;
+__device__ __forceinline__ uint32_t compact_entry(int n, float d) {
+    return ((uint32_t)n << 8) | (fbits(d) >> 23);
+}
+__device__ __forceinline__ float compact_dist(uint32_t e) { return bitsf((e & 0xffu) << 23); }
+static inline size_t lds_compact_bytes(int stack) { return (size_t)stack * 64 * sizeof(uint32_t); }
+
+template <int KM, bool COMPACT>
 __global__ __launch_bounds__(64) void k_knn(KnnArgs a) {
     const LdsStack st = lds_stack(a.stack);
+    extern __shared__ int lds_dyn[];
+    uint32_t* const cst = reinterpret_cast<uint32_t*>(lds_dyn);
     const int lane = threadIdx.x;
     const int chunk = xcd_chunk(blockIdx.x, (a.nq + 63) / 64);
     const int gq = chunk * 64 + lane;
@@ -262,8 +278,12 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs a) {
             const float top = best.d[0];
             if (fn >= 0 && df <= top && df <= c.thr) {
                 if (sp < a.stack) {
-                    st.n[sp * 64 + lane] = fn;
-                    st.d[sp * 64 + lane] = dist_key(df);
+                    if (COMPACT) {
+                        cst[sp * 64 + lane] = compact_entry(fn, df);
+                    } else {
+                        st.n[sp * 64 + lane] = fn;
+                        st.d[sp * 64 + lane] = dist_key(df);
+                    }
                     ++sp;
                 } else {
                     atomicOr(a.overflow, 1);
@@ -279,11 +299,21 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs a) {
         n = -1;
         while (sp > 0) {
             --sp;
-            const float d = dist_val(st.d[sp * 64 + lane]);
-            if (d <= best.d[0]) {
-                n = st.n[sp * 64 + lane];
-                bd = d;
-                break;
+            if (COMPACT) {
+                const uint32_t e = cst[sp * 64 + lane];
+                const float d = compact_dist(e);
+                if (d <= best.d[0]) {
+                    n = (int)(e >> 8);
+                    bd = d;
+                    break;
+                }
+            } else {
+                const float d = dist_val(st.d[sp * 64 + lane]);
+                if (d <= best.d[0]) {
+                    n = st.n[sp * 64 + lane];
+                    bd = d;
+                    break;
+                }
             }
         }
         if (n < 0) break;
@@ -731,7 +761,12 @@ static int launch_knn_km(const KnnArgs& a0, int lanes, int stack, hipStream_t s,
                 hipLaunchKernelGGL((k_knn_staged<KM>), dim3(xcd_grid(blocks_for(a.nq, 64))), block, lds_t, s, a);
                 return 0;
             }
-            hipLaunchKernelGGL((k_knn<KM>), dim3(xcd_grid(blocks_for(a.nq, 64))), block, lds, s, a);
+            if (a.compact_stack) {
+                hipLaunchKernelGGL((k_knn<KM, true>), dim3(xcd_grid(blocks_for(a.nq, 64))), block,
+                                   lds_compact_bytes(stack), s, a);
+                return 0;
+            }
+            hipLaunchKernelGGL((k_knn<KM, false>), dim3(xcd_grid(blocks_for(a.nq, 64))), block, lds, s, a);
             return 0;
         case 0: {  // persistent lanes; the 8 per-XCD query counters were zeroed by the caller
             int waves = std::min(blocks_for(a.nq, 64), persist_waves);

@@ -210,6 +210,12 @@ int ikd_device_bytes(ikd_tree_t t, int64_t* out_bytes);
  *                       from there (capped so that a workgroup's LDS stays
  *                       within 64 KiB); the results are the same. */
 #define IKD_TUNE_KNN_STAGE 8
+/*  IKD_TUNE_KNN_COMPACT 1 (default): one-lane kNN launches over trees of at
+ *                       most 2^24 node slots keep 4-byte traversal stack
+ *                       entries in LDS (more wavefronts resident); 0: the
+ *                       6-byte entries.  Results are the same.  Also a
+ *                       batch key. */
+#define IKD_TUNE_KNN_COMPACT 9
 int ikd_set_tuning(ikd_tree_t t, int key, int value);
 
 /* Run the deferred rebuilds that are still due, in the order they were

@@ -171,6 +171,33 @@ def test_knn_lds_staging_invariance(ikd, k, max_dist):
         assert np.array_equal(ref[1][i, :bc[i]].view(np.uint32), bd[i, :bc[i]].view(np.uint32))
 
 
+@pytest.mark.parametrize("k,max_dist", [(1, INF), (5, 5.0), (20, INF)])
+def test_knn_compact_stack_invariance(ikd, k, max_dist):
+    """The 4-byte LDS stack entries (power-of-two distance bound) return the
+    same bits as the 6-byte ones, one lane per query, on a tree with deleted
+    points and subtrees and with exact distance ties."""
+    rng = np.random.default_rng(360 + k)
+    pts = np.concatenate([uniform_cloud(rng, 50000), grid_cloud(rng, 20000, extent=16)])
+    ids = np.arange(pts.shape[0], dtype=np.uint32)
+    g, o = _pair(ikd, pts, ids)
+    g.Delete_Point_Boxes(random_boxes(rng, 6, smin=5, smax=25))
+    g.Delete_Points(pts[rng.choice(pts.shape[0], 3000, replace=False)])
+    q = np.concatenate([uniform_cloud(rng, 2500), grid_cloud(rng, 500, extent=16) + np.float32(0.125)])
+    g.set_tuning(ikd.IKD_TUNE_LANES_PER_QUERY, 1)
+    out = []
+    for compact in (0, 1):
+        g.set_tuning(ikd.IKD_TUNE_KNN_COMPACT, compact)
+        out.append(g.knn(q, k, max_dist))
+    for a_, b_ in zip(*out):
+        assert np.array_equal(a_.view(np.uint32) if a_.dtype == np.float32 else a_,
+                              b_.view(np.uint32) if b_.dtype == np.float32 else b_)
+    live, lid = g.flatten()
+    bd, bi, bc = brute_knn(live, lid, q, k, max_dist, tie="xyz")
+    assert np.array_equal(out[1][3], bc)
+    for i in range(q.shape[0]):
+        assert np.array_equal(out[1][1][i, :bc[i]].view(np.uint32), bd[i, :bc[i]].view(np.uint32))
+
+
 def test_box_search_order_is_preorder(ikd):
     """A box holding the whole tree returns the live points in flatten's
     pre-order (own point, left subtree, right subtree), through the walk and
