@@ -23,6 +23,7 @@
 // passes thr = the largest float <= max_dist*max_dist, for which
 // (double)d <= max_dist*max_dist  <=>  d <= thr  for every float d.
 #include <algorithm>
+#include <atomic>
 
 #include "ikd_internal.h"
 
@@ -658,9 +659,14 @@ __global__ __launch_bounds__(64) void k_knn_team(KnnArgs a) {
 // b % 8) walks the batch; a lane that finishes its query takes the next one
 // of its XCD's contiguous eighth of the batch.  Idle lanes of a wavefront
 // claim their queries with one atomic per wavefront.
-template <int KM>
+// idle lanes a persistent wavefront collects before it takes new queries
+constexpr int PERSIST_REFILL = 16;
+
+template <int KM, bool COMPACT>
 __global__ __launch_bounds__(64) void k_knn_persist(KnnArgs a, int* __restrict__ next) {
     const LdsStack st = lds_stack(a.stack);
+    extern __shared__ int lds_dyn[];
+    uint32_t* const cst = reinterpret_cast<uint32_t*>(lds_dyn);
     const int lane = threadIdx.x;
     const unsigned long long below = (1ull << lane) - 1ull;
     const int xcd = blockIdx.x % NUM_XCD;
@@ -674,8 +680,12 @@ __global__ __launch_bounds__(64) void k_knn_persist(KnnArgs a, int* __restrict__
     unsigned int visits = 0, boxes = 0;
     bool drained = false;
     while (true) {
+        // refill idle lanes in bunches (one counter update per bunch), and
+        // when every lane is idle
         const unsigned long long imask = __ballot(!has);
-        if (imask != 0 && !drained) {
+        const int idle = __popcll(imask);
+        if (idle == 64 && drained) break;
+        if (!drained && (idle >= PERSIST_REFILL || idle == 64)) {
             const int leader = __ffsll((long long)imask) - 1;
             int base = 0;
             if (lane == leader) base = atomicAdd(&next[xcd], __popcll(imask));
@@ -698,7 +708,6 @@ __global__ __launch_bounds__(64) void k_knn_persist(KnnArgs a, int* __restrict__
                 }
             }
         }
-        if (__ballot(has) == 0) break;
         if (!has) continue;
         if ((int)visits > a.max_steps) {
             atomicOr(a.overflow, 2);
@@ -713,8 +722,12 @@ __global__ __launch_bounds__(64) void k_knn_persist(KnnArgs a, int* __restrict__
             const float top = best.d[0];
             if (fn >= 0 && df <= top && df <= c.thr) {
                 if (sp < a.stack) {
-                    st.n[sp * 64 + lane] = fn;
-                    st.d[sp * 64 + lane] = dist_key(df);
+                    if (COMPACT) {
+                        cst[sp * 64 + lane] = compact_entry(fn, df);
+                    } else {
+                        st.n[sp * 64 + lane] = fn;
+                        st.d[sp * 64 + lane] = dist_key(df);
+                    }
                     ++sp;
                 } else {
                     atomicOr(a.overflow, 1);
@@ -729,17 +742,40 @@ __global__ __launch_bounds__(64) void k_knn_persist(KnnArgs a, int* __restrict__
         n = -1;
         while (sp > 0) {
             --sp;
-            const float d = dist_val(st.d[sp * 64 + lane]);
-            if (d <= best.d[0]) {
-                n = st.n[sp * 64 + lane];
-                bd = d;
-                break;
+            if (COMPACT) {
+                const uint32_t e = cst[sp * 64 + lane];
+                const float d = compact_dist(e);
+                if (d <= best.d[0]) {
+                    n = (int)(e >> 8);
+                    bd = d;
+                    break;
+                }
+            } else {
+                const float d = dist_val(st.d[sp * 64 + lane]);
+                if (d <= best.d[0]) {
+                    n = st.n[sp * 64 + lane];
+                    bd = d;
+                    break;
+                }
             }
         }
         if (n >= 0) continue;
         finish_query<KM>(a, c, best, visits, boxes);
         has = false;
     }
+}
+
+// compute units of the current device (cached per device)
+static int device_cus() {
+    static std::atomic<int> cus[16];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) return 256;
+    int v = cus[dev].load(std::memory_order_relaxed);
+    if (v <= 0) {
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+        cus[dev].store(v, std::memory_order_relaxed);
+    }
+    return v;
 }
 
 template <int KM>
@@ -769,8 +805,14 @@ static int launch_knn_km(const KnnArgs& a0, int lanes, int stack, hipStream_t s,
             hipLaunchKernelGGL((k_knn<KM, false>), dim3(xcd_grid(blocks_for(a.nq, 64))), block, lds, s, a);
             return 0;
         case 0: {  // persistent lanes; the 8 per-XCD query counters were zeroed by the caller
-            int waves = std::min(blocks_for(a.nq, 64), persist_waves);
-            hipLaunchKernelGGL((k_knn_persist<KM>), dim3(xcd_grid(waves)), block, lds, s, a, persist_next);
+            // as many single-wavefront workgroups as the LDS stacks let each CU hold
+            const size_t lb = a.compact_stack ? lds_compact_bytes(stack) : lds;
+            const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(32, (160u << 10) / std::max<size_t>(lb, 1)));
+            int waves = std::min(blocks_for(a.nq, 64), std::max(persist_waves, device_cus() * per_cu));
+            if (a.compact_stack)
+                hipLaunchKernelGGL((k_knn_persist<KM, true>), dim3(xcd_grid(waves)), block, lb, s, a, persist_next);
+            else
+                hipLaunchKernelGGL((k_knn_persist<KM, false>), dim3(xcd_grid(waves)), block, lb, s, a, persist_next);
             return 0;
         }
         case 2:

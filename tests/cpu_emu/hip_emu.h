@@ -143,6 +143,8 @@ hipError_t hipMallocFromPoolAsync(void** p, size_t n, hipMemPool_t mp, hipStream
 hipError_t hipFreeAsync(void* p, hipStream_t st);
 hipError_t hipMallocAsync(void** p, size_t n, hipStream_t st);
 hipError_t hipGetDevice(int* dev);
+enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 1 };
+hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t attr, int dev);
 hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind k);
 hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t st);
 hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t st);
