@@ -79,7 +79,7 @@ def main():
     ap.add_argument("--robots", type=int, default=16, help="maps in the serving phase (0 skips it)")
     ap.add_argument("--box-points", type=int, default=5_000_000,
                     help="map points of the box/radius/trim phase (0 skips it)")
-    ap.add_argument("--serve-seconds", type=float, default=5.0, help="simulated arrival window of the serving phase")
+    ap.add_argument("--serve-seconds", type=float, default=30.0, help="simulated arrival window of the serving phase")
     ap.add_argument("--serve-defer", type=int, default=1_000,
                     help="serving: subtrees above this many live points are rebuilt between requests")
     ap.add_argument("--serve-slots", type=int, default=8, help="serving: requests on the device at once")

@@ -26,6 +26,7 @@ namespace ikd {
 // segment belongs to the other shape.
 constexpr int MICRO_MAX = 8;
 constexpr int TINY_MAX = 128;
+constexpr int SMALL_SB = 256;  // threads of the workgroup build of segments above TINY_MAX points
 
 // Nodes with at least one child in the median-split tree of m points.  A node
 // of size x >= 2 leaves children of sizes (x-1)>>1 and x-1-((x-1)>>1), so a
@@ -547,7 +548,7 @@ void launch_build_small(const float4* pts, const int4* seg, const int* seg_depth
     // wide enough to fill the device when a list is long
     hipLaunchKernelGGL((k_build_small<64, TINY_MAX>), dim3(std::min(S, 2048)), dim3(64), 0, s, pts, seg, seg_depth,
                        pair_scan, lists, &hdr->small_lists[0], orig, pre, pool, hdr);
-    hipLaunchKernelGGL((k_build_small<256, SMALL_MAX>), dim3(std::min(S, 512)), dim3(256), 0, s, pts, seg,
+    hipLaunchKernelGGL((k_build_small<SMALL_SB, SMALL_MAX>), dim3(std::min(S, 512)), dim3(SMALL_SB), 0, s, pts, seg,
                        seg_depth, pair_scan, lists + S, &hdr->small_lists[1], orig, pre, pool, hdr);
 }
 void launch_small_commit(const int* pairs, const int* pair_scan, int S, DevHdr* hdr, hipStream_t s) {
