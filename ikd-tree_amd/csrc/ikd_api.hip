@@ -275,7 +275,7 @@ enum Buf {
     B_PFLAG, B_PSCAN, B_ISEG, B_ISEGD, B_IO0, B_IO1, B_IO2, B_IO3, B_IO4, B_VSTART, B_DELBOX, B_DELFL, B_INSPT,
     B_INSFL, B_CBOX, B_CPTS, B_MISC, B_MORTK, B_MORTV, B_MORTK2, B_MORTV2, B_CNT64, B_OFF64, B_ADDX, B_ADDI, B_PNEXT, B_SPAIRS, B_SPSCAN,
     B_DEFER, B_DINFO, B_DBEFORE, B_DKEEP, B_PTS3, B_PEND, B_START, B_INV0, B_LASTP, B_PSEED, B_DBELOW, B_MAXDEL,
-    B_SLISTS, B_ENUMROOT, B_ENUMGRP, B_TOPT, B_TOPI, B_TCNT64, B_TOFF64, B_RTASKS, B_COUNT
+    B_SLISTS, B_ENUMROOT, B_ENUMGRP, B_TOPT, B_TOPI, B_TCNT64, B_TOFF64, B_RTASKS, B_ICNT64, B_COUNT
 };
 static_assert(B_COUNT <= 96, "scratch table");
 
@@ -1505,6 +1505,50 @@ int knn_dev(ikd_tree_t t, const float* d_q, int nq, int k, double max_dist, floa
     return IKD_OK;
 }
 
+// Range search plan on the owner's stream (count walk, item lists, item
+// counts, offsets): d_offsets (device, nq + 1) gets every query's first
+// output position; *total and *ntask come back to the host (two waits: the
+// item count sizes the lists, the total sizes the output).  slots: scratch
+// slots for the per-query item counts and offsets, the items, the item
+// counts and the item offsets.
+hipError_t owner_wait(ikd_tree_t t) { return wait_stream(t); }
+template <class Owner>
+hipError_t owner_wait(Owner* b) { return hipStreamSynchronize(b->st); }
+
+template <class Owner>
+int range_plan(Owner* t, RangeArgs& a, int nq, long long* d_offsets, const int (&slots)[5], long long pool_bound,
+               long long* total, long long* ntask_out) {
+    hipStream_t st = t->st;
+    long long *tcnt, *toff, *icnt, *ioff;
+    int4* items;
+    CHK(ensure(t, slots[0], nq + 1, &tcnt));
+    CHK(ensure(t, slots[1], nq + 1, &toff));
+    launch_fill_i32(reinterpret_cast<int*>(tcnt + nq), 0, 2, st);  // the scan's trailing zero
+    a.task_nodes = RANGE_TASK_NODES;
+    a.tcounts = tcnt;
+    launch_range_top(a, 0, st);
+    CHK(exscan_i64(t, tcnt, toff, nq + 1));
+    long long ntask = 0;
+    HIPCHK(hipMemcpyAsync(&ntask, toff + nq, sizeof(long long), hipMemcpyDeviceToHost, st));
+    HIPCHK(owner_wait(t));
+    CHK(check_count(t, "range search: items", ntask, 0, std::min<long long>((long long)nq * pool_bound, INT_MAX - 1)));
+    CHK(ensure(t, slots[2], std::max<long long>(ntask, 1), &items));
+    CHK(ensure(t, slots[3], ntask + 1, &icnt));
+    CHK(ensure(t, slots[4], ntask + 1, &ioff));
+    a.toffsets = toff;
+    a.tasks = items;
+    launch_range_top(a, 1, st);
+    launch_fill_i32(reinterpret_cast<int*>(icnt + ntask), 0, 2, st);
+    launch_range_item_count(a, ntask, icnt, st);
+    CHK(exscan_i64(t, icnt, ioff, (int)ntask + 1));
+    launch_range_query_offsets(toff, ioff, nq, d_offsets, st);
+    a.item_offsets = ioff;
+    HIPCHK(hipMemcpyAsync(total, d_offsets + nq, sizeof(long long), hipMemcpyDeviceToHost, st));
+    HIPCHK(owner_wait(t));
+    *ntask_out = ntask;
+    return IKD_OK;
+}
+
 int range_dev(ikd_tree_t t, int mode, const float* d_boxes, const float* d_centers, const float* d_radii, int nq,
               int64_t* h_offsets, float* h_xyz, uint32_t* h_ids, int64_t capacity) {
     if (t->max_depth + 2 > MAX_DEPTH) {
@@ -1512,16 +1556,9 @@ int range_dev(ikd_tree_t t, int mode, const float* d_boxes, const float* d_cente
         CHK(compact_if_needed(t, true));
     }
     hipStream_t st = t->st;
-    long long *cnt, *off, *tcnt, *toff;
-    CHK(ensure(t, B_CNT64, nq + 1, &cnt));
+    long long* off;
     CHK(ensure(t, B_OFF64, nq + 1, &off));
-    CHK(ensure(t, B_TCNT64, nq + 1, &tcnt));
-    CHK(ensure(t, B_TOFF64, nq + 1, &toff));
-    launch_fill_i32(reinterpret_cast<int*>(cnt + nq), 0, 2, st);  // the scans' trailing zeros
-    launch_fill_i32(reinterpret_cast<int*>(tcnt + nq), 0, 2, st);
     RangeArgs a{};
-    a.task_nodes = RANGE_TASK_NODES;
-    a.tcounts = tcnt;
     a.max_steps = t->pool_used + 16;
     a.pool = t->pool;
     a.boxes = d_boxes;
@@ -1529,19 +1566,14 @@ int range_dev(ikd_tree_t t, int mode, const float* d_boxes, const float* d_cente
     a.radii = d_radii;
     a.nq = nq;
     a.mode = mode;
-    a.counts = cnt;
     a.overflow = &t->d_hdr->overflow;
     a.stack = std::min(t->max_depth + 2, MAX_DEPTH);
-    launch_range(a, 0, st);
-    CHK(exscan_i64(t, cnt, off, nq + 1));
-    CHK(exscan_i64(t, tcnt, toff, nq + 1));
+    long long total = 0, ntask = 0;
+    static const int slots[5] = {B_TCNT64, B_TOFF64, B_RTASKS, B_CNT64, B_ICNT64};
+    CHK(range_plan(t, a, nq, off, slots, std::max<long long>(t->pool_used, 1), &total, &ntask));
     HIPCHK(hipMemcpyAsync(h_offsets, off, (nq + 1) * sizeof(long long), hipMemcpyDeviceToHost, st));
-    long long ntask = 0;
-    HIPCHK(hipMemcpyAsync(&ntask, toff + nq, sizeof(long long), hipMemcpyDeviceToHost, st));
     CHK(sync_hdr(t));
-    long long total = h_offsets[nq];
     CHK(check_count(t, "range search: results", total, 0, (long long)nq * std::max<long long>(t->pool_used, 1)));
-    CHK(check_count(t, "range search: covered subtrees", ntask, 0, total));
     if (!h_xyz) return IKD_OK;
     if (capacity < total) {
         t->err = "output capacity below the result count";
@@ -1552,15 +1584,9 @@ int range_dev(ikd_tree_t t, int mode, const float* d_boxes, const float* d_cente
     uint32_t* di;
     CHK(ensure(t, B_IO0, 3 * total, &dx));
     CHK(ensure(t, B_IO1, total, &di));
-    int4* tasks;
-    CHK(ensure(t, B_RTASKS, std::max<long long>(ntask, 1), &tasks));
-    a.offsets = off;
-    a.toffsets = toff;
-    a.tasks = tasks;
     a.out_xyz = dx;
     a.out_ids = di;
-    launch_range(a, 1, st);
-    launch_range_tasks(a, ntask, st);
+    launch_range_item_fill(a, ntask, a.item_offsets, st);
     HIPCHK(hipMemcpyAsync(h_xyz, dx, 3 * total * sizeof(float), hipMemcpyDeviceToHost, st));
     if (h_ids) HIPCHK(hipMemcpyAsync(h_ids, di, total * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     return sync_hdr(t);
@@ -2257,7 +2283,7 @@ const char* ikd_last_error(ikd_tree_t t) {
 
 // ------------------------------------------------------------ multi-tree batches
 enum BatchBuf { BB_TMP = 0, BB_TREES, BB_THR, BB_NEXT, BB_FLAG, BB_CNT, BB_POOLS, BB_TCNT, BB_TOFF, BB_TASKS,
-                BB_COUNT };
+                BB_ICNT, BB_COUNT };
 static_assert(BB_COUNT <= 16, "batch scratch table");
 
 // ------------------------------------- allocation-free multi-tree kNN scratch
@@ -2755,20 +2781,11 @@ int ikd_batch_radius_search_device(ikd_batch_t b, const ikd_tree_t* trees, int n
     std::vector<Pool> pools((size_t)ntrees);
     for (int i = 0; i < ntrees; ++i) pools[i] = trees[i]->pool;
     Pool* d_pools;
-    long long* cnt;
     int* flag;
-    long long *tcnt, *toff;
     CHK(ensure(b, BB_POOLS, ntrees, &d_pools));
-    CHK(ensure(b, BB_CNT, nq + 1, &cnt));
-    CHK(ensure(b, BB_TCNT, nq + 1, &tcnt));
-    CHK(ensure(b, BB_TOFF, nq + 1, &toff));
     CHK(ensure(b, BB_FLAG, 1, &flag));
     CHK(stage_copy(b, d_pools, pools.data(), ntrees * sizeof(Pool)));
-    launch_fill_i32(reinterpret_cast<int*>(cnt + nq), 0, 2, b->st);
-    launch_fill_i32(reinterpret_cast<int*>(tcnt + nq), 0, 2, b->st);
     RangeArgs a{};
-    a.task_nodes = RANGE_TASK_NODES;
-    a.tcounts = tcnt;
     a.pools = d_pools;
     a.qtree = d_query_tree;
     a.ntrees = ntrees;
@@ -2776,32 +2793,22 @@ int ikd_batch_radius_search_device(ikd_batch_t b, const ikd_tree_t* trees, int n
     a.radii = d_radii;
     a.nq = (int)nq;
     a.mode = 1;
-    a.counts = cnt;
     a.overflow = flag;
     a.max_steps = steps;
     a.stack = std::min(depth + 2, MAX_DEPTH);
-    launch_range(a, 0, b->st);
-    CHK(exscan_i64(b, cnt, reinterpret_cast<long long*>(d_out_offsets), (int)nq + 1));
-    CHK(exscan_i64(b, tcnt, toff, (int)nq + 1));
     long long total = 0, ntask = 0;
-    HIPCHK(hipMemcpyAsync(&total, d_out_offsets + nq, sizeof(long long), hipMemcpyDeviceToHost, b->st));
-    HIPCHK(hipMemcpyAsync(&ntask, toff + nq, sizeof(long long), hipMemcpyDeviceToHost, b->st));
-    HIPCHK(hipStreamSynchronize(b->st));
+    static const int slots[5] = {BB_TCNT, BB_TOFF, BB_TASKS, BB_CNT, BB_ICNT};
+    CHK(range_plan(b, a, (int)nq, reinterpret_cast<long long*>(d_out_offsets), slots, std::max<long long>(steps, 1),
+                   &total, &ntask));
     *out_total = total;
-    if (total < 0 || ntask < 0 || ntask > total) {
-        b->err = "batch: device result counts out of range";
+    if (total < 0) {
+        b->err = "batch: device result count out of range";
         return IKD_ERR_INTERNAL;
     }
     if (d_out_xyz && d_out_ids && capacity >= total && total > 0) {
-        int4* tasks;
-        CHK(ensure(b, BB_TASKS, std::max<long long>(ntask, 1), &tasks));
-        a.offsets = reinterpret_cast<const long long*>(d_out_offsets);
-        a.toffsets = toff;
-        a.tasks = tasks;
         a.out_xyz = d_out_xyz;
         a.out_ids = d_out_ids;
-        launch_range(a, 1, b->st);
-        launch_range_tasks(a, ntask, b->st);
+        launch_range_item_fill(a, ntask, a.item_offsets, b->st);
         HIPCHK(hipGetLastError());
     }
     CHK(batch_end(b, trees, ntrees));

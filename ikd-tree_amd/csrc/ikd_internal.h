@@ -241,8 +241,6 @@ struct RangeArgs {
     const float* radii;
     int nq;
     int mode;             // 0 box, 1 radius
-    const long long* offsets;  // fill pass: output offsets (nq+1)
-    long long* counts;    // count pass output
     float* out_xyz;
     uint32_t* out_ids;
     int* overflow;
@@ -251,17 +249,24 @@ struct RangeArgs {
     const Pool* pools;    // multi-tree launch: pool table with qtree[q] per query (else null)
     const int* qtree;
     int ntrees;           // a query naming no valid tree finds nothing
-    // Covered-subtree tasks (task_nodes > 0): the walks stop at subtrees the
-    // query covers whole with at most task_nodes nodes; the count pass counts
-    // them per query (tcounts), the fill pass reserves each one's output range
-    // in pre-order and lists it (tasks at toffsets), launch_range_tasks writes them.
+    // items: the walks stop at subtrees of at most task_nodes nodes; a
+    // query's items (own points of walked nodes, covered and cut subtrees)
+    // are counted per query (tcounts), listed from toffsets[q] (tasks:
+    // node, query, kind) and written from item_offsets[i]
     int task_nodes;
     long long* tcounts;
     const long long* toffsets;
-    int4* tasks;          // (node, query, output offset low, high)
+    int4* tasks;
+    const long long* item_offsets;
 };
-void launch_range(const RangeArgs& a, int fill, hipStream_t s);
-void launch_range_tasks(const RangeArgs& a, long long ntask, hipStream_t s);
+// list 0: tcounts[q] = items of query q; list 1: the items at toffsets[q]
+void launch_range_top(const RangeArgs& a, int list, hipStream_t s);
+// counts[i] = points of item i
+void launch_range_item_count(const RangeArgs& a, long long ntask, long long* counts, hipStream_t s);
+// offsets[q] = item_off[toffsets[q]] for q in [0, nq]
+void launch_range_query_offsets(const long long* toffsets, const long long* item_off, int nq, long long* offsets,
+                                hipStream_t s);
+void launch_range_item_fill(const RangeArgs& a, long long ntask, const long long* item_off, hipStream_t s);
 
 // ---------------------------------------------------------------- misc
 void launch_fill_i32(int* p, int v, long long n, hipStream_t s);

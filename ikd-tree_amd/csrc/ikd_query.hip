@@ -936,14 +936,30 @@ __device__ __forceinline__ int classify(const RangeArgs& a, int qi, const float*
     return 2;
 }
 
-// One lane per query, 64 per workgroup; the depth-first stack lives in
-// dynamic LDS as [stack][64] entries (node index, top bit = whole subtree).
-// Each pop pushes at most its two children, so depth + 2 entries suffice.
-// Stack entries: node index (< 2^28) with bit 31 = covered whole, bit 30 = a
-// covered-subtree task (handled when popped, so tasks keep pre-order).
+// Range search in items.  A walk per query (one lane each, depth-first
+// stack in LDS as [stack][64] entries) goes down only through subtrees of
+// more than task_nodes nodes; everything at or below that size, and the own
+// point of every node it walks, becomes an item of the query's list, in
+// pre-order:
+//   RANGE_OWN   the node's own point (the walk checked the predicate),
+//   RANGE_COVER a subtree the query covers whole,
+//   RANGE_PART  a subtree the query cuts.
+// The count walk (k_range_top, list 0) counts each query's items, the list
+// walk (list 1) writes them; one lane per item then counts its points
+// (k_range_item_count: 1, the subtree's live count, or a walk of the cut
+// subtree with covered parts counted from their stored counts), a scan over
+// the items gives every item its output offset and every query its first,
+// and one lane per item writes its points (k_range_item_fill) in pre-order.
+// The output is the pre-order sequence of one walk per query.
+//
+// Stack entries: node index (< 2^28) with bit 31 = covered whole, bit 30 =
+// an item left for its pop (so items keep pre-order), bit 29 = the item is
+// a cut subtree.
 constexpr int RANGE_WHOLE = (int)0x80000000;
 constexpr int RANGE_TASK = 0x40000000;
-constexpr int RANGE_NODE = 0x3fffffff;
+constexpr int RANGE_CUT = 0x20000000;
+constexpr int RANGE_NODE = 0x1fffffff;
+constexpr int RANGE_OWN = 0, RANGE_COVER = 1, RANGE_PART = 2;
 
 __device__ __forceinline__ Pool range_pool(const RangeArgs& a, int qi, bool* ok) {
     *ok = true;
@@ -956,7 +972,26 @@ __device__ __forceinline__ Pool range_pool(const RangeArgs& a, int qi, bool* ok)
     return a.pools[ti];
 }
 
-__global__ __launch_bounds__(64) void k_range(RangeArgs a, int fill) {
+struct RangeQuery {
+    float qx, qy, qz, r, r2;
+};
+__device__ __forceinline__ RangeQuery range_query(const RangeArgs& a, int qi) {
+    RangeQuery q{0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (a.mode == 1) {
+        q.qx = a.centers[3 * qi];
+        q.qy = a.centers[3 * qi + 1];
+        q.qz = a.centers[3 * qi + 2];
+        q.r = a.radii[qi];
+        q.r2 = q.r * q.r;
+    }
+    return q;
+}
+__device__ __forceinline__ bool range_takes(const RangeArgs& a, int qi, const RangeQuery& q, const float* f) {
+    if (a.mode == 0) return box_contains(a.boxes + 6 * qi, f[0], f[1], f[2]);
+    return calc_dist(f[0], f[1], f[2], q.qx, q.qy, q.qz) <= q.r2;
+}
+
+__global__ __launch_bounds__(64) void k_range_top(RangeArgs a, int list) {
     extern __shared__ int lds_dyn[];
     int* stk = lds_dyn;
     const int lane = threadIdx.x;
@@ -965,63 +1000,37 @@ __global__ __launch_bounds__(64) void k_range(RangeArgs a, int fill) {
     bool ok;
     const Pool P = range_pool(a, qi, &ok);
     if (!ok) {
-        if (!fill) {
-            a.counts[qi] = 0;
-            if (a.tcounts) a.tcounts[qi] = 0;
-        }
+        if (!list) a.tcounts[qi] = 0;
         return;
     }
     const NodeHot* __restrict__ hot = P.hot;
     const int G = a.task_nodes;
-    float qx = 0, qy = 0, qz = 0, r = 0;
-    if (a.mode == 1) {
-        qx = a.centers[3 * qi];
-        qy = a.centers[3 * qi + 1];
-        qz = a.centers[3 * qi + 2];
-        r = a.radii[qi];
-    }
-    long long total = 0, ntask = 0;
-    long long pos = fill ? a.offsets[qi] : 0;
-    const long long lim_end = fill ? a.offsets[qi + 1] : 0;
-    long long tpos = (fill && G > 0) ? a.toffsets[qi] : 0;
-    const long long tlim = (fill && G > 0) ? a.toffsets[qi + 1] : 0;
-    auto emit = [&](int u, const float* f) {
-        if (fill) {
-            if (pos >= lim_end) {
-                atomicOr(a.overflow, 8);
-                return;
-            }
-            a.out_xyz[3 * pos] = f[0];
-            a.out_xyz[3 * pos + 1] = f[1];
-            a.out_xyz[3 * pos + 2] = f[2];
-            a.out_ids[pos] = P.ids[u];
-            ++pos;
+    const RangeQuery q = range_query(a, qi);
+    long long items = 0;
+    long long tpos = list ? a.toffsets[qi] : 0;
+    const long long tlim = list ? a.toffsets[qi + 1] : 0;
+    auto item = [&](int u, int kind) {
+        if (!list) {
+            ++items;
+        } else if (tpos < tlim) {
+            a.tasks[tpos++] = make_int4(u, qi, kind, 0);
         } else {
-            ++total;
+            atomicOr(a.overflow, 8);
         }
     };
-    // a subtree the query covers whole: counted at once, or a task, or walked
-    auto covered = [&](int c) -> int {
-        if (G > 0 && P.cnt[c].x <= G) return c | RANGE_TASK;
-        if (G == 0 && !fill) {
-            const int4 cc = P.cnt[c];
-            total += cc.x - cc.y;
-            return -1;
-        }
-        return c | RANGE_WHOLE;
+    // an entry for child c of class cls (1 covered, 2 cut)
+    auto entry = [&](int c, int cls) -> int {
+        if (P.cnt[c].x <= G) return c | RANGE_TASK | (cls == 2 ? RANGE_CUT : 0);
+        return c | (cls == 1 ? RANGE_WHOLE : 0);
     };
-    uint32_t lk0 = fbits(hot[SENTINEL].f[3]);
+    const uint32_t lk0 = fbits(hot[SENTINEL].f[3]);
     if ((lk0 & LK_HASL) && !(lk0 & LK_TDL)) {
         int sp = 0;
-        const int c0 = classify(a, qi, hot[SENTINEL].f + 4, qx, qy, qz, r);
+        const int c0 = classify(a, qi, hot[SENTINEL].f + 4, q.qx, q.qy, q.qz, q.r);
         if (c0 != 0) {
-            const int e0 = c0 == 1 ? covered(ROOT) : ROOT;
-            if (e0 != -1) {
-                stk[lane] = e0;
-                sp = 1;
-            }
+            stk[lane] = entry(ROOT, c0);
+            sp = 1;
         }
-        const float r2 = r * r;
         long long steps = 0;
         while (sp > 0) {
             if (++steps > a.max_steps) {
@@ -1031,57 +1040,115 @@ __global__ __launch_bounds__(64) void k_range(RangeArgs a, int fill) {
             const int e = stk[--sp * 64 + lane];
             const int u = e & RANGE_NODE;
             if (e & RANGE_TASK) {
-                const int4 cc = P.cnt[u];
-                const long long live = cc.x - cc.y;
-                if (live <= 0) continue;
-                if (!fill) {
-                    total += live;
-                    ++ntask;
-                } else if (tpos < tlim && pos + live <= lim_end) {
-                    a.tasks[tpos++] = make_int4(u, qi, (int)(unsigned)(pos & 0xffffffffll), (int)(pos >> 32));
-                    pos += live;
-                } else {
-                    atomicOr(a.overflow, 8);
-                }
+                item(u, (e & RANGE_CUT) ? RANGE_PART : RANGE_COVER);
                 continue;
             }
             const bool whole = (e & RANGE_WHOLE) != 0;
             float f[16];
             load_record(hot[u].f, f);
             const uint32_t lk = fbits(f[3]);
-            if (!(lk & LK_PDEL)) {
-                bool take;
-                if (whole) take = true;
-                else if (a.mode == 0) take = box_contains(a.boxes + 6 * qi, f[0], f[1], f[2]);
-                else take = calc_dist(f[0], f[1], f[2], qx, qy, qz) <= r2;
-                if (take) emit(u, f);
-            }
+            if (!(lk & LK_PDEL) && (whole || range_takes(a, qi, q, f))) item(u, RANGE_OWN);
             const int pair = (int)lk_pair(lk);
 #pragma unroll
             for (int s = 1; s >= 0; --s) {
                 if (!(lk & lk_has(s)) || (lk & lk_td(s))) continue;
                 const int c = 2 * pair + s;
-                const int cls = whole ? 1 : classify(a, qi, f + 4 + 6 * s, qx, qy, qz, r);
+                const int cls = whole ? 1 : classify(a, qi, f + 4 + 6 * s, q.qx, q.qy, q.qz, q.r);
                 if (cls == 0) continue;
-                const int ce = cls == 1 ? covered(c) : c;
-                if (ce == -1) continue;
-                if (sp < a.stack) stk[sp++ * 64 + lane] = ce;
+                if (sp < a.stack) stk[sp++ * 64 + lane] = entry(c, cls);
                 else atomicOr(a.overflow, 8);
             }
         }
     }
-    if (!fill) {
-        a.counts[qi] = total;
-        if (a.tcounts) a.tcounts[qi] = ntask;
-    }
+    if (!list) a.tcounts[qi] = items;
 }
 
-// One lane per covered-subtree task: every live point of the subtree in
-// pre-order (own point, left subtree, right subtree; deleted subtrees
-// skipped), written from the task's output offset.
-__global__ __launch_bounds__(64) void k_range_tasks(RangeArgs a, long long ntask) {
+// The points one item holds: counted (fill false) or written from pos on in
+// pre-order (fill true, at most lim).  Returns the count / the end position.
+__device__ long long range_item(const RangeArgs& a, const Pool& P, int qi, int u, int kind, int* stk, int lane,
+                                long long pos, long long lim, bool fill) {
+    const RangeQuery q = range_query(a, qi);
+    const NodeHot* __restrict__ hot = P.hot;
+    auto emit = [&](int n, const float* f) {
+        if (!fill) {
+            ++pos;
+        } else if (pos < lim) {
+            a.out_xyz[3 * pos] = f[0];
+            a.out_xyz[3 * pos + 1] = f[1];
+            a.out_xyz[3 * pos + 2] = f[2];
+            a.out_ids[pos] = P.ids[n];
+            ++pos;
+        } else {
+            atomicOr(a.overflow, 8);
+        }
+    };
+    if (kind == RANGE_OWN) {
+        float f[16];
+        load_record(hot[u].f, f);
+        emit(u, f);
+        return pos;
+    }
+    if (kind == RANGE_COVER && !fill) {
+        const int4 cc = P.cnt[u];
+        return pos + (cc.x - cc.y);
+    }
+    int sp = 1;
+    stk[lane] = u | (kind == RANGE_COVER ? RANGE_WHOLE : 0);
+    long long steps = 0;
+    while (sp > 0) {
+        if (++steps > a.max_steps) {
+            atomicOr(a.overflow, 8);
+            break;
+        }
+        const int e = stk[--sp * 64 + lane];
+        const int n = e & RANGE_NODE;
+        const bool whole = (e & RANGE_WHOLE) != 0;
+        float f[16];
+        load_record(hot[n].f, f);
+        const uint32_t lk = fbits(f[3]);
+        if (!(lk & LK_PDEL) && (whole || range_takes(a, qi, q, f))) emit(n, f);
+        const int pair = (int)lk_pair(lk);
+#pragma unroll
+        for (int s = 1; s >= 0; --s) {
+            if (!(lk & lk_has(s)) || (lk & lk_td(s))) continue;
+            const int c = 2 * pair + s;
+            const int cls = whole ? 1 : classify(a, qi, f + 4 + 6 * s, q.qx, q.qy, q.qz, q.r);
+            if (cls == 0) continue;
+            if (cls == 1 && !fill) {  // counted from the stored counts
+                const int4 cc = P.cnt[c];
+                pos += cc.x - cc.y;
+                continue;
+            }
+            if (sp < a.stack) stk[sp++ * 64 + lane] = c | (cls == 1 ? RANGE_WHOLE : 0);
+            else atomicOr(a.overflow, 8);
+        }
+    }
+    return pos;
+}
+
+// counts[i] = points of item i (counts[ntask], the scan's trailing zero, set by the caller)
+__global__ __launch_bounds__(64) void k_range_item_count(RangeArgs a, long long ntask, long long* __restrict__ counts) {
     extern __shared__ int lds_dyn[];
-    int* stk = lds_dyn;
+    const int lane = threadIdx.x;
+    const long long ti = (long long)blockIdx.x * 64 + lane;
+    if (ti >= ntask) return;
+    const int4 t = a.tasks[ti];
+    bool ok;
+    const Pool P = range_pool(a, t.y, &ok);
+    counts[ti] = ok ? range_item(a, P, t.y, t.x, t.z, lds_dyn, lane, 0, 0, false) : 0;
+}
+
+// offsets[q] = output offset of query q's first item (items of a query are
+// consecutive; toffsets[nq] = ntask gives offsets[nq] = the total)
+__global__ void k_range_query_offsets(const long long* __restrict__ toffsets, const long long* __restrict__ item_off,
+                                      int nq, long long* __restrict__ offsets) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q <= nq) offsets[q] = item_off[toffsets[q]];
+}
+
+__global__ __launch_bounds__(64) void k_range_item_fill(RangeArgs a, long long ntask,
+                                                        const long long* __restrict__ item_off) {
+    extern __shared__ int lds_dyn[];
     const int lane = threadIdx.x;
     const long long ti = (long long)blockIdx.x * 64 + lane;
     if (ti >= ntask) return;
@@ -1089,51 +1156,29 @@ __global__ __launch_bounds__(64) void k_range_tasks(RangeArgs a, long long ntask
     bool ok;
     const Pool P = range_pool(a, t.y, &ok);
     if (!ok) return;
-    long long pos = (long long)(unsigned)t.z | ((long long)t.w << 32);
-    const long long lim_end = a.offsets[t.y + 1];
-    int sp = 1;
-    stk[lane] = t.x;
-    long long steps = 0;
-    while (sp > 0) {
-        if (++steps > a.max_steps) {
-            atomicOr(a.overflow, 8);
-            break;
-        }
-        const int u = stk[--sp * 64 + lane];
-        const gv4* rec = reinterpret_cast<const gv4*>(P.hot[u].f);
-        const gv4 r0 = gload4(rec);
-        const uint32_t lk = fbits(r0.w);
-        if (!(lk & LK_PDEL)) {
-            if (pos < lim_end) {
-                a.out_xyz[3 * pos] = r0.x;
-                a.out_xyz[3 * pos + 1] = r0.y;
-                a.out_xyz[3 * pos + 2] = r0.z;
-                a.out_ids[pos] = P.ids[u];
-                ++pos;
-            } else {
-                atomicOr(a.overflow, 8);
-            }
-        }
-        const int pair = (int)lk_pair(lk);
-#pragma unroll
-        for (int s = 1; s >= 0; --s) {
-            if (!(lk & lk_has(s)) || (lk & lk_td(s))) continue;
-            if (sp < a.stack) stk[sp++ * 64 + lane] = 2 * pair + s;
-            else atomicOr(a.overflow, 8);
-        }
-    }
+    const long long end = range_item(a, P, t.y, t.x, t.z, lds_dyn, lane, item_off[ti], item_off[ti + 1], true);
+    if (end != item_off[ti + 1]) atomicOr(a.overflow, 8);  // counts and walk disagree
 }
 
-void launch_range(const RangeArgs& a, int fill, hipStream_t s) {
+void launch_range_top(const RangeArgs& a, int list, hipStream_t s) {
     if (a.nq <= 0) return;
-    hipLaunchKernelGGL(k_range, dim3(blocks_for(a.nq, 64)), dim3(64), (size_t)a.stack * 64 * sizeof(int), s, a,
-                       fill);
+    hipLaunchKernelGGL(k_range_top, dim3(blocks_for(a.nq, 64)), dim3(64), (size_t)a.stack * 64 * sizeof(int), s, a,
+                       list);
 }
-
-void launch_range_tasks(const RangeArgs& a, long long ntask, hipStream_t s) {
+void launch_range_item_count(const RangeArgs& a, long long ntask, long long* counts, hipStream_t s) {
     if (ntask <= 0) return;
-    hipLaunchKernelGGL(k_range_tasks, dim3((unsigned)blocks_for(ntask, 64)), dim3(64),
-                       (size_t)a.stack * 64 * sizeof(int), s, a, ntask);
+    hipLaunchKernelGGL(k_range_item_count, dim3((unsigned)blocks_for(ntask, 64)), dim3(64),
+                       (size_t)a.stack * 64 * sizeof(int), s, a, ntask, counts);
+}
+void launch_range_query_offsets(const long long* toffsets, const long long* item_off, int nq, long long* offsets,
+                                hipStream_t s) {
+    hipLaunchKernelGGL(k_range_query_offsets, dim3(blocks_for(nq + 1, 256)), dim3(256), 0, s, toffsets, item_off, nq,
+                       offsets);
+}
+void launch_range_item_fill(const RangeArgs& a, long long ntask, const long long* item_off, hipStream_t s) {
+    if (ntask <= 0) return;
+    hipLaunchKernelGGL(k_range_item_fill, dim3((unsigned)blocks_for(ntask, 64)), dim3(64),
+                       (size_t)a.stack * 64 * sizeof(int), s, a, ntask, item_off);
 }
 
 }  // namespace ikd

@@ -306,14 +306,15 @@ int ikd_batch_knn_graph_create(ikd_batch_t b, const ikd_tree_t* trees, int ntree
 int ikd_batch_knn_graph_launch(ikd_knn_graph_t g);
 int ikd_batch_knn_graph_destroy(ikd_knn_graph_t g);
 
-/* Radius_Search (ikd_Tree.cpp:406-411) for nq queries spread over trees, in
- * one count launch and one fill launch: query i searches
+/* Radius_Search (ikd_Tree.cpp:406-411) for nq queries spread over trees
+ * (count walks, then one lane per piece of each query's result): query i searches
  * trees[d_query_tree[i]] around d_centers[3i..] with radius d_radii[i] (an
  * index outside the list finds nothing).  d_out_offsets (device, nq+1) gets
  * the exclusive prefix of the counts and *out_total (host) the total; the
  * points and ids are written to d_out_xyz / d_out_ids when both are given and
  * capacity >= total (else IKD_ERR_CAPACITY if they were given).  Result order
- * within a query is unspecified.  Waits for the count on the host. */
+ * within a query is unspecified.  Waits twice on the host (the piece count,
+ * then the total). */
 int ikd_batch_radius_search_device(ikd_batch_t b, const ikd_tree_t* trees, int ntrees, const int32_t* d_query_tree,
                                    const float* d_centers, const float* d_radii, int64_t nq, int64_t* d_out_offsets,
                                    float* d_out_xyz, uint32_t* d_out_ids, int64_t capacity, int64_t* out_total);
