@@ -769,7 +769,7 @@ def knn_kernel_name(k, lanes, lds_levels=0, compact=1):
         return f"ikd::k_knn_persist<{knn_bucket(k)}>"
     if lanes < 0:
         return f"ikd::k_knn*<{knn_bucket(k)}> (lanes per query from the batch size)"
-    return f"ikd::k_knn_team<{knn_bucket(k)}, {lanes}>"
+    return f"ikd::k_knn_team<{knn_bucket(k)}, {lanes}, {'true' if compact else 'false'}>"
 
 
 def knn_bucket(k):

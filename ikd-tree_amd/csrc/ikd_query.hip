@@ -502,10 +502,33 @@ __device__ __forceinline__ void wave_lds_fence() {
     __builtin_amdgcn_wave_barrier();
 }
 
-template <int KM, int T>
+// A deque slot of either layout: 6-byte (index, 16-bit distance) or compact.
+template <bool COMPACT>
+__device__ __forceinline__ void slot_put(const LdsStack& st, uint32_t* cst, int i, int node, float d) {
+    if (COMPACT) {
+        cst[i] = compact_entry(node, d);
+    } else {
+        st.n[i] = node;
+        st.d[i] = dist_key(d);
+    }
+}
+template <bool COMPACT>
+__device__ __forceinline__ int slot_get(const LdsStack& st, const uint32_t* cst, int i, float* d) {
+    if (COMPACT) {
+        const uint32_t e = cst[i];
+        *d = compact_dist(e);
+        return (int)(e >> 8);
+    }
+    *d = dist_val(st.d[i]);
+    return st.n[i];
+}
+
+template <int KM, int T, bool COMPACT>
 __global__ __launch_bounds__(64) void k_knn_team(KnnArgs a) {
     constexpr int TEAMS = 64 / T;
     const LdsStack st = lds_stack(a.stack);
+    extern __shared__ int lds_dyn[];
+    uint32_t* const cst = reinterpret_cast<uint32_t*>(lds_dyn);
     const int smask = a.stack - 1;  // team launches use a power-of-two stack
     __shared__ int dn_n[TEAMS][T];
     __shared__ float dn_d[TEAMS][T];
@@ -540,9 +563,10 @@ __global__ __launch_bounds__(64) void k_knn_team(KnnArgs a) {
         while (n < 0 && tail != head) {
             --tail;
             const int s = tail & smask;
-            const float d = dist_val(st.d[s * 64 + lane]);
+            float d;
+            const int e = slot_get<COMPACT>(st, cst, s * 64 + lane, &d);
             if (d <= fminf(B, best.d[0])) {
-                n = st.n[s * 64 + lane];
+                n = e;
                 bd = d;
             }
         }
@@ -559,8 +583,9 @@ __global__ __launch_bounds__(64) void k_knn_team(KnnArgs a) {
                 const int r = __popcll(dmask & below);
                 if (r < ni) {
                     const int s = head & smask;
-                    dn_n[team][r] = st.n[s * 64 + lane];
-                    dn_d[team][r] = dist_val(st.d[s * 64 + lane]);
+                    float d;
+                    dn_n[team][r] = slot_get<COMPACT>(st, cst, s * 64 + lane, &d);
+                    dn_d[team][r] = d;
                     ++head;
                 }
             }
@@ -593,8 +618,7 @@ __global__ __launch_bounds__(64) void k_knn_team(KnnArgs a) {
                 if (fn >= 0 && df <= bound && df <= c.thr) {
                     if (tail - head < a.stack) {
                         const int s = tail & smask;
-                        st.n[s * 64 + lane] = fn;
-                        st.d[s * 64 + lane] = dist_key(df);
+                        slot_put<COMPACT>(st, cst, s * 64 + lane, fn, df);
                         ++tail;
                     } else {
                         atomicOr(a.overflow, 1);
@@ -816,16 +840,36 @@ static int launch_knn_km(const KnnArgs& a0, int lanes, int stack, hipStream_t s,
             return 0;
         }
         case 2:
-            hipLaunchKernelGGL((k_knn_team<KM, 2>), dim3(xcd_grid(blocks_for(a.nq, 32))), block, lds, s, a);
+            if (a.compact_stack)
+                hipLaunchKernelGGL((k_knn_team<KM, 2, true>), dim3(xcd_grid(blocks_for(a.nq, 32))), block,
+                                   lds_compact_bytes(stack), s, a);
+            else
+                hipLaunchKernelGGL((k_knn_team<KM, 2, false>), dim3(xcd_grid(blocks_for(a.nq, 32))), block, lds,
+                                   s, a);
             return 0;
         case 4:
-            hipLaunchKernelGGL((k_knn_team<KM, 4>), dim3(xcd_grid(blocks_for(a.nq, 16))), block, lds, s, a);
+            if (a.compact_stack)
+                hipLaunchKernelGGL((k_knn_team<KM, 4, true>), dim3(xcd_grid(blocks_for(a.nq, 16))), block,
+                                   lds_compact_bytes(stack), s, a);
+            else
+                hipLaunchKernelGGL((k_knn_team<KM, 4, false>), dim3(xcd_grid(blocks_for(a.nq, 16))), block, lds,
+                                   s, a);
             return 0;
         case 8:
-            hipLaunchKernelGGL((k_knn_team<KM, 8>), dim3(xcd_grid(blocks_for(a.nq, 8))), block, lds, s, a);
+            if (a.compact_stack)
+                hipLaunchKernelGGL((k_knn_team<KM, 8, true>), dim3(xcd_grid(blocks_for(a.nq, 8))), block,
+                                   lds_compact_bytes(stack), s, a);
+            else
+                hipLaunchKernelGGL((k_knn_team<KM, 8, false>), dim3(xcd_grid(blocks_for(a.nq, 8))), block, lds,
+                                   s, a);
             return 0;
         case 16:
-            hipLaunchKernelGGL((k_knn_team<KM, 16>), dim3(xcd_grid(blocks_for(a.nq, 4))), block, lds, s, a);
+            if (a.compact_stack)
+                hipLaunchKernelGGL((k_knn_team<KM, 16, true>), dim3(xcd_grid(blocks_for(a.nq, 4))), block,
+                                   lds_compact_bytes(stack), s, a);
+            else
+                hipLaunchKernelGGL((k_knn_team<KM, 16, false>), dim3(xcd_grid(blocks_for(a.nq, 4))), block, lds,
+                                   s, a);
             return 0;
         default:
             return -1;

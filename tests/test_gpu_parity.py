@@ -174,8 +174,9 @@ def test_knn_lds_staging_invariance(ikd, k, max_dist):
 @pytest.mark.parametrize("k,max_dist", [(1, INF), (5, 5.0), (20, INF)])
 def test_knn_compact_stack_invariance(ikd, k, max_dist):
     """The 4-byte LDS stack entries (power-of-two distance bound) return the
-    same bits as the 6-byte ones, one lane per query, on a tree with deleted
-    points and subtrees and with exact distance ties."""
+    same bits as the 6-byte ones, for one lane per query, teams of lanes and
+    persistent lanes, on a tree with deleted points and subtrees and with
+    exact distance ties."""
     rng = np.random.default_rng(360 + k)
     pts = np.concatenate([uniform_cloud(rng, 50000), grid_cloud(rng, 20000, extent=16)])
     ids = np.arange(pts.shape[0], dtype=np.uint32)
@@ -183,14 +184,16 @@ def test_knn_compact_stack_invariance(ikd, k, max_dist):
     g.Delete_Point_Boxes(random_boxes(rng, 6, smin=5, smax=25))
     g.Delete_Points(pts[rng.choice(pts.shape[0], 3000, replace=False)])
     q = np.concatenate([uniform_cloud(rng, 2500), grid_cloud(rng, 500, extent=16) + np.float32(0.125)])
-    g.set_tuning(ikd.IKD_TUNE_LANES_PER_QUERY, 1)
     out = []
-    for compact in (0, 1):
-        g.set_tuning(ikd.IKD_TUNE_KNN_COMPACT, compact)
-        out.append(g.knn(q, k, max_dist))
-    for a_, b_ in zip(*out):
-        assert np.array_equal(a_.view(np.uint32) if a_.dtype == np.float32 else a_,
-                              b_.view(np.uint32) if b_.dtype == np.float32 else b_)
+    for lanes in (1, 4, 0):  # one lane, a team of 4, persistent lanes
+        g.set_tuning(ikd.IKD_TUNE_LANES_PER_QUERY, lanes)
+        for compact in (0, 1):
+            g.set_tuning(ikd.IKD_TUNE_KNN_COMPACT, compact)
+            out.append(g.knn(q, k, max_dist))
+    for o in out[1:]:
+        for a_, b_ in zip(o, out[0]):
+            assert np.array_equal(a_.view(np.uint32) if a_.dtype == np.float32 else a_,
+                                  b_.view(np.uint32) if b_.dtype == np.float32 else b_)
     live, lid = g.flatten()
     bd, bi, bc = brute_knn(live, lid, q, k, max_dist, tie="xyz")
     assert np.array_equal(out[1][3], bc)
