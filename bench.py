@@ -779,49 +779,85 @@ def knn_bucket(k):
     return k
 
 
-def pipeline_latency(ikd, torch, dev, scene, frames, k, max_dist):
-    """Mapping pipeline as specified: Build(scan 0), then per scan kNN + Add_Points(0.5 m)."""
+def pipeline_latency(ikd, torch, dev, scene, frames, k, max_dist, defer_points=1500):
+    """Mapping pipeline as specified: Build(scan 0), then per scan kNN + Add_Points(0.5 m).
+
+    Two forms on the same scans.  'inline' rebuilds every unbalanced subtree
+    inside the Add_Points call.  'deferred' is the reference's two-thread split
+    (ikd_Tree.cpp:627-633: subtrees of >= Multi_Thread_Rebuild_Point_Num = 1500
+    points go to Rebuild_Thread): subtrees above defer_points live points are
+    listed instead and rebuilt by rebuild_deferred between scans, timed apart
+    as 'maintenance'.  The kNN results of every scan and the final live point
+    sets of the two forms are compared."""
     scans = [scene.scan(i) for i in range(frames)]
-    tree = ikd.KD_TREE(0.5, 0.6, 0.5)
-    tree.set_tuning(ikd.IKD_TUNE_QUERY_ORDER, 0)
-    tree.Build(scans[0])
     qd = [torch.from_numpy(x).to(dev) for x in scans]
     n_max = max(x.shape[0] for x in scans)
-    o = (torch.empty((n_max, k, 3), dtype=torch.float32, device=dev),
-         torch.empty((n_max, k), dtype=torch.float32, device=dev),
-         torch.empty((n_max, k), dtype=torch.int32, device=dev),
-         torch.empty((n_max,), dtype=torch.int32, device=dev))
-    torch.cuda.synchronize()
-    lat, knn_lat, ins_lat, added, nq = [], [], [], 0, 0
-    t_start = time.perf_counter()
-    for i in range(1, frames):
-        n = qd[i].shape[0]
-        ts = time.perf_counter()
-        tree.knn_device(qd[i], k, max_dist, o[0][:n], o[1][:n], o[2][:n], o[3][:n])
-        tree.synchronize()
-        tk = time.perf_counter()
-        added += tree.add_points_device(qd[i], True)
-        te = time.perf_counter()
-        lat.append(te - ts)
-        knn_lat.append(tk - ts)
-        ins_lat.append(te - tk)
-        nq += n
-    total = time.perf_counter() - t_start
-    lat_ms = np.array(lat) * 1e3
-    return {
-        "frames": frames,
-        "map_points_end": int(tree.validnum()),
-        "scan_latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99)),
-                            "max": float(lat_ms.max()), "mean": float(lat_ms.mean())},
-        "knn_ms_p50": float(np.percentile(np.array(knn_lat) * 1e3, 50)),
-        "insert_ms_p50": float(np.percentile(np.array(ins_lat) * 1e3, 50)),
-        "insert_ms_p99": float(np.percentile(np.array(ins_lat) * 1e3, 99)),
-        "queries_per_s": nq / total,
-        "inserted_points_per_s": nq / sum(ins_lat),
-        "add_points_returned": int(added),
-        "pool": tree.pool_stats(),
-        "counters": tree.counters(),
-    }
+    outs = [tuple(torch.empty(shape, dtype=dt, device=dev) for shape, dt in
+                  (((n_max, k, 3), torch.float32), ((n_max, k), torch.float32), ((n_max, k), torch.int32),
+                   ((n_max,), torch.int32))) for _ in range(2)]
+    digests = {}
+
+    def run(form):
+        tree = ikd.KD_TREE(0.5, 0.6, 0.5)
+        tree.set_tuning(ikd.IKD_TUNE_QUERY_ORDER, 0)
+        if form == "deferred":
+            tree.set_tuning(ikd.IKD_TUNE_DEFER_REBUILD, defer_points)
+        tree.Build(scans[0])
+        o = outs[0 if form == "inline" else 1]
+        torch.cuda.synchronize()
+        lat, knn_lat, ins_lat, maint, added, nq, n_defer = [], [], [], [], 0, 0, 0
+        dig = []
+        t_start = time.perf_counter()
+        for i in range(1, frames):
+            n = qd[i].shape[0]
+            ts = time.perf_counter()
+            tree.knn_device(qd[i], k, max_dist, o[0][:n], o[1][:n], o[2][:n], o[3][:n])
+            tree.synchronize()
+            tk = time.perf_counter()
+            added += tree.add_points_device(qd[i], True)
+            te = time.perf_counter()
+            if form == "deferred":
+                n_defer += tree.rebuild_deferred(0)
+                maint.append(time.perf_counter() - te)
+            lat.append(te - ts)
+            knn_lat.append(tk - ts)
+            ins_lat.append(te - tk)
+            nq += n
+            # a digest of this scan's kNN distances and counts (ids differ between forms: rebuilds reorder nodes)
+            dig.append(torch.stack([o[1][:n].double().sum(), o[3][:n].double().sum()]))
+        total = time.perf_counter() - t_start
+        digests[form] = torch.stack(dig).cpu().numpy()
+        live = tree.flatten()[0]
+        live = live[np.lexsort(live.T[::-1])]
+        lat_ms = np.array(lat) * 1e3
+        r = {
+            "map_points_end": int(tree.validnum()),
+            "scan_latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99)),
+                                "max": float(lat_ms.max()), "mean": float(lat_ms.mean())},
+            "knn_ms_p50": float(np.percentile(np.array(knn_lat) * 1e3, 50)),
+            "insert_ms_p50": float(np.percentile(np.array(ins_lat) * 1e3, 50)),
+            "insert_ms_p99": float(np.percentile(np.array(ins_lat) * 1e3, 99)),
+            "queries_per_s": nq / total,
+            "inserted_points_per_s": nq / sum(ins_lat),
+            "add_points_returned": int(added),
+            "pool": tree.pool_stats(),
+            "counters": tree.counters(),
+        }
+        if form == "deferred":
+            m_ms = np.array(maint) * 1e3
+            r["maintenance_ms"] = {"p50": float(np.percentile(m_ms, 50)), "p99": float(np.percentile(m_ms, 99)),
+                                   "max": float(m_ms.max()), "mean": float(m_ms.mean())}
+            r["deferred_rebuilds"] = int(n_defer)
+            r["defer_above_live_points"] = int(defer_points)
+        tree.close()
+        return r, live
+
+    inline, live_a = run("inline")
+    deferred, live_b = run("deferred")
+    res = {"frames": frames, **inline, "deferred": deferred,
+           "identical_live_sets": bool(live_a.shape == live_b.shape and np.array_equal(live_a, live_b)),
+           "identical_knn_digests": bool(np.array_equal(digests["inline"], digests["deferred"]))}
+    return res
 
 
 def cpu_baseline(seed_pts, cpu_scans, gpu_out, count_scans, k, max_dist, min_seconds=10.0):
