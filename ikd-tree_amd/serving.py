@@ -64,18 +64,41 @@ def now_ns() -> int:
     return int(_lib().ikd_now_ns())
 
 
-class Ticket:
-    """One submitted request: keeps its buffers alive and exposes the results."""
+def result_buffers(n_max: int, k: int):
+    """Host result arrays for requests of up to n_max queries at k neighbours,
+    written once so that their pages are resident: a client that reuses them
+    (submit(..., outs=...)) keeps page faults out of the workers' result copies."""
+    outs = (np.empty((n_max, k, 3), np.float32), np.empty((n_max, k), np.float32),
+            np.empty((n_max, k), np.uint32), np.empty(n_max, np.int32))
+    for a in outs:
+        a.fill(0)
+    return outs
 
-    def __init__(self, map_index, xyz, k, max_dist, downsample, trim, tag):
+
+class Ticket:
+    """One submitted request: keeps its buffers alive and exposes the results.
+    With outs (arrays from result_buffers, at least n rows and the request's
+    k) the results are written there instead of into new arrays."""
+
+    def __init__(self, map_index, xyz, k, max_dist, downsample, trim, tag, outs=None):
         self.xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
         n = self.xyz.shape[0]
         self.k = int(k)
         kk = max(self.k, 0)
-        self.out_xyz = np.zeros((n, kk, 3), np.float32)
-        self.out_d2 = np.zeros((n, kk), np.float32)
-        self.out_ids = np.zeros((n, kk), np.uint32)
-        self.out_counts = np.zeros(n, np.int32)
+        self.outs = outs
+        if outs is not None:
+            ox, od, oi, oc = outs
+            if (ox.shape[0] < n or ox.shape[1:] != (kk, 3) or od.shape[1:] != (kk,) or oi.shape[1:] != (kk,)
+                    or min(od.shape[0], oi.shape[0], oc.shape[0]) < n or ox.dtype != np.float32
+                    or od.dtype != np.float32 or oi.dtype != np.uint32 or oc.dtype != np.int32
+                    or not all(a.flags.c_contiguous for a in outs)):
+                raise ValueError("result buffers do not fit the request")
+            self.out_xyz, self.out_d2, self.out_ids, self.out_counts = ox[:n], od[:n], oi[:n], oc[:n]
+        else:
+            self.out_xyz = np.zeros((n, kk, 3), np.float32)
+            self.out_d2 = np.zeros((n, kk), np.float32)
+            self.out_ids = np.zeros((n, kk), np.uint32)
+            self.out_counts = np.zeros(n, np.int32)
         self.trim = None if trim is None else np.ascontiguousarray(trim, np.float32).reshape(-1, 6)
         r = Request()
         r.map = map_index
@@ -152,8 +175,8 @@ class Server:
         return int(m.value)
 
     def submit(self, map_index: int, xyz, k: int = 5, max_dist: float = 5.0, downsample: int = 1, trim=None,
-               tag: int = 0, arrival_ns: int | None = None) -> Ticket:
-        t = Ticket(map_index, xyz, k, max_dist, downsample, trim, tag)
+               tag: int = 0, arrival_ns: int | None = None, outs=None) -> Ticket:
+        t = Ticket(map_index, xyz, k, max_dist, downsample, trim, tag, outs)
         t.req.t_arrival_ns = now_ns() if arrival_ns is None else int(arrival_ns)
         rc = self._lib.ikd_server_submit(self._h, ctypes.byref(t.req))
         if rc != 0:
@@ -225,6 +248,17 @@ def simulate(server: Server, map_ids, scan_fn, rate_hz: float = 10.0, duration_s
     keys = [(r, j) for (_, r, _, j) in events]
     with ThreadPoolExecutor(max_workers=16) as ex:  # scan synthesis releases the GIL
         scans = dict(zip(keys, ex.map(lambda rj: scan_fn(*rj), keys)))
+    # each robot's result buffers are reused once its request is back, as a
+    # client keeps them (two sets per robot to start; more if requests queue)
+    n_max = max(s.shape[0] for s in scans.values())
+    free = {r: [result_buffers(n_max, k) for _ in range(2)] for r in range(len(map_ids))}
+
+    def collect(got):
+        for d in got:
+            free[d.req.tag].append(d.outs)
+            d.outs = None
+        done.extend(got)
+
     t0 = now_ns() + 50_000_000
     done = []
     points = 0
@@ -235,14 +269,15 @@ def simulate(server: Server, map_ids, scan_fn, rate_hz: float = 10.0, duration_s
             if wait_ms <= 0:
                 break
             got = server.wait(timeout_ms=max(1, int(wait_ms)))
-            done.extend(got)
+            collect(got)
             if not got:  # idle server: wait() returned at once
                 time.sleep(min(wait_ms, 1.0) / 1e3)
         trim = trim_fn(r, j) if (trim_fn is not None and j % trim_every == trim_every - 1) else None
         s = scans[(r, j)]
         points += s.shape[0]
-        server.submit(m, s, k, max_dist, 1, trim, tag=r, arrival_ns=due)
-    done.extend(server.drain())
+        outs = free[r].pop() if free[r] else result_buffers(n_max, k)
+        server.submit(m, s, k, max_dist, 1, trim, tag=r, arrival_ns=due, outs=outs)
+    collect(server.drain())
     t1 = max(d.req.t_done_ns for d in done)
     lat = np.array([d.latency_ms for d in done])
     wait = np.array([(d.req.t_start_ns - d.req.t_arrival_ns) / 1e6 for d in done])
@@ -253,6 +288,15 @@ def simulate(server: Server, map_ids, scan_fn, rate_hz: float = 10.0, duration_s
     def pct(a):
         return {"p50": float(np.percentile(a, 50)), "p99": float(np.percentile(a, 99)), "max": float(a.max())}
 
+    # latency percentiles per 5 s of arrivals: how the tail moves over the run
+    arr_s = np.array([(d.req.t_arrival_ns - t0) / 1e9 for d in done])
+    by_window = []
+    for w0 in np.arange(0.0, duration_s, 5.0):
+        sel = lat[(arr_s >= w0) & (arr_s < w0 + 5.0)]
+        if sel.size:
+            by_window.append({"from_s": float(w0), "requests": int(sel.size), "p50": float(np.percentile(sel, 50)),
+                              "p99": float(np.percentile(sel, 99))})
+
     return {
         "requests": len(done),
         "failed": len(bad),
@@ -260,6 +304,7 @@ def simulate(server: Server, map_ids, scan_fn, rate_hz: float = 10.0, duration_s
                        "max": float(lat.max()), "mean": float(lat.mean())},
         "queue_wait_ms": pct(wait),
         "run_ms": pct(run),
+        "latency_ms_by_window": by_window,
         "slowest": [{"robot": int(d.req.tag), "points": int(d.req.n), "trim": int(d.req.n_trim > 0),
                      "wait_ms": (d.req.t_start_ns - d.req.t_arrival_ns) / 1e6,
                      "run_ms": (d.req.t_done_ns - d.req.t_start_ns) / 1e6,

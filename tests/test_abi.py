@@ -77,3 +77,19 @@ def test_cpp_facade_compiles_and_links(tmp_path):
     the C ABI, exactly as a reference caller would include ikd_Tree.h."""
     r = _compile_facade(tmp_path / "facade_check")
     assert r.returncode == 0, r.stderr
+
+
+def test_served_result_buffers_are_checked():
+    """Caller-owned result buffers must hold the request (rows, k, types); a
+    mismatch is an error before anything reaches the library."""
+    import numpy as np
+    serving = importlib.import_module("ikd-tree_amd.serving")
+    pts = np.zeros((10, 3), np.float32)
+    outs = serving.result_buffers(16, 5)
+    t = serving.Ticket(0, pts, 5, 5.0, 1, None, 0, outs)
+    assert t.out_xyz.shape == (10, 5, 3) and t.out_counts.shape == (10,)
+    assert t.req.out_d2 == outs[1].ctypes.data
+    for bad in (serving.result_buffers(9, 5), serving.result_buffers(16, 4),
+                (outs[0], outs[1].astype(np.float64), outs[2], outs[3])):
+        with pytest.raises(ValueError):
+            serving.Ticket(0, pts, 5, 5.0, 1, None, 0, bad)

@@ -63,7 +63,10 @@ def test_mixed_equals_alone(ikd, serving, scenes, defer):
         trees.append(t)
     srv = serving.Server(slots=4, hbm_budget_bytes=8 << 30)
     ids = [srv.add_map(t, defer_rebuild_points=defer) for t in trees]
-    tickets = [srv.submit(ids[m], pts, 5, 5.0, 1, trim, tag=i) for i, (m, pts, trim) in enumerate(reqs)]
+    # every other request writes into caller-owned result buffers larger than it needs
+    tickets = [srv.submit(ids[m], pts, 5, 5.0, 1, trim, tag=i,
+                          outs=serving.result_buffers(len(pts) + 97, 5) if i % 2 else None)
+               for i, (m, pts, trim) in enumerate(reqs)]
     done = srv.drain()
     srv.close()
     assert len(done) == len(tickets)
