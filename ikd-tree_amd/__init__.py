@@ -48,7 +48,7 @@ EXPORTS = (
     "ikd_batch_knn_graph_destroy",
     "ikd_now_ns", "ikd_server_create", "ikd_server_add_map", "ikd_server_submit", "ikd_server_wait",
     "ikd_server_pending", "ikd_server_destroy", "ikd_server_set_maintenance", "ikd_server_stats",
-    "ikd_server_memory",
+    "ikd_server_memory", "ikd_host_alloc", "ikd_host_free",
     "ikd_rebuild_deferred", "ikd_list_payload_ids", "ikd_relabel_payload_ids",
 )
 IKD_TUNE_QUERY_ORDER = 1

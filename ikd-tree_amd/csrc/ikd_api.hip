@@ -217,6 +217,7 @@ struct ikd_tree_s {
     RootInfo* h_root = nullptr;
     uint32_t next_id = 0;
     long long rebuilds = 0;
+    bool compact_due = false;  // a sparse pool left to ikd_rebuild_deferred
     int max_depth = 0;
     long long pool_used = 4;
     int query_order = 0;  // 1: Morton-sort large query batches before the kNN launch
@@ -1055,12 +1056,22 @@ int rebuild_tops(ikd_tree_t t) {
 // garbage or the tree got deeper than the traversal stacks allow.
 // fresh: the header and root summary were refreshed after the last change.
 // Returns with both fresh.
+bool pool_sparse(ikd_tree_t t, long long live) { return t->pool_used > 4 * live + (1ll << 22); }
+
 int compact_if_needed(ikd_tree_t t, bool force, bool fresh = false) {
     if (!fresh) CHK(refresh(t));
     long long live = live_count(t);
     bool deep = t->max_depth + 2 > MAX_DEPTH;
-    bool sparse = t->pool_used > 4 * live + (1ll << 22);
+    bool sparse = pool_sparse(t, live);
     if (!force && !deep && !sparse) return IKD_OK;
+    // With deferred rebuilds on, a sparse pool is reclaimed by
+    // ikd_rebuild_deferred (off the update's latency) until it holds twice
+    // the garbage; a tree too deep for the traversal stacks is rebuilt now.
+    if (!force && !deep && t->prm.defer_points > 0 && t->pool_used <= 8 * live + (1ll << 23)) {
+        t->compact_due = true;
+        return IKD_OK;
+    }
+    t->compact_due = false;
     t->ctr.compactions += 1;
     if (!has_root(t)) {
         CHK(reset_pool(t));
@@ -2125,6 +2136,16 @@ int ikd_rebuild_deferred(ikd_tree_t t, int64_t max_points, int32_t* out_rebuilt)
     CHECK_T();
     t->hdr_valid = false;
     if (out_rebuilt) *out_rebuilt = 0;
+    if (t->compact_due) {
+        // the whole tree, whatever max_points says; it empties the list too
+        t->compact_due = false;
+        CHK(refresh(t));
+        if (pool_sparse(t, live_count(t))) {
+            CHK(compact_if_needed(t, true, true));
+            if (out_rebuilt) *out_rebuilt = 1;
+            return IKD_OK;
+        }
+    }
     CHK(sync_hdr(t));
     CHK(check_count(t, "deferred rebuilds", t->h_hdr->defer_count, 0, INT_MAX));
     int n = std::min(t->h_hdr->defer_count, DEFER_CAP);

@@ -77,8 +77,11 @@ struct ikd_server_s {
     int64_t budget = 0;
     int64_t in_flight_bytes = 0;
     int64_t in_flight = 0;
-    int64_t maintain_points = -1;  // deferred-rebuild budget per idle map (-1: no limit, 0: never)
+    int64_t maintain_points = -1;  // deferred-rebuild budget per maintenance step (-1: no limit, 0: never)
     int64_t maintenance_runs = 0;
+    std::deque<int> maint_due;     // maps that may have deferred rebuilds left, oldest first
+    std::vector<char> maint_queued;  // map is in maint_due
+    int64_t maint_running = 0;     // maintenance steps on the device now (at most one)
     bool stop = false;
     std::string err;
 
@@ -178,13 +181,20 @@ struct ikd_server_s {
             rc = ikd_nearest_search_device(t, b.xyz, n, r->k, r->max_dist, b.out_xyz, b.out_d2, b.out_ids,
                                            b.out_cnt);
             if (rc != IKD_OK) return rc;
-            if (r->out_d2 && hipMemcpyAsync(b.h_d2, b.out_d2, res * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+            // page-locked caller buffers take the results straight from the
+            // device; others through the slot's page-locked mirrors
+            const bool direct = (r->flags & IKD_REQ_PINNED_RESULTS) != 0;
+            float* hd2 = direct ? r->out_d2 : b.h_d2;
+            uint32_t* hids = direct ? r->out_ids : b.h_ids;
+            float* hxyz = direct ? r->out_xyz : b.h_xyz;
+            int32_t* hcnt = direct ? r->out_counts : b.h_cnt;
+            if (r->out_d2 && hipMemcpyAsync(hd2, b.out_d2, res * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
                 return IKD_ERR_HIP;
-            if (r->out_ids && hipMemcpyAsync(b.h_ids, b.out_ids, res * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+            if (r->out_ids && hipMemcpyAsync(hids, b.out_ids, res * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
                 return IKD_ERR_HIP;
-            if (r->out_xyz && hipMemcpyAsync(b.h_xyz, b.out_xyz, res * 12, hipMemcpyDeviceToHost, st) != hipSuccess)
+            if (r->out_xyz && hipMemcpyAsync(hxyz, b.out_xyz, res * 12, hipMemcpyDeviceToHost, st) != hipSuccess)
                 return IKD_ERR_HIP;
-            if (r->out_counts && hipMemcpyAsync(b.h_cnt, b.out_cnt, n * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+            if (r->out_counts && hipMemcpyAsync(hcnt, b.out_cnt, n * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
                 return IKD_ERR_HIP;
             if (hipEventRecord(b.ev, st) != hipSuccess) return IKD_ERR_HIP;
             // The results are handed over before the insert is queued.  (Queuing
@@ -192,10 +202,12 @@ struct ikd_server_s {
             // work pile up in the shared hardware queues, which measured
             // worse for the slowest requests.)
             if (hipEventSynchronize(b.ev) != hipSuccess) return IKD_ERR_HIP;
-            if (r->out_d2) std::memcpy(r->out_d2, b.h_d2, res * 4);
-            if (r->out_ids) std::memcpy(r->out_ids, b.h_ids, res * 4);
-            if (r->out_xyz) std::memcpy(r->out_xyz, b.h_xyz, res * 12);
-            if (r->out_counts) std::memcpy(r->out_counts, b.h_cnt, n * 4);
+            if (!direct) {
+                if (r->out_d2) std::memcpy(r->out_d2, b.h_d2, res * 4);
+                if (r->out_ids) std::memcpy(r->out_ids, b.h_ids, res * 4);
+                if (r->out_xyz) std::memcpy(r->out_xyz, b.h_xyz, res * 12);
+                if (r->out_counts) std::memcpy(r->out_counts, b.h_cnt, n * 4);
+            }
         }
         if (r->downsample >= 0 && n > 0) {
             int32_t added = 0;
@@ -213,12 +225,53 @@ struct ikd_server_s {
         return ikd_synchronize(t);
     }
 
+    // The map's device bytes after a request or a maintenance step (its pool
+    // and scratch may have grown or been compacted); called with lk held.
+    void account(std::unique_lock<std::mutex>& lk, int map, ikd_tree_t tree) {
+        int64_t mb = 0;
+        lk.unlock();
+        ikd_device_bytes(tree, &mb);
+        lk.lock();
+        resident += mb - map_bytes[map];
+        map_bytes[map] = mb;
+    }
+
+    void queue_maintenance(int map) {
+        if (maintain_points != 0 && !maint_queued[map]) {
+            maint_queued[map] = 1;
+            maint_due.push_back(map);
+        }
+    }
+
+    // One step of a map's deferred rebuilds (ikd_rebuild_deferred with the
+    // maintenance budget), run only while no request is waiting and one step
+    // at a time, so that the rebuilds take the device's idle time rather than
+    // the requests'.  A request for the map that arrives meanwhile waits for
+    // this one step.  Called with lk held and busy[map] set.
+    void maintain(std::unique_lock<std::mutex>& lk, int map) {
+        ikd_tree_t tree = maps[map];
+        const int64_t budget_pts = maintain_points;
+        lk.unlock();
+        int32_t nreb = 0;
+        const int mrc = ikd_rebuild_deferred(tree, budget_pts, &nreb);
+        lk.lock();
+        maintenance_runs += nreb;
+        account(lk, map, tree);
+        // a bounded step that ran something may have left more
+        if (mrc == IKD_OK && nreb > 0 && budget_pts > 0) queue_maintenance(map);
+        busy[map] = 0;
+        --maint_running;
+        cv_done.notify_all();
+        cv_work.notify_all();
+    }
+
     void worker(int slot) {
         SlotBufs& b = bufs[slot];
         std::unique_lock<std::mutex> lk(mu);
         while (true) {
             ikd_request_t* r = nullptr;
             int64_t need = 0;
+            int mmap = -1;
             while (true) {
                 if (stop && pending.empty()) return;
                 for (auto it = pending.begin(); it != pending.end(); ++it) {
@@ -235,7 +288,24 @@ struct ikd_server_s {
                     break;
                 }
                 if (r) break;
+                // no request waits: the oldest idle map with deferred rebuilds
+                if (pending.empty() && maint_running == 0 && !stop) {
+                    for (auto it = maint_due.begin(); it != maint_due.end(); ++it) {
+                        if (busy[*it]) continue;
+                        mmap = *it;
+                        maint_due.erase(it);
+                        maint_queued[mmap] = 0;
+                        break;
+                    }
+                    if (mmap >= 0) break;
+                }
                 cv_work.wait(lk);
+            }
+            if (mmap >= 0) {
+                busy[mmap] = 1;
+                ++maint_running;
+                maintain(lk, mmap);
+                continue;
             }
             busy[r->map] = 1;
             in_flight_bytes += need;
@@ -260,32 +330,11 @@ struct ikd_server_s {
             r->delta.host_waits = c1.host_waits - c0.host_waits;
             if (rc != IKD_OK && r->status == IKD_OK) r->status = rc;
             lk.lock();
-            // hand the request back before any maintenance of its map
             done.push_back(r);
             r = nullptr;
             cv_done.notify_all();
-            // While the map is idle, run its deferred rebuilds off the
-            // requests' latency path, max_points at a time, and give the map
-            // back as soon as a request for it arrives (it waits at most one
-            // chunk).
-            while (maintain_points != 0) {
-                bool waiting = false;
-                for (ikd_request_t* c : pending) waiting = waiting || c->map == map;
-                if (waiting) break;
-                lk.unlock();
-                int32_t nreb = 0;
-                int mrc = ikd_rebuild_deferred(tree, maintain_points, &nreb);
-                lk.lock();
-                maintenance_runs += nreb;
-                if (mrc != IKD_OK || nreb == 0 || maintain_points < 0) break;
-            }
-            // the map may have grown (pool, scratch) over the request and its maintenance
-            int64_t mb = 0;
-            lk.unlock();
-            ikd_device_bytes(tree, &mb);
-            lk.lock();
-            resident += mb - map_bytes[map];
-            map_bytes[map] = mb;
+            queue_maintenance(map);
+            account(lk, map, tree);
             busy[map] = 0;
             in_flight_bytes -= need;
             --in_flight;
@@ -298,6 +347,17 @@ struct ikd_server_s {
 extern "C" {
 
 int64_t ikd_now_ns(void) { return now_ns(); }
+
+int ikd_host_alloc(int64_t bytes, void** out) {
+    if (!out || bytes <= 0) return IKD_ERR_ARG;
+    *out = nullptr;
+    return hipHostMalloc(out, (size_t)bytes, hipHostMallocDefault) == hipSuccess ? IKD_OK : IKD_ERR_NOMEM;
+}
+
+int ikd_host_free(void* p) {
+    if (!p) return IKD_OK;
+    return hipHostFree(p) == hipSuccess ? IKD_OK : IKD_ERR_HIP;
+}
 
 int ikd_server_create(int slots, int64_t hbm_budget_bytes, ikd_server_t* out) {
     if (!out || slots < 1 || slots > 64 || hbm_budget_bytes <= 0) return IKD_ERR_ARG;
@@ -317,6 +377,7 @@ int ikd_server_add_map(ikd_server_t s, ikd_tree_t tree, int32_t* out_map) {
     std::lock_guard<std::mutex> g(s->mu);
     s->maps.push_back(tree);
     s->busy.push_back(0);
+    s->maint_queued.push_back(0);
     s->map_bytes.push_back(mb);
     s->resident += mb;
     *out_map = (int32_t)(s->maps.size() - 1);
@@ -382,7 +443,7 @@ int ikd_server_pending(ikd_server_t s, int64_t* out_pending, int64_t* out_in_fli
     if (!s) return IKD_ERR_ARG;
     std::lock_guard<std::mutex> g(s->mu);
     if (out_pending) *out_pending = (int64_t)s->pending.size();
-    if (out_in_flight) *out_in_flight = s->in_flight;
+    if (out_in_flight) *out_in_flight = s->in_flight + s->maint_running + (int64_t)s->maint_due.size();
     return IKD_OK;
 }
 

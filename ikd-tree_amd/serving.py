@@ -32,8 +32,11 @@ class Request(ctypes.Structure):
         ("out_counts", ctypes.c_void_p), ("t_arrival_ns", ctypes.c_int64), ("tag", ctypes.c_int64),
         ("t_start_ns", ctypes.c_int64), ("t_done_ns", ctypes.c_int64),
         ("added", ctypes.c_int32), ("trimmed", ctypes.c_int32), ("status", ctypes.c_int32),
-        ("t_insert_done_ns", ctypes.c_int64), ("delta", _pkg.Counters),
+        ("t_insert_done_ns", ctypes.c_int64), ("delta", _pkg.Counters), ("flags", ctypes.c_int32),
     ]
+
+
+REQ_PINNED_RESULTS = 1  # IKD_REQ_PINNED_RESULTS
 
 
 def _lib():
@@ -52,6 +55,8 @@ def _lib():
             "ikd_server_set_maintenance": [vp, i64],
             "ikd_server_stats": [vp, ctypes.POINTER(i64)],
             "ikd_server_memory": [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)],
+            "ikd_host_alloc": [i64, ctypes.POINTER(vp)],
+            "ikd_host_free": [vp],
         }.items():
             fn = getattr(lib, name)
             fn.argtypes = args
@@ -64,12 +69,54 @@ def now_ns() -> int:
     return int(_lib().ikd_now_ns())
 
 
-def result_buffers(n_max: int, k: int):
+class _PinnedBlock:
+    """Page-locked host memory from ikd_host_alloc, freed with the object."""
+
+    def __init__(self, nbytes: int):
+        self._lib = _lib()
+        p = ctypes.c_void_p()
+        rc = self._lib.ikd_host_alloc(int(nbytes), ctypes.byref(p))
+        if rc != 0:
+            raise _pkg.IkdError(rc, "page-locked allocation failed")
+        self.ptr = p.value
+
+    def array(self, offset: int, shape, dtype):
+        dt = np.dtype(dtype)
+        n = int(np.prod(shape)) * dt.itemsize
+        raw = (ctypes.c_byte * max(n, 1)).from_address(self.ptr + offset)
+        raw.block = self  # the arrays keep the memory alive
+        return np.frombuffer(raw, dt, count=int(np.prod(shape))).reshape(shape)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self._lib.ikd_host_free(self.ptr)
+            self.ptr = None
+
+
+class ResultBuffers(tuple):
+    """(out_xyz, out_d2, out_ids, out_counts) for requests of up to n_max
+    queries at k neighbours; .pinned: page-locked (ikd_host_alloc)."""
+    pinned = False
+
+
+def result_buffers(n_max: int, k: int, pinned: bool = False) -> ResultBuffers:
     """Host result arrays for requests of up to n_max queries at k neighbours,
     written once so that their pages are resident: a client that reuses them
-    (submit(..., outs=...)) keeps page faults out of the workers' result copies."""
-    outs = (np.empty((n_max, k, 3), np.float32), np.empty((n_max, k), np.float32),
-            np.empty((n_max, k), np.uint32), np.empty(n_max, np.int32))
+    (submit(..., outs=...)) keeps page faults out of the workers' result
+    copies.  pinned=True puts them in page-locked memory, which the server
+    fills by DMA with no host copy of its own."""
+    shapes = (((n_max, k, 3), np.float32), ((n_max, k), np.float32), ((n_max, k), np.uint32),
+              ((n_max,), np.int32))
+    if pinned:
+        sizes = [int(np.prod(sh)) * np.dtype(dt).itemsize for sh, dt in shapes]
+        offs = [0]
+        for sz in sizes[:-1]:
+            offs.append(offs[-1] + (sz + 255) // 256 * 256)
+        block = _PinnedBlock(offs[-1] + sizes[-1] + 256)
+        outs = ResultBuffers(block.array(o, sh, dt) for o, (sh, dt) in zip(offs, shapes))
+        outs.pinned = True
+    else:
+        outs = ResultBuffers(np.empty(sh, dt) for sh, dt in shapes)
     for a in outs:
         a.fill(0)
     return outs
@@ -94,11 +141,13 @@ class Ticket:
                     or not all(a.flags.c_contiguous for a in outs)):
                 raise ValueError("result buffers do not fit the request")
             self.out_xyz, self.out_d2, self.out_ids, self.out_counts = ox[:n], od[:n], oi[:n], oc[:n]
+            pinned = bool(getattr(outs, "pinned", False))
         else:
             self.out_xyz = np.zeros((n, kk, 3), np.float32)
             self.out_d2 = np.zeros((n, kk), np.float32)
             self.out_ids = np.zeros((n, kk), np.uint32)
             self.out_counts = np.zeros(n, np.int32)
+            pinned = False
         self.trim = None if trim is None else np.ascontiguousarray(trim, np.float32).reshape(-1, 6)
         r = Request()
         r.map = map_index
@@ -115,6 +164,7 @@ class Ticket:
             r.out_ids = self.out_ids.ctypes.data
             r.out_counts = self.out_counts.ctypes.data
         r.tag = int(tag)
+        r.flags = REQ_PINNED_RESULTS if pinned else 0
         self.req = r
 
     @property
@@ -229,11 +279,13 @@ class Server:
 
 
 def simulate(server: Server, map_ids, scan_fn, rate_hz: float = 10.0, duration_s: float = 5.0, k: int = 5,
-             max_dist: float = 5.0, trim_fn=None, trim_every: int = 10, seed: int = 1005):
+             max_dist: float = 5.0, trim_fn=None, trim_every: int = 10, seed: int = 1005,
+             pinned_results: bool = True):
     """Real-time Poisson arrivals: robot r's scans arrive at rate_hz (exponential
     gaps) for duration_s seconds; scan_fn(r, j) gives robot r's j-th scan and
-    trim_fn(r, j) its trim boxes (every trim_every-th request).  Returns
-    latencies (ms) and throughput."""
+    trim_fn(r, j) its trim boxes (every trim_every-th request).  Each robot's
+    client reuses its result buffers (page-locked with pinned_results).
+    Returns latencies (ms) and throughput."""
     rng = np.random.default_rng(seed)
     events = []
     for r, m in enumerate(map_ids):
@@ -251,7 +303,7 @@ def simulate(server: Server, map_ids, scan_fn, rate_hz: float = 10.0, duration_s
     # each robot's result buffers are reused once its request is back, as a
     # client keeps them (two sets per robot to start; more if requests queue)
     n_max = max(s.shape[0] for s in scans.values())
-    free = {r: [result_buffers(n_max, k) for _ in range(2)] for r in range(len(map_ids))}
+    free = {r: [result_buffers(n_max, k, pinned_results) for _ in range(2)] for r in range(len(map_ids))}
 
     def collect(got):
         for d in got:
@@ -262,20 +314,33 @@ def simulate(server: Server, map_ids, scan_fn, rate_hz: float = 10.0, duration_s
     t0 = now_ns() + 50_000_000
     done = []
     points = 0
+    def wait_until(due):
+        """Collect finished requests until the clock reaches due, submitting
+        late by tens of microseconds at most (a late submission would count as
+        queue wait): ms-granular waits while far, short sleeps, then a spin."""
+        while True:
+            left = (due - now_ns()) / 1e6
+            if left <= 0:
+                return
+            if left > 1.5:
+                t_call = now_ns()
+                got = server.wait(timeout_ms=int(left - 0.5))
+                collect(got)
+                if not got and now_ns() - t_call < 200_000:  # idle server: wait() returned at once
+                    time.sleep(min(left - 0.5, 1.0) / 1e3)
+            elif left > 0.3:
+                collect(server.wait(timeout_ms=0))
+                time.sleep((left - 0.25) / 1e3)
+            else:
+                collect(server.wait(timeout_ms=0))
+
     for (t, r, m, j) in events:
         due = t0 + int(t * 1e9)
-        while True:
-            wait_ms = (due - now_ns()) / 1e6
-            if wait_ms <= 0:
-                break
-            got = server.wait(timeout_ms=max(1, int(wait_ms)))
-            collect(got)
-            if not got:  # idle server: wait() returned at once
-                time.sleep(min(wait_ms, 1.0) / 1e3)
+        wait_until(due)
         trim = trim_fn(r, j) if (trim_fn is not None and j % trim_every == trim_every - 1) else None
         s = scans[(r, j)]
         points += s.shape[0]
-        outs = free[r].pop() if free[r] else result_buffers(n_max, k)
+        outs = free[r].pop() if free[r] else result_buffers(n_max, k, pinned_results)
         server.submit(m, s, k, max_dist, 1, trim, tag=r, arrival_ns=due, outs=outs)
     collect(server.drain())
     t1 = max(d.req.t_done_ns for d in done)
@@ -302,6 +367,7 @@ def simulate(server: Server, map_ids, scan_fn, rate_hz: float = 10.0, duration_s
         "failed": len(bad),
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "max": float(lat.max()), "mean": float(lat.mean())},
+        "pinned_results": bool(pinned_results),
         "queue_wait_ms": pct(wait),
         "run_ms": pct(run),
         "latency_ms_by_window": by_window,

@@ -221,7 +221,12 @@ int ikd_set_tuning(ikd_tree_t t, int key, int value);
 /* Run the deferred rebuilds that are still due, in the order they were
  * listed, until their live points exceed max_points (at least one; <= 0: no
  * limit); *out_rebuilt gets the number run.  Rebuild_Thread's work
- * (ikd_Tree.cpp:200-350), run when the caller has time, e.g. between scans. */
+ * (ikd_Tree.cpp:200-350), run when the caller has time, e.g. between scans.
+ * With IKD_TUNE_DEFER_REBUILD on, an update that leaves the node pool more
+ * than 4x live points + 4M nodes leaves its reclamation (a whole-tree
+ * rebuild) to this call, which then does only that, whatever max_points
+ * says; past 8x live + 8M nodes, or when the tree is too deep for the
+ * traversal stacks, the update reclaims at once. */
 int ikd_rebuild_deferred(ikd_tree_t t, int64_t max_points, int32_t* out_rebuilt);
 
 /* Payload-id housekeeping (no counterpart in the reference; for callers that
@@ -359,7 +364,20 @@ typedef struct ikd_request {
     int32_t added, trimmed, status;
     int64_t t_insert_done_ns; /* when Add_Points (and the kNN before it) had finished */
     ikd_counters_t delta;     /* the map's counters over this request */
+    /* set by the caller */
+    int32_t flags;            /* IKD_REQ_* bits */
 } ikd_request_t;
+
+/* out_xyz, out_d2, out_ids and out_counts are page-locked memory from
+ * ikd_host_alloc: the results are copied into them by DMA, with no host copy
+ * through the server's own buffers. */
+#define IKD_REQ_PINNED_RESULTS 1
+
+/* Page-locked host memory (hipHostMalloc) for request buffers; free it with
+ * ikd_host_free once no request that names it is in the server.  No
+ * counterpart in the reference, whose searches write host vectors. */
+int ikd_host_alloc(int64_t bytes, void** out);
+int ikd_host_free(void* p);
 
 /* Monotonic clock of the server (ns). */
 int64_t ikd_now_ns(void);
@@ -371,12 +389,15 @@ int ikd_server_submit(ikd_server_t s, ikd_request_t* req);
 /* Hand back up to max_out finished requests (timeout_ms < 0: wait for one, or
  * until nothing is pending or running; 0: do not wait). */
 int ikd_server_wait(ikd_server_t s, ikd_request_t** out, int32_t max_out, int32_t timeout_ms, int32_t* out_n);
+/* Requests waiting; requests running plus maintenance steps running or
+ * queued (both 0: the server is idle). */
 int ikd_server_pending(ikd_server_t s, int64_t* out_pending, int64_t* out_in_flight);
-/* After a request, while no other request for the same map is pending, the
- * worker runs ikd_rebuild_deferred(map, max_points) repeatedly until the
- * map's deferred list is empty, then takes new work; a request for the map
- * that arrives meanwhile waits for at most one such chunk (-1, the default:
- * one call with no limit; 0: never).  Deferral itself is per tree
+/* A map that has served a request is queued for maintenance: while no
+ * request is waiting, one worker at a time runs ikd_rebuild_deferred(map,
+ * max_points) on the oldest queued map that is not busy, and queues the map
+ * again while such steps still find work.  A request for a map in
+ * maintenance waits for at most that one step (-1, the default: one call
+ * with no limit; 0: never).  Deferral itself is per tree
  * (IKD_TUNE_DEFER_REBUILD). */
 int ikd_server_set_maintenance(ikd_server_t s, int64_t max_points);
 /* Deferred rebuilds the workers have run so far. */

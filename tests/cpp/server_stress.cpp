@@ -5,8 +5,10 @@
 // succeed and nothing may touch freed or out-of-bounds memory.  This is the
 // path (worker -> ikd_rebuild_deferred -> rebuild -> build -> scan) of a host
 // segmentation fault once seen under a profiler on the GPU.
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <thread>
 #include <vector>
 
 #include "ikd_tree_amd.h"
@@ -76,8 +78,20 @@ int main() {
         }
         done += got;
     }
+    // let the workers finish the maps' maintenance, then require that some ran
+    for (int spin = 0;; ++spin) {
+        int64_t pend = 0, running = 0;
+        ikd_server_pending(srv, &pend, &running);
+        if (pend == 0 && running == 0) break;
+        if (spin > 600000) return 7;  // 10 minutes
+        std::this_thread::sleep_for(std::chrono::milliseconds(1));
+    }
     int64_t rebuilt = 0;
     ikd_server_stats(srv, &rebuilt);
+    if (rebuilt == 0) {
+        std::printf("no deferred rebuild ran\n");
+        return 8;
+    }
     ikd_server_destroy(srv);
     for (auto* r : reqs) {
         std::free(r->out_d2);

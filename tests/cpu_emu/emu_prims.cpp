@@ -20,6 +20,7 @@ hipError_t prim_sort_pairs_u64(void* tmp, size_t& tmp_bytes, const unsigned long
         tmp_bytes = 16;
         return hipSuccess;
     }
+    emu_tally("prim:sort_pairs_u64");
     std::vector<int> idx(n);
     std::iota(idx.begin(), idx.end(), 0);
     std::stable_sort(idx.begin(), idx.end(),
@@ -41,6 +42,7 @@ hipError_t prim_sort_keys_u64(void* tmp, size_t& tmp_bytes, const unsigned long 
         tmp_bytes = 16;
         return hipSuccess;
     }
+    emu_tally("prim:sort_keys_u64");
     std::vector<unsigned long long> k2(kin, kin + n);
     std::stable_sort(k2.begin(), k2.end(), [&](unsigned long long a, unsigned long long b) {
         return bits(a, begin_bit, end_bit) < bits(b, begin_bit, end_bit);
@@ -55,6 +57,7 @@ static hipError_t exscan(void* tmp, size_t& tmp_bytes, const T* in, T* out, int 
         tmp_bytes = 16;
         return hipSuccess;
     }
+    emu_tally("prim:exscan");
     std::vector<T> v(in, in + n);
     T acc = 0;
     for (int i = 0; i < n; ++i) {

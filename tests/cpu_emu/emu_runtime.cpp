@@ -10,7 +10,9 @@
 #include <setjmp.h>
 
 #include <cstdio>
+#include <map>
 #include <mutex>
+#include <string>
 #include <set>
 #include <typeindex>
 #include <vector>
@@ -218,6 +220,31 @@ hipError_t hipGraphDestroy(hipGraph_t g) {
     return hipSuccess;
 }
 
+static std::mutex g_tally_mu;
+static std::map<std::string, long long> g_tally;
+void emu_tally(const char* name) {
+    std::lock_guard<std::mutex> g(g_tally_mu);
+    ++g_tally[name];
+}
+extern "C" void emu_dispatch_reset() {
+    std::lock_guard<std::mutex> g(g_tally_mu);
+    g_tally.clear();
+}
+// "name count" lines into buf (truncated to len); returns the total count
+extern "C" long long emu_dispatch_tally(char* buf, int len) {
+    std::lock_guard<std::mutex> g(g_tally_mu);
+    std::string out;
+    long long total = 0;
+    for (auto& kv : g_tally) {
+        out += kv.first + " " + std::to_string(kv.second) + "\n";
+        total += kv.second;
+    }
+    if (buf && len > 0) {
+        std::snprintf(buf, (size_t)len, "%s", out.c_str());
+    }
+    return total;
+}
+
 void emu_launch(dim3 grid, dim3 block, size_t shmem, const std::function<void()>& body) {
     if (t_capture) {
         t_capture->nodes.push_back(emu_node{grid, block, shmem, body});
@@ -306,8 +333,12 @@ hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) {
     if (n) std::memmove(d, s, n);
     return hipSuccess;
 }
-hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t) { return hipMemcpy(d, s, n, k); }
+hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t) {
+    emu_tally(k == hipMemcpyHostToDevice ? "copy:h2d" : k == hipMemcpyDeviceToHost ? "copy:d2h" : "copy:d2d");
+    return hipMemcpy(d, s, n, k);
+}
 hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) {
+    emu_tally("memset");
     std::memset(d, v, n);
     return hipSuccess;
 }
@@ -321,7 +352,10 @@ hipError_t hipStreamDestroy(hipStream_t s) {
     delete s;
     return hipSuccess;
 }
-hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+hipError_t hipStreamSynchronize(hipStream_t) {
+    emu_tally("wait:stream");
+    return hipSuccess;
+}
 hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
 hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) {
     *e = new emu_event{0};
@@ -332,7 +366,10 @@ hipError_t hipEventDestroy(hipEvent_t e) {
     return hipSuccess;
 }
 hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
-hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+hipError_t hipEventSynchronize(hipEvent_t) {
+    emu_tally("wait:event");
+    return hipSuccess;
+}
 hipError_t hipGetSymbolAddress(void** p, const void* symbol) {
     *p = const_cast<void*>(symbol);
     return hipSuccess;

@@ -177,4 +177,8 @@ hipError_t hipGraphExecDestroy(hipGraphExec_t e);
 hipError_t hipGraphDestroy(hipGraph_t g);
 
 void emu_launch(dim3 grid, dim3 block, size_t shmem, const std::function<void()>& body);
-#define hipLaunchKernelGGL(F, G, B, S, ST, ...) emu_launch(dim3(G), dim3(B), (S), [=]() { F(__VA_ARGS__); })
+// device operations by name (kernels, copies, fills, primitive calls), for
+// counting what one library call issues: emu_dispatch_tally / _reset
+void emu_tally(const char* name);
+#define hipLaunchKernelGGL(F, G, B, S, ST, ...) \
+    (emu_tally(#F), emu_launch(dim3(G), dim3(B), (S), [=]() { F(__VA_ARGS__); }))

@@ -101,3 +101,49 @@ def test_budgeted_deferred_rebuilds(ikd):
         _same_queries(t, ref, rng)
         assert total < 10_000
     assert total > 0
+
+
+def test_sparse_pool_reclaimed_by_deferred_call(ikd):
+    """With deferral on, an update that leaves the node pool mostly garbage
+    does not rebuild the whole tree itself: the next rebuild_deferred does
+    (whatever its point budget), and the live set, kNN and box results match
+    the tree that reclaimed inside its updates."""
+    rng = np.random.default_rng(46)
+    base = uniform_cloud(rng, 50_000, -50, 50)
+    plain = ikd.KD_TREE(0.5, 0.6, 0.5)
+    lazy = ikd.KD_TREE(0.5, 0.6, 0.5)
+    lazy.set_tuning(ikd.IKD_TUNE_DEFER_REBUILD, 2000)
+    for t in (plain, lazy):
+        t.Build(base)
+    far = np.array([[90.0, -60.0, -60.0, 210.0, 60.0, 60.0]], np.float32)
+    in_deferred = 0
+    for step in range(6):
+        # a million points away from the base cloud, then deleted again: the
+        # pool keeps their nodes as garbage
+        pts = uniform_cloud(rng, 1_000_000, 100, 200)
+        pts[:, 1:] -= 150.0
+        ids = np.arange(1_000_000 * (step + 1), 1_000_000 * (step + 2), dtype=np.uint32)
+        c0 = lazy.counters()["compactions"]
+        assert plain.Add_Points(pts, False, ids) == lazy.Add_Points(pts, False, ids)
+        assert plain.Delete_Point_Boxes(far) == lazy.Delete_Point_Boxes(far) == 1_000_000
+        assert lazy.counters()["compactions"] == c0, f"step {step}: an update reclaimed the pool itself"
+        assert plain.validnum() == lazy.validnum() == 50_000
+        _same_queries(plain, lazy, rng)
+        # the caller's idle time: one subtree's worth of budget
+        while lazy.rebuild_deferred(1000) > 0:
+            pass
+        in_deferred += lazy.counters()["compactions"] - c0
+    assert plain.counters()["compactions"] >= 1
+    assert in_deferred >= 1
+    assert lazy.pool_stats()["used_nodes"] < 4 * 50_000 + (1 << 22)
+    a, ia = plain.flatten()
+    b, ib = lazy.flatten()
+    assert_same_multiset(a, b)
+    assert np.array_equal(np.sort(ia), np.sort(ib))
+    _same_queries(plain, lazy, rng)
+    boxes = random_boxes(rng, 10, -50, 50, 5, 30)
+    go, _, gi = lazy.box_search_batch(boxes)
+    po, _, pi = plain.box_search_batch(boxes)
+    assert np.array_equal(go, po)
+    for i in range(10):
+        assert np.array_equal(np.sort(gi[go[i]:go[i + 1]]), np.sort(pi[po[i]:po[i + 1]]))

@@ -63,10 +63,12 @@ def test_mixed_equals_alone(ikd, serving, scenes, defer):
         trees.append(t)
     srv = serving.Server(slots=4, hbm_budget_bytes=8 << 30)
     ids = [srv.add_map(t, defer_rebuild_points=defer) for t in trees]
-    # every other request writes into caller-owned result buffers larger than it needs
+    # requests write into their own arrays, into caller-owned buffers larger
+    # than they need, or into page-locked caller buffers (filled by DMA)
     tickets = [srv.submit(ids[m], pts, 5, 5.0, 1, trim, tag=i,
-                          outs=serving.result_buffers(len(pts) + 97, 5) if i % 2 else None)
+                          outs=None if i % 3 == 0 else serving.result_buffers(len(pts) + 97, 5, i % 3 == 2))
                for i, (m, pts, trim) in enumerate(reqs)]
+    assert [t.req.flags for t in tickets[:3]] == [0, 0, serving.REQ_PINNED_RESULTS]
     done = srv.drain()
     srv.close()
     assert len(done) == len(tickets)
