@@ -1191,18 +1191,19 @@ int delete_boxes_dev(ikd_tree_t t, const float* d_boxes, int nb, int ds) {
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, t->d_hdr, 2);
     launch_fill_i32(t->d_hdr->front3, 0, 3, st);
     launch_fill_i32(&t->d_hdr->front_over, 0, 1, st);
-    // levels until the frontier is empty (the host's depth may lag by a call)
-    int L0 = 0, L1 = t->max_depth + 2;
+    // steps of BOXDEL_LEVELS levels until the frontier is empty (the host's
+    // depth may lag by a call)
+    int S0 = 0, S1 = (t->max_depth + 2 + BOXDEL_LEVELS - 1) / BOXDEL_LEVELS;
     while (true) {
-        launch_box_delete_bfs(t->pool, d_boxes, nb, f0, f1, (int)fcap, L0, L1, recs, t->d_hdr, (int)cap, st);
+        launch_box_delete_bfs(t->pool, d_boxes, nb, f0, f1, (int)fcap, S0, S1, recs, t->d_hdr, (int)cap, st);
         CHK(sync_hdr(t));
-        if (t->h_hdr->front_over || t->h_hdr->front3[L1 % 3] == 0) break;
-        if (L1 > 4 * MAX_DEPTH) {
+        if (t->h_hdr->front_over || t->h_hdr->front3[S1 % 3] == 0) break;
+        if (S1 * BOXDEL_LEVELS > 4 * MAX_DEPTH) {
             t->err = "box delete: frontier did not drain";
             return IKD_ERR_INTERNAL;
         }
-        L0 = L1;
-        L1 += 8;
+        S0 = S1;
+        S1 += 2;
     }
     int nrec = t->h_hdr->rec_count;
     CHK(check_count(t, "box delete: records", nrec, 0, INT_MAX));

@@ -38,6 +38,7 @@ struct DevHdr {
 };
 static_assert(sizeof(DevHdr) % sizeof(int) == 0, "the header is cleared as ints");
 constexpr int DEFER_CAP = 4096;  // deferred-rebuild list entries
+constexpr int BOXDEL_LEVELS = 4;  // tree levels one box-delete launch takes each frontier pair down
 
 struct Pool {
     NodeHot* hot;

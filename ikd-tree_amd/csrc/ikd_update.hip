@@ -724,9 +724,12 @@ constexpr uint32_t REC_COVER = 0x80000000u;
 constexpr uint32_t REC_PARTIAL = 0x40000000u;  // a node the box cuts: Updated, labels unchanged
 
 // Pass 1 runs breadth-first over all boxes at once: a frontier of (node, box)
-// pairs per tree level, one lane per pair, so a box that cuts through much of
-// the tree (a sliding-window trim) is walked by many lanes instead of one.
-// front3[lvl % 3] holds the level's frontier size.
+// pairs, one lane per pair, so a box that cuts through much of the tree (a
+// sliding-window trim) is walked by many lanes instead of one.  Each launch
+// (a step) takes every pair BOXDEL_LEVELS tree levels down, depth first with
+// a small stack per lane, and hands the pairs below on as the next frontier;
+// front3[step % 3] holds the step's frontier size.  Every (node, box) pair the
+// box cuts is visited once, whatever the step depth.
 __global__ void k_box_delete_init(Pool pool, const float* __restrict__ boxes, int nb, int2* __restrict__ front,
                                   unsigned int* __restrict__ recs, DevHdr* hdr, int cap) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -746,54 +749,68 @@ __global__ void k_box_delete_init(Pool pool, const float* __restrict__ boxes, in
 }
 
 __global__ void k_box_delete_level(Pool pool, const float* __restrict__ boxes, const int2* __restrict__ front,
-                                   int lvl, int2* __restrict__ front_next, unsigned int* __restrict__ recs,
+                                   int step, int2* __restrict__ front_next, unsigned int* __restrict__ recs,
                                    DevHdr* hdr, int fcap, int cap) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid == 0) hdr->front3[(lvl + 2) % 3] = 0;
-    const int count = min(hdr->front3[lvl % 3], fcap);
+    if (tid == 0) hdr->front3[(step + 2) % 3] = 0;
+    const int count = min(hdr->front3[step % 3], fcap);
     for (int i = tid; i < count; i += gridDim.x * blockDim.x) {
         const int2 e = front[i];
-        const int u = e.x;
         const float* box = boxes + 6 * e.y;
-        float f[16];
-        load_record(pool.hot[u].f, f);
-        const uint32_t lk = fbits(f[3]);
-        // a node the box cuts is Updated on the way out (cpp:703-704)
-        int pos = atomicAdd(&hdr->rec_count, 1);
-        if (pos < cap) recs[pos] = REC_PARTIAL | (uint32_t)u;
-        if (!(lk & LK_PDEL) && box_contains(box, f[0], f[1], f[2])) {
-            pos = atomicAdd(&hdr->rec_count, 1);
-            if (pos < cap) recs[pos] = (uint32_t)u;
-        }
-        const int pair = (int)lk_pair(lk);
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            if (!(lk & lk_has(s)) || (lk & lk_td(s))) continue;
-            const float* cb = f + 4 + 6 * s;
-            if (box_disjoint(box, cb)) continue;
-            const int c = 2 * pair + s;
-            if (box_covers(box, cb)) {
+        // depth-first below e.x: a node at local depth d pushes its cut
+        // children at d + 1 < BOXDEL_LEVELS (at most one pending sibling per
+        // level, so BOXDEL_LEVELS entries hold the stack)
+        int stk[BOXDEL_LEVELS], sdep[BOXDEL_LEVELS];
+        int top = 0;
+        int u = e.x, d = 0;
+        while (true) {
+            float f[16];
+            load_record(pool.hot[u].f, f);
+            const uint32_t lk = fbits(f[3]);
+            // a node the box cuts is Updated on the way out (cpp:703-704)
+            int pos = atomicAdd(&hdr->rec_count, 1);
+            if (pos < cap) recs[pos] = REC_PARTIAL | (uint32_t)u;
+            if (!(lk & LK_PDEL) && box_contains(box, f[0], f[1], f[2])) {
                 pos = atomicAdd(&hdr->rec_count, 1);
-                if (pos < cap) recs[pos] = REC_COVER | (uint32_t)c;
-            } else {
-                pos = atomicAdd(&hdr->front3[(lvl + 1) % 3], 1);
-                if (pos < fcap) front_next[pos] = make_int2(c, e.y);
-                else hdr->front_over = 1;
+                if (pos < cap) recs[pos] = (uint32_t)u;
             }
+            const int pair = (int)lk_pair(lk);
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                if (!(lk & lk_has(s)) || (lk & lk_td(s))) continue;
+                const float* cb = f + 4 + 6 * s;
+                if (box_disjoint(box, cb)) continue;
+                const int c = 2 * pair + s;
+                if (box_covers(box, cb)) {
+                    pos = atomicAdd(&hdr->rec_count, 1);
+                    if (pos < cap) recs[pos] = REC_COVER | (uint32_t)c;
+                } else if (d + 1 < BOXDEL_LEVELS && top < BOXDEL_LEVELS) {
+                    stk[top] = c;
+                    sdep[top++] = d + 1;
+                } else {
+                    pos = atomicAdd(&hdr->front3[(step + 1) % 3], 1);
+                    if (pos < fcap) front_next[pos] = make_int2(c, e.y);
+                    else hdr->front_over = 1;
+                }
+            }
+            if (top == 0) break;
+            --top;
+            u = stk[top];
+            d = sdep[top];
         }
     }
 }
 
-void launch_box_delete_bfs(Pool pool, const float* boxes, int nb, int2* f0, int2* f1, int fcap, int L0, int L1,
+void launch_box_delete_bfs(Pool pool, const float* boxes, int nb, int2* f0, int2* f1, int fcap, int S0, int S1,
                            unsigned int* recs, DevHdr* hdr, int cap, hipStream_t s) {
     if (nb <= 0) return;
-    if (L0 == 0)
+    if (S0 == 0)
         hipLaunchKernelGGL(k_box_delete_init, dim3(blocks_for(nb, 64)), dim3(64), 0, s, pool, boxes, nb, f0, recs,
                            hdr, cap);
-    for (int L = L0; L < L1; ++L) {
-        const int2* fin = (L & 1) ? f1 : f0;
-        int2* fout = (L & 1) ? f0 : f1;
-        hipLaunchKernelGGL(k_box_delete_level, dim3(512), dim3(256), 0, s, pool, boxes, fin, L, fout, recs, hdr, fcap,
+    for (int S = S0; S < S1; ++S) {
+        const int2* fin = (S & 1) ? f1 : f0;
+        int2* fout = (S & 1) ? f0 : f1;
+        hipLaunchKernelGGL(k_box_delete_level, dim3(512), dim3(256), 0, s, pool, boxes, fin, S, fout, recs, hdr, fcap,
                            cap);
     }
 }
