@@ -33,9 +33,10 @@ against one call per tree and (k, max_dist) group.
 
 Phase D.  --robots maps of 0.5-3M points served by the request loop with
 Poisson scan arrivals at 10 Hz per robot: p50/p99 of completion - arrival.
-Subtrees above --serve-defer live points are rebuilt between a map's
-requests, --serve-chunk points at a time; each finished request carries the
-map's event counts (rebuilds, rebuilt points, allocations, host waits).
+Subtrees above --serve-defer live points are rebuilt while no request
+waits, --serve-chunk points at a time; each finished request carries the
+map's event counts (rebuilds, rebuilt points, allocations, host waits), and
+the report gives p50/p99 per 5 s of arrivals.
 
 The CPU baseline is the oracle (CPU restatement of the reference) on a bounded
 sample of the same queries and map.
@@ -80,7 +81,7 @@ def main():
     ap.add_argument("--box-points", type=int, default=5_000_000,
                     help="map points of the box/radius/trim phase (0 skips it)")
     ap.add_argument("--serve-seconds", type=float, default=30.0, help="simulated arrival window of the serving phase")
-    ap.add_argument("--serve-defer", type=int, default=1_000,
+    ap.add_argument("--serve-defer", type=int, default=128,
                     help="serving: subtrees above this many live points are rebuilt between requests")
     ap.add_argument("--serve-slots", type=int, default=8, help="serving: requests on the device at once")
     ap.add_argument("--serve-chunk", type=int, default=100_000,
