@@ -52,6 +52,13 @@ __global__ void k_link_bits(NodeHot* hot, int node, uint32_t set, uint32_t clear
     *p = (v | set) & ~clear;
 }
 
+// items[i] = (parent, link bits to clear): empty child slots, several of
+// which may share a parent's link word
+__global__ void k_clear_links(NodeHot* hot, const int2* __restrict__ items, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAnd(reinterpret_cast<uint32_t*>(&hot[items[i].x].f[3]), ~(uint32_t)items[i].y);
+}
+
 __global__ void k_init_pool(Pool pool, DevHdr* hdr) {
     // sentinel (node 0): its left child slot is the root (node 2, pair 1)
     float* f = pool.hot[SENTINEL].f;
@@ -276,7 +283,7 @@ enum Buf {
     B_PFLAG, B_PSCAN, B_ISEG, B_ISEGD, B_IO0, B_IO1, B_IO2, B_IO3, B_IO4, B_VSTART, B_DELBOX, B_DELFL, B_INSPT,
     B_INSFL, B_CBOX, B_CPTS, B_MISC, B_MORTK, B_MORTV, B_MORTK2, B_MORTV2, B_CNT64, B_OFF64, B_ADDX, B_ADDI, B_PNEXT, B_SPAIRS, B_SPSCAN,
     B_DEFER, B_DINFO, B_DBEFORE, B_DKEEP, B_PTS3, B_PEND, B_START, B_INV0, B_LASTP, B_PSEED, B_DBELOW, B_MAXDEL,
-    B_SLISTS, B_ENUMROOT, B_ENUMGRP, B_TOPT, B_TOPI, B_TCNT64, B_TOFF64, B_RTASKS, B_ICNT64, B_COUNT
+    B_SLISTS, B_ENUMROOT, B_ENUMGRP, B_TOPT, B_TOPI, B_TCNT64, B_TOFF64, B_RTASKS, B_ICNT64, B_EMPTYL, B_COUNT
 };
 static_assert(B_COUNT <= 96, "scratch table");
 
@@ -966,6 +973,7 @@ int rebuild_listed(ikd_tree_t t, const std::vector<int4>& hinfo, const std::vect
     long long total_nodes = 0, total_live = 0;
     int maxlen = 0;
     bool root_rebuilt = false;
+    std::vector<int2> empties;  // (parent, child-slot bits) of all-deleted subtrees
     for (int i = 0; i < ntop; ++i) {
         int u = hinfo[i].x, live = hinfo[i].y, depth = hinfo[i].z, parent = hinfo[i].w;
         CHK(check_count(t, "rebuild: subtree root", u, ROOT, t->cap - 1));
@@ -977,8 +985,7 @@ int rebuild_listed(ikd_tree_t t, const std::vector<int4>& hinfo, const std::vect
         if (live <= 0) {
             // Rebuild of an all-deleted subtree leaves an empty slot (BuildTree of
             // an empty range, cpp:574).
-            uint32_t clr = lk_has(u & 1) | lk_td(u & 1);
-            hipLaunchKernelGGL(k_link_bits, dim3(1), dim3(1), 0, st, t->pool.hot, parent, 0u, clr);
+            empties.push_back(make_int2(parent, (int)(lk_has(u & 1) | lk_td(u & 1))));
             continue;
         }
         segs.push_back(make_int4((int)total_live, (int)(total_live + live - 1), u, parent));
@@ -991,6 +998,13 @@ int rebuild_listed(ikd_tree_t t, const std::vector<int4>& hinfo, const std::vect
         total_live += live;
         t->ctr.rebuilt_points += live;
         maxlen = std::max(maxlen, live);
+    }
+    if (!empties.empty()) {
+        const int ne = (int)empties.size();
+        int2* de;
+        CHK(ensure(t, B_EMPTYL, ne, &de));
+        CHK(stage_copy(t, de, empties.data(), ne * sizeof(int2)));
+        hipLaunchKernelGGL(k_clear_links, dim3(nblk(ne)), dim3(BS), 0, st, t->pool.hot, de, ne);
     }
     if (!roots.empty()) {
         float4* pts;
@@ -1330,7 +1344,7 @@ int add_points_ds_chunk(ikd_tree_t t, const float* d_xyz, const uint32_t* d_ids,
         hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, t->d_hdr, 2 | 8);
         launch_voxel_fold2(t->pool, pts, vb, kb, vstart, n, n, t->prm.box_length, check, delbox, delfl, inspt, insfl,
                            t->d_hdr, t->pool_used + 16, deleting ? dseeds : nullptr, (int)dcap, fold_stack, fold_lanes,
-                           st);
+                           bad, st);
         int hbad = 0;
         HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
         CHK(sync_hdr(t));

@@ -175,7 +175,7 @@ void launch_voxel_starts(const int* flag, const int* scan, int n, int* vstart, D
 void launch_voxel_fold2(Pool pool, const float4* pts, const uint32_t* order, const unsigned long long* keys,
                         const int* vstart, int n, int bound, float ds, int check, float* del_boxes, int* del_flag,
                         float4* ins_pts, int* ins_flag, DevHdr* hdr, long long max_steps, int* del_out, int del_cap,
-                        int stack_cap, int lanes, hipStream_t s);
+                        int stack_cap, int lanes, const int* bad, hipStream_t s);
 void launch_compact_boxes(const int* flag, const int* scan, const float* in, int bound, DevHdr* hdr, float* out,
                           int* out_count, hipStream_t s);
 void launch_compact_pts(const int* flag, const int* scan, const float4* in, int bound, DevHdr* hdr, float4* out,

@@ -123,11 +123,13 @@ __global__ void k_voxel_fold(Pool pool, const float4* __restrict__ pts, const ui
                              const unsigned long long* __restrict__ keys, const int* __restrict__ vstart, int n,
                              int bound, float ds, int check, float* __restrict__ del_boxes, int* __restrict__ del_flag,
                              float4* __restrict__ ins_pts, int* __restrict__ ins_flag, DevHdr* hdr,
-                             long long max_steps, int* __restrict__ del_out, int del_cap, int stack_cap) {
+                             long long max_steps, int* __restrict__ del_out, int del_cap, int stack_cap,
+                             const int* __restrict__ bad) {
     // per-lane depth-first stacks in LDS, [stack_cap][64]
     extern __shared__ int lds_dyn[];
     const int lane = threadIdx.x;
     int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (*bad) return;  // a voxel index out of range: the call fails and must change nothing
     if (v >= bound || v >= hdr->group_count) return;
     int j0 = vstart[v], j1 = vstart[v + 1];
     float4 P0 = pts[order[j0]];
@@ -399,8 +401,10 @@ __global__ __launch_bounds__(64) void k_voxel_fold_team(Pool pool, const float4*
                                                         int check, float* __restrict__ del_boxes,
                                                         int* __restrict__ del_flag, float4* __restrict__ ins_pts,
                                                         int* __restrict__ ins_flag, DevHdr* hdr, long long max_steps,
-                                                        int* __restrict__ del_out, int del_cap, int stack_cap) {
+                                                        int* __restrict__ del_out, int del_cap, int stack_cap,
+                                                        const int* __restrict__ bad) {
     constexpr int TEAMS = 64 / T;
+    if (*bad) return;  // a voxel index out of range: the call fails and must change nothing (uniform exit)
     extern __shared__ int lds_dyn[];  // per-lane deques, [stack_cap][64]
     __shared__ int dn[TEAMS][T];
     const int lane = threadIdx.x;
@@ -571,13 +575,13 @@ void launch_voxel_starts(const int* flag, const int* scan, int n, int* vstart, D
 void launch_voxel_fold2(Pool pool, const float4* pts, const uint32_t* order, const unsigned long long* keys,
                         const int* vstart, int n, int bound, float ds, int check, float* del_boxes, int* del_flag,
                         float4* ins_pts, int* ins_flag, DevHdr* hdr, long long max_steps, int* del_out, int del_cap,
-                        int stack_cap, int lanes, hipStream_t s) {
+                        int stack_cap, int lanes, const int* bad, hipStream_t s) {
     if (bound <= 0) return;
     const size_t lds = (size_t)stack_cap * 64 * sizeof(int);
 #define IKD_FOLD_TEAM(T)                                                                                            \
     hipLaunchKernelGGL(k_voxel_fold_team<T>, dim3(blocks_for(bound, 64 / (T))), dim3(64), lds, s, pool, pts, order, \
                        keys, vstart, n, bound, ds, check, del_boxes, del_flag, ins_pts, ins_flag, hdr, max_steps,     \
-                       del_out, del_cap, stack_cap)
+                       del_out, del_cap, stack_cap, bad)
     switch (lanes) {
         case 4: IKD_FOLD_TEAM(4); break;
         case 8: IKD_FOLD_TEAM(8); break;
@@ -585,7 +589,7 @@ void launch_voxel_fold2(Pool pool, const float4* pts, const uint32_t* order, con
         default:
             hipLaunchKernelGGL(k_voxel_fold, dim3(blocks_for(bound, 64)), dim3(64), lds, s, pool, pts, order, keys,
                                vstart, n, bound, ds, check, del_boxes, del_flag, ins_pts, ins_flag, hdr, max_steps,
-                               del_out, del_cap, stack_cap);
+                               del_out, del_cap, stack_cap, bad);
     }
 #undef IKD_FOLD_TEAM
 }

@@ -253,6 +253,31 @@ def test_argument_errors(ikd):
     assert e.value.code == ikd.IKD_ERR_UNSUPPORTED
 
 
+def test_downsample_out_of_range_voxel_changes_nothing(ikd):
+    """A downsampled batch holding a point whose voxel index floor(p / box)
+    is beyond +-2^20 is refused (IKD_ERR_UNSUPPORTED) before any of its
+    deletions lands: the tree keeps every point and answers as before."""
+    rng = np.random.default_rng(5)
+    g = ikd.KD_TREE(0.5, 0.6, 0.5)
+    base = uniform_cloud(rng, 5000, -5, 5)
+    g.Build(base)
+    q = uniform_cloud(rng, 500, -6, 6)
+    _, d0, i0, c0 = g.knn(q, 5)
+    before, bid = g.flatten()
+    batch = uniform_cloud(rng, 2000, -5, 5)  # its voxels hold base points the fold would delete
+    batch[17] = [1e9, 0.25, 0.25]
+    with pytest.raises(ikd.IkdError) as e:
+        g.Add_Points(batch, True)
+    assert e.value.code == ikd.IKD_ERR_UNSUPPORTED
+    after, aid = g.flatten()
+    assert g.validnum() == 5000
+    assert_same_multiset(before, after)
+    assert np.array_equal(np.sort(bid), np.sort(aid))
+    _, d1, i1, c1 = g.knn(q, 5)
+    assert np.array_equal(c0, c1) and np.array_equal(d0.view(np.uint32), d1.view(np.uint32))
+    assert np.array_equal(i0, i1)
+
+
 @pytest.mark.parametrize("grid", [False, True])
 def test_box_search(ikd, grid):
     rng = np.random.default_rng(20 + grid)
