@@ -389,6 +389,26 @@ def test_add_points_downsample_vs_oracle(ikd, ds):
         assert_same_multiset(g.flatten()[0], o.flatten()[0], f"downsampled live set, batch {it}")
 
 
+@pytest.mark.parametrize("ds", [0.5, 0.3])
+def test_downsample_wide_batches_vs_oracle(ikd, ds):
+    """Batches over a map 1.4 km wide (thousands of voxels per axis) and
+    narrow ones after them, with exact and near duplicates of map points:
+    every batch matches the oracle (0.3 m: the conflict check's neighbour
+    lookups)."""
+    rng = np.random.default_rng(62)
+    base = uniform_cloud(rng, 6000, -700, 700)
+    g, o = _pair(ikd, base, params=(0.5, 0.6, ds))
+    for it, (lo, hi) in enumerate([(-700, 700), (-60, 60), (-700, 700), (-3, 3)]):
+        batch = uniform_cloud(rng, 3000, lo, hi)
+        batch[::10] = base[rng.integers(0, 6000, batch[::10].shape[0])]  # exact duplicates of map points
+        batch[1::10] = batch[::10] + np.float32(0.01)  # and near ones in the same voxels
+        a_g = g.Add_Points(batch, True)
+        a_o = o.Add_Points(batch, True)
+        assert a_g == a_o, f"Add_Points return, batch {it}"
+        assert g.validnum() == o.validnum()
+        assert_same_multiset(g.flatten()[0], o.flatten()[0], f"downsampled live set, batch {it}")
+
+
 def _dense_voxel_cloud(rng):
     """Thousands of points in a few 0.5 m voxels, exact duplicates, and pairs of
     distinct points at the same distance from their voxel's centre (the
