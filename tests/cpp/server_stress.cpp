@@ -1,7 +1,8 @@
 // Server stress on the CPU emulation build with AddressSanitizer: 8 workers,
-// 8 maps with deferred large-subtree rebuilds and chunked idle-map
-// maintenance, many requests in flight at once (kNN + Add_Points + a trim on
-// every third request), maps added while requests run.  Every request must
+// 8 maps with deferred large-subtree rebuilds and chunked maintenance while
+// no request waits, many requests in flight at once (kNN + Add_Points + a trim on
+// every third request; every other request's results in page-locked
+// buffers), maps added while requests run.  Every request must
 // succeed and nothing may touch freed or out-of-bounds memory.  This is the
 // path (worker -> ikd_rebuild_deferred -> rebuild -> build -> scan) of a host
 // segmentation fault once seen under a profiler on the GPU.
@@ -56,9 +57,22 @@ int main() {
             r->downsample = 1;
             r->n_trim = (j % 3 == 2) ? 1 : 0;
             r->trim_boxes = trim;
-            r->out_d2 = (float*)std::malloc(sizeof(float) * n * 5);
-            r->out_ids = (uint32_t*)std::malloc(sizeof(uint32_t) * n * 5);
-            r->out_counts = (int32_t*)std::malloc(sizeof(int32_t) * n);
+            if (j % 2) {
+                // page-locked result buffers, filled by DMA
+                void *d2 = nullptr, *ids = nullptr, *cnt = nullptr;
+                if (ikd_host_alloc(sizeof(float) * n * 5, &d2) != IKD_OK ||
+                    ikd_host_alloc(sizeof(uint32_t) * n * 5, &ids) != IKD_OK ||
+                    ikd_host_alloc(sizeof(int32_t) * n, &cnt) != IKD_OK)
+                    return 9;
+                r->out_d2 = (float*)d2;
+                r->out_ids = (uint32_t*)ids;
+                r->out_counts = (int32_t*)cnt;
+                r->flags = IKD_REQ_PINNED_RESULTS;
+            } else {
+                r->out_d2 = (float*)std::malloc(sizeof(float) * n * 5);
+                r->out_ids = (uint32_t*)std::malloc(sizeof(uint32_t) * n * 5);
+                r->out_counts = (int32_t*)std::malloc(sizeof(int32_t) * n);
+            }
             if (ikd_server_submit(srv, r) != IKD_OK) return 5;
             reqs.push_back(r);
             bufs.push_back(scan);
@@ -94,9 +108,15 @@ int main() {
     }
     ikd_server_destroy(srv);
     for (auto* r : reqs) {
-        std::free(r->out_d2);
-        std::free(r->out_ids);
-        std::free(r->out_counts);
+        if (r->flags & IKD_REQ_PINNED_RESULTS) {
+            ikd_host_free(r->out_d2);
+            ikd_host_free(r->out_ids);
+            ikd_host_free(r->out_counts);
+        } else {
+            std::free(r->out_d2);
+            std::free(r->out_ids);
+            std::free(r->out_counts);
+        }
         std::free(r);
     }
     for (float* b : bufs) std::free(b);
