@@ -125,7 +125,8 @@ def result_buffers(n_max: int, k: int, pinned: bool = False) -> ResultBuffers:
 class Ticket:
     """One submitted request: keeps its buffers alive and exposes the results.
     With outs (arrays from result_buffers, at least n rows and the request's
-    k) the results are written there instead of into new arrays."""
+    k) the results are written there instead of into new arrays; out_* are
+    then views of outs, valid until the caller hands outs to another request."""
 
     def __init__(self, map_index, xyz, k, max_dist, downsample, trim, tag, outs=None):
         self.xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
