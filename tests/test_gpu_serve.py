@@ -109,6 +109,54 @@ def test_bad_requests_rejected(ikd, serving):
     srv.close()
 
 
+def test_partial_and_empty_requests(ikd, serving, scenes):
+    """Requests that skip a stage: an empty scan, an insert without kNN
+    (k = 0), a kNN without insert (downsample -1), a trim alone; with own,
+    caller and page-locked result buffers.  Each matches the same calls run
+    alone through the plain entry points."""
+    sc = scenes.MappingScene(1009)
+    seed = np.concatenate([sc.scan(f) for f in range(2)])
+    trees = []
+    for _ in range(2):
+        t = ikd.KD_TREE(0.5, 0.6, 0.5)
+        t.Build(seed)
+        trees.append(t)
+    srv = serving.Server(slots=2, hbm_budget_bytes=4 << 30)
+    m = srv.add_map(trees[0])
+    s2, s3, s4 = sc.scan(2)[:9000], sc.scan(3)[:7000], sc.scan(4)[:5000]
+    trim = _trim_box(sc, 4, half=40.0)
+    reqs = [  # (points, k, downsample, trim, outs)
+        (np.zeros((0, 3), np.float32), 5, 1, None, None),
+        (s2, 0, 1, None, None),
+        (s3, 5, -1, None, serving.result_buffers(7000, 5, True)),
+        (s3, 5, -1, None, serving.result_buffers(7100, 5, False)),
+        (np.zeros((0, 3), np.float32), 0, -1, trim, None),
+        (s4, 5, 1, None, serving.result_buffers(5000, 5, True)),
+    ]
+    tickets = [srv.submit(m, p, k, 5.0, ds, tr, tag=i, outs=o) for i, (p, k, ds, tr, o) in enumerate(reqs)]
+    srv.drain()
+    srv.close()
+    assert all(t.status == 0 for t in tickets), [t.status for t in tickets]
+    alone = trees[1]
+    for i, (p, k, ds, tr, o) in enumerate(reqs):
+        tk = tickets[i]
+        if k > 0 and p.shape[0] > 0:
+            x, d, idx, c = alone.knn(p, k, 5.0)
+            assert np.array_equal(tk.out_counts, c), f"request {i}: counts"
+            assert np.array_equal(tk.out_d2.view(np.uint32), d.view(np.uint32)), f"request {i}: d2"
+            assert np.array_equal(tk.out_ids, idx), f"request {i}: ids"
+            assert np.array_equal(tk.out_xyz, x), f"request {i}: points"
+        if ds >= 0 and p.shape[0] > 0:
+            assert tk.req.added == alone.Add_Points(p, bool(ds)), f"request {i}: Add_Points return"
+        else:
+            assert tk.req.added == 0
+        if tr is not None:
+            assert tk.req.trimmed == alone.Delete_Point_Boxes(tr), f"request {i}: trim count"
+    a, _ = trees[0].flatten()
+    b, _ = alone.flatten()
+    assert_same_multiset(a, b)
+
+
 def test_budget_counts_resident_maps(ikd, serving, scenes):
     """The admission budget covers the maps' own device memory: with room for
     the resident maps plus one request, requests for four different maps run
