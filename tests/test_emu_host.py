@@ -122,3 +122,24 @@ def test_pcd_map_through_the_facade(emu_lib, tmp_path):
     r = subprocess.run([str(exe), str(tmp_path / "map.pcd")], capture_output=True, text=True, timeout=600, env=env)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-3000:]
     assert r.stdout.startswith("ok")
+
+
+def test_update_and_search_dispatch_budget(emu_lib, tmp_path):
+    """Device operations one call issues (kernels, copies, fills, primitive
+    calls; the emulation build's tally): a kNN call, Add_Points of 8000-point
+    scans with 0.5 m voxels into a 60k-point map with its rebuilds inline, and
+    a sliding-window trim.  These calls are chains of dependent launches, so a
+    launch per subtree or per item where one launch would do shows up here."""
+    exe = tmp_path / "dispatch_budget"
+    env = _build_emu_exe("dispatch_budget.cpp", exe)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=900, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-3000:]
+    counts = {}
+    for line in r.stdout.splitlines():
+        if " " in line:
+            name, n = line.rsplit(" ", 1)
+            counts.setdefault(name, []).append(int(n))
+    assert r.stdout.rstrip().endswith("ok")
+    assert len(counts["knn"]) == 4 and max(counts["knn"]) <= 10, counts
+    assert len(counts["add_points"]) == 4 and max(counts["add_points"]) <= 160, counts
+    assert counts["trim"][0] <= 105, counts
